@@ -1,0 +1,271 @@
+/*
+ * pyprop_amd: MI355X (gfx950) implementation of PyProp's basis-propagation hot path.
+ *
+ * C ABI.  Plain pointers and sizes only; complex numbers are pp_cplx (two doubles,
+ * layout-compatible with C99 double complex and std::complex<double>).  Arrays are
+ * C-ordered with extent 0 the slowest rank, exactly as the reference passes blitz
+ * arrays to its generated Fortran kernels.  Device pointers are HIP device memory on
+ * the current device; every call is stream-ordered on the hipStream_t it receives
+ * (NULL = legacy default stream).  Functions return 0 (PP_OK) or a negative error
+ * code; pp_last_error() gives the message.  Buffers belong to the caller unless a
+ * function says otherwise; library objects (plans, solvers) own their internal
+ * device memory until the matching *_destroy.
+ *
+ * Each declaration cites the reference interface it replaces (paths relative to the
+ * PyProp source tree).
+ */
+#ifndef PYPROP_AMD_H
+#define PYPROP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pp_cplx {
+  double re;
+  double im;
+} pp_cplx;
+
+typedef struct pp_cplxf {
+  float re;
+  float im;
+} pp_cplxf;
+
+typedef void* pp_stream; /* hipStream_t */
+
+enum {
+  PP_OK = 0,
+  PP_ERR_INVALID = -1,
+  PP_ERR_HIP = -2,
+  PP_ERR_ALLOC = -3,
+  PP_ERR_INTERNAL = -4,
+  PP_ERR_NOT_CONVERGED = -5
+};
+
+/* Storage ids of one tensor-potential rank
+ * (core/tensorpotential/tensorpotentialmultiply_generator.py:1336-1346). */
+enum {
+  PP_STORAGE_SIMP = 0,   /* pair list                            generator.py:302-363 */
+  PP_STORAGE_HERM = 1,   /* pair list + conjugated transpose     generator.py:370-398 */
+  PP_STORAGE_DIAG = 2,   /* row == col == potential index        generator.py:403-432 */
+  PP_STORAGE_IDENT = 3,  /* potential index 0, row == col        generator.py:436-462 */
+  PP_STORAGE_BAND = 4,   /* zhbmv 'L' band, lda = k              generator.py:467-618 */
+  PP_STORAGE_BANDNH = 5, /* zgbmv band, lda = 2b+1               generator.py:622-762 */
+  PP_STORAGE_DENSE = 6   /* row-major N x N                      generator.py:1227-1330 */
+};
+
+/* Library-level */
+const char* pp_last_error(void);
+const char* pp_version(void);
+/* Synchronize a stream (convenience for FFI callers without HIP bindings). */
+int pp_stream_synchronize(pp_stream stream);
+
+/* ------------------------------------------------------------------------------
+ * Tensor potential multiply
+ * ------------------------------------------------------------------------------ */
+
+/* One-shot drop-in for the generated kernels
+ *   tensorpotentialmultiply_<ids>_(potential, potentialExtent0.., scaling, source, ...,
+ *                                  dest, ..., pair0, pair0Extent0, pair0Extent1, ...)
+ * (parameter meaning: generator.py:1373-1385 and 1428-1451; C++ wrapper
+ * TensorPotentialMultiply_<ids>_Wrapper, generator.py:1486-1493).
+ *   dest += V * scaling * source  (dest accumulates; the caller zeroes it)
+ * storage_ids[r]: PP_STORAGE_*; pairs[r]: host int32 [pair_counts[r]][2] (row, col)
+ * for SIMP/HERM ranks, ignored otherwise.  potential, source, dest: device.
+ * Builds an internal plan, runs it on `stream`, synchronizes the stream and frees the
+ * plan: for repeated products use pp_tp_create/pp_tp_multiply. */
+int pp_tensor_potential_multiply(int rank, const int* storage_ids,
+                                 const pp_cplx* potential, const int* potential_extents,
+                                 double scaling,
+                                 const pp_cplx* source, const int* source_extents,
+                                 pp_cplx* dest, const int* dest_extents,
+                                 const int32_t* const* pairs, const int* pair_counts,
+                                 pp_stream stream);
+
+/* Prepared multiply plan (replaces TensorPotential.SetupStep's lookup of
+ * core.TensorPotentialMultiply_<ids> plus the PotentialData it multiplies,
+ * pyprop/tensorpotential/TensorPotential.py:80-112).  The potential is copied into a
+ * device layout chosen by the plan (diagonal-major bands, real storage when every
+ * imaginary part is zero); the caller's `potential` buffer may be freed afterwards. */
+typedef struct pp_tensor_potential pp_tensor_potential;
+
+int pp_tp_create(int rank, const int* storage_ids, const pp_cplx* potential,
+                 const int* potential_extents, const int* source_extents,
+                 const int32_t* const* pairs, const int* pair_counts,
+                 pp_stream stream, pp_tensor_potential** out);
+/* dest += scaling * V * source (TensorPotential.MultiplyPotential, TensorPotential.py:92-112) */
+int pp_tp_multiply(const pp_tensor_potential* tp, double scaling, const pp_cplx* source,
+                   pp_cplx* dest, pp_stream stream);
+/* Single-precision variant on complex64 vectors: dest += scaling * V * source, with V
+ * held in a float32 copy made on first use and FP32 accumulation (no reference
+ * counterpart: inner solves of the mixed-precision Cayley step). */
+int pp_tp_multiply_f32(const pp_tensor_potential* tp, float scaling, const pp_cplxf* source,
+                       pp_cplxf* dest, pp_stream stream);
+/* Bytes of potential data the plan streams per multiply (for roofline accounting). */
+int pp_tp_info(const pp_tensor_potential* tp, int64_t* stream_bytes, int64_t* tuple_count,
+               int* is_real, int* inner_kind);
+int pp_tp_destroy(pp_tensor_potential* tp);
+
+/* Set-up helper with no reference counterpart (the reference evaluates every potential
+ * on the quadrature grid and projects it, TensorPotentialGenerator.py:80-274): expand a
+ * separable potential in place on the device,
+ *   out[a * r_count + r] = sum_t coeffs[t * p0 + a] * radial[t * r_count + r]   (real -> complex128)
+ * coeffs, radial, out: device. */
+int pp_expand_potential(int terms, int64_t p0, int64_t r_count, const double* coeffs, const double* radial,
+                        pp_cplx* out, pp_stream stream);
+
+/* ------------------------------------------------------------------------------
+ * Overlap (banded Cholesky of the B-spline overlap)
+ * ------------------------------------------------------------------------------ */
+
+/* Factor the banded overlap of one B-spline rank: s_upper is host (n, sd+1) with
+ * element (i, j), i <= j <= i+sd, at [j][sd + i - j] (BLAS upper Hermitian band as
+ * OverlapMatrix::Setup fills it, core/representation/overlapmatrix.cpp:9-37; the
+ * factorization is zpbtrf('U')).  Real-valued (the B-spline overlap is real). */
+typedef struct pp_overlap pp_overlap;
+int pp_overlap_create(int n, int superdiagonals, const double* s_upper, pp_overlap** out);
+int pp_overlap_get_factor(const pp_overlap* ov, double* u_upper /* host (n, sd+1) */);
+int pp_overlap_destroy(pp_overlap* ov);
+
+/* Apply to every fibre of `rank` of a C-ordered tensor with `ndim` extents, in place
+ * (OverlapMatrix::*Tensor loops, overlapmatrix.cpp:155-229):
+ *   op 0 multiply   x <- U^H U x            (MultiplyOverlapVector, :123-128)
+ *   op 1 solve      x <- U^-1 U^-H x        (SolveOverlapVector, :130-135)
+ *   op 2 sqrt mul   x <- U x  | U^H x       (MultiplySqrtOverlapVector, :137-142)
+ *   op 3 sqrt solve x <- U^-1 x | U^-H x    (SolveSqrtOverlapVector, :144-149)
+ * conj selects the ^H form for ops 2 and 3. */
+enum { PP_OVERLAP_MULTIPLY = 0, PP_OVERLAP_SOLVE = 1, PP_OVERLAP_SQRT_MULTIPLY = 2,
+       PP_OVERLAP_SQRT_SOLVE = 3 };
+int pp_overlap_apply(const pp_overlap* ov, int op, int conj, int ndim, const int* extents,
+                     int rank, pp_cplx* data, pp_stream stream);
+
+/* ------------------------------------------------------------------------------
+ * Combined representation: the wavefunction's per-rank overlaps
+ * (core/representation/combinedrepresentation.cpp:165-238, 288-481)
+ * ------------------------------------------------------------------------------ */
+typedef struct pp_repr pp_repr;
+/* overlaps[r] = NULL for an orthogonal rank (IsOrthogonalBasis, representation.h:189-192) */
+int pp_repr_create(int ndim, const int* extents, pp_overlap* const* overlaps, pp_repr** out);
+int pp_repr_destroy(pp_repr* repr);
+/* MultiplyOverlap / SolveOverlap / MultiplySqrtOverlap / SolveSqrtOverlap over all
+ * non-orthogonal ranks in rank order, in place */
+int pp_repr_overlap(const pp_repr* repr, int op, int conj, pp_cplx* psi, pp_stream stream);
+/* <a|b> = a^H (S x ... x S) b (InnerProduct Algorithm 4, combinedrepresentation.cpp:423-474);
+ * `work` is a caller-provided device vector of the wavefunction size; result on host. */
+int pp_repr_inner_product(const pp_repr* repr, const pp_cplx* a, const pp_cplx* b,
+                          pp_cplx* work, pp_cplx* result, pp_stream stream);
+
+/* ------------------------------------------------------------------------------
+ * Hamiltonian: sum of tensor potentials (BasisPropagator,
+ * pyprop/tensorpotential/BasisPropagator.py:116-159)
+ * ------------------------------------------------------------------------------ */
+typedef struct pp_hamiltonian pp_hamiltonian;
+/* Fuses the given plans into one launch where their storage allows.  The plans stay
+ * owned by the caller and must outlive the Hamiltonian. */
+int pp_ham_create(const pp_repr* repr, int count, pp_tensor_potential* const* potentials,
+                  pp_hamiltonian** out);
+int pp_ham_destroy(pp_hamiltonian* ham);
+/* dest = sum_p scalings[p] V_p src      (MultiplyHamiltonianNoOverlap, :126-129) */
+int pp_ham_multiply_nooverlap(pp_hamiltonian* ham, const double* scalings,
+                              const pp_cplx* src, pp_cplx* dest, pp_stream stream);
+/* dest = S^-1 sum_p scalings[p] V_p src (MultiplyHamiltonian, :116-124) */
+int pp_ham_multiply(pp_hamiltonian* ham, const double* scalings, const pp_cplx* src,
+                    pp_cplx* dest, pp_stream stream);
+/* dest = U^-H (sum_p scalings[p] V_p) U^-1 src, src unchanged
+ * (MultiplyHamiltonianBalancedOverlap, :131-159) */
+int pp_ham_multiply_balanced(pp_hamiltonian* ham, const double* scalings,
+                             const pp_cplx* src, pp_cplx* dest, pp_stream stream);
+
+/* ------------------------------------------------------------------------------
+ * BLAS-1 on device vectors (piram functors / blitzblas,
+ * core/krylov/piram/piram/functors.h:185-241; blitzblas.h:379-487)
+ * Results returned to host; deterministic reduction order.
+ * ------------------------------------------------------------------------------ */
+int pp_vec_dot(int64_t n, const pp_cplx* x, const pp_cplx* y, pp_cplx* result, pp_stream stream); /* sum conj(x) y */
+int pp_vec_norm(int64_t n, const pp_cplx* x, double* result, pp_stream stream);                  /* sqrt(Re <x,x>) */
+
+/* ------------------------------------------------------------------------------
+ * Krylov solvers.  An operator is a function table: out = A in on `stream`.
+ * ------------------------------------------------------------------------------ */
+typedef int (*pp_operator_fn)(void* ctx, const pp_cplx* in, pp_cplx* out, pp_stream stream);
+
+/* GMRES (core/krylov/gmres/gmres.h:28-525; wrapper core/krylov/gmres/gmreswrapper.cpp):
+ * single-cycle, x0 = 0, absolute residual tolerance clamped to >= 1e-13, optional
+ * DGKS re-orthogonalisation (threshold 0.717, at most 5 passes). */
+typedef struct pp_gmres pp_gmres;
+int pp_gmres_create(int64_t n, int basis_size, double tolerance, int double_orthogonalization,
+                    pp_gmres** out);
+int pp_gmres_destroy(pp_gmres* g);
+/* Solve A x = rhs; rhs is preserved; x overwritten (GMRES::SolveVector, gmres.h:433-505). */
+int pp_gmres_solve(pp_gmres* g, pp_operator_fn op, void* ctx, const pp_cplx* rhs, pp_cplx* x,
+                   pp_stream stream);
+/* Error estimate of the last solve, operator applications, Arnoldi steps
+ * (GetErrorEstimate, GetOperatorCount; gmres.h:161-194). */
+int pp_gmres_stats(const pp_gmres* g, double* error_estimate, int* operator_count,
+                   int* orthogonalization_count);
+/* Error estimate per Arnoldi step of the last solve (GetErrorEstimateList); `out`
+ * holds basis_size doubles. */
+int pp_gmres_error_list(const pp_gmres* g, double* out);
+
+/* pAMP (core/krylov/pamp/pamp/pamp.h:28-690; wrapper pampwrapper.cpp): Arnoldi with
+ * classical Gram-Schmidt (+ optional re-orthogonalisation, threshold 0.717, at most 3
+ * passes), scaling-and-squaring Pade exponential of the Hessenberg matrix, all on the
+ * device (no host synchronisation inside a step). */
+typedef struct pp_pamp pp_pamp;
+int pp_pamp_create(int64_t n, int basis_size, int pade_order, int double_orthogonalization,
+                   pp_pamp** out);
+int pp_pamp_destroy(pp_pamp* p);
+/* v <- exp(-i dt A) v (pAMP::PropagateVector, pamp.h:613-670).  Must not be given an
+ * operator that synchronises the stream if the caller wants to capture the step. */
+int pp_pamp_propagate(pp_pamp* p, pp_operator_fn op, void* ctx, pp_cplx* v, pp_cplx dt,
+                      pp_stream stream);
+/* |exp(H)[m-1,0]| of the last step (GetPropagationErrorEstimate) and happy-breakdown
+ * flag; synchronises the stream. */
+int pp_pamp_stats(const pp_pamp* p, pp_cplx* error_estimate, int* breakdown, int* operator_count);
+
+/* ------------------------------------------------------------------------------
+ * Block-diagonal ("radial") preconditioner
+ * (examples/tensor/helium_stabilization/preconditioner.py:77-198)
+ * ------------------------------------------------------------------------------ */
+typedef struct pp_precond pp_precond;
+/* Exact factorization of M_c = scaling_s * (S1 x S2) + scaling_h * P_c for every angular
+ * channel c of a rank-3 wavefunction (nchan, n1, n2).
+ *   s1, s2: host dense (n1, n1), (n2, n2) overlaps; band = half bandwidth of S (k-1).
+ *   p_data: host complex (nchan, p1_count, p2_count) angular-diagonal potential with
+ *   Simp pair lists pairs1 (p1_count, 2) and pairs2 (p2_count, 2) for the radial ranks.
+ * LU without pivoting (the real part of M_c is positive definite); the SuperLU variant
+ * of the reference solves the same systems exactly. */
+int pp_precond_create(int nchan, int n1, int n2, int band, const double* s1, const double* s2,
+                      const pp_cplx* p_data, int p1_count, const int32_t* pairs1, int p2_count,
+                      const int32_t* pairs2, pp_cplx scaling_h, pp_cplx scaling_s,
+                      pp_precond** out);
+int pp_precond_destroy(pp_precond* pc);
+/* psi <- M^-1 psi per channel (Preconditioner.Solve, preconditioner.py:189-198) */
+int pp_precond_solve(const pp_precond* pc, pp_cplx* psi, pp_stream stream);
+
+/* ------------------------------------------------------------------------------
+ * Cayley / Crank-Nicolson step (pyprop/propagator/CayleyPropagator.py:71-150)
+ *   (S + i dt/2 H(t+dt/2)) psi(t+dt) = (S - i dt/2 H(t+dt/2)) psi(t)
+ * left-preconditioned by M (optional), solved by pp_gmres.
+ * ------------------------------------------------------------------------------ */
+typedef struct pp_cayley pp_cayley;
+int pp_cayley_create(pp_hamiltonian* ham, const pp_precond* pc, int basis_size, double tolerance,
+                     int double_orthogonalization, pp_cayley** out);
+int pp_cayley_destroy(pp_cayley* c);
+/* scalings: per-potential time factors at t + |dt|/2 */
+int pp_cayley_step(pp_cayley* c, pp_cplx* psi, double dt, const double* scalings,
+                   pp_stream stream);
+int pp_cayley_stats(const pp_cayley* c, double* error_estimate, int* operator_count);
+
+/* pAMP step in the balanced-overlap form (PamPropagator.AdvanceStep,
+ * pyprop/propagator/PamPropagator.py:50-59): psi <- U^-1 exp(-i dt U^-H H U^-1) U psi */
+int pp_pamp_step_balanced(pp_pamp* p, pp_hamiltonian* ham, pp_cplx* psi, pp_cplx dt,
+                          const double* scalings, pp_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYPROP_AMD_H */

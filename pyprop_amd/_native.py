@@ -1,0 +1,107 @@
+"""ctypes binding of the gfx950 library libpyprop_amd.so (C ABI in include/pyprop_amd.h).
+
+There is no fallback: if the library is missing or no GPU is visible, the product path
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_lib" / "libpyprop_amd.so"
+
+OP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
+
+
+class PPCplx(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+STORAGE = {"Simp": 0, "Herm": 1, "Diag": 2, "Ident": 3, "Band": 4, "BandNH": 5, "Dense": 6}
+OVERLAP_OPS = {"multiply": 0, "solve": 1, "sqrt_multiply": 2, "sqrt_solve": 3}
+
+# (name, restype, argtypes) for every exported symbol of include/pyprop_amd.h
+P = C.c_void_p
+I = C.c_int
+D = C.c_double
+I64 = C.c_int64
+SIGNATURES = [
+    ("pp_last_error", C.c_char_p, []),
+    ("pp_version", C.c_char_p, []),
+    ("pp_stream_synchronize", I, [P]),
+    ("pp_tensor_potential_multiply", I, [I, P, P, P, D, P, P, P, P, P, P, P]),
+    ("pp_tp_create", I, [I, P, P, P, P, P, P, P, P]),
+    ("pp_tp_multiply", I, [P, D, P, P, P]),
+    ("pp_tp_multiply_f32", I, [P, C.c_float, P, P, P]),
+    ("pp_tp_info", I, [P, P, P, P, P]),
+    ("pp_tp_destroy", I, [P]),
+    ("pp_expand_potential", I, [I, I64, I64, P, P, P, P]),
+    ("pp_overlap_create", I, [I, I, P, P]),
+    ("pp_overlap_get_factor", I, [P, P]),
+    ("pp_overlap_destroy", I, [P]),
+    ("pp_overlap_apply", I, [P, I, I, I, P, I, P, P]),
+    ("pp_repr_create", I, [I, P, P, P]),
+    ("pp_repr_destroy", I, [P]),
+    ("pp_repr_overlap", I, [P, I, I, P, P]),
+    ("pp_repr_inner_product", I, [P, P, P, P, P, P]),
+    ("pp_ham_create", I, [P, I, P, P]),
+    ("pp_ham_destroy", I, [P]),
+    ("pp_ham_multiply_nooverlap", I, [P, P, P, P, P]),
+    ("pp_ham_multiply", I, [P, P, P, P, P]),
+    ("pp_ham_multiply_balanced", I, [P, P, P, P, P]),
+    ("pp_vec_dot", I, [I64, P, P, P, P]),
+    ("pp_vec_norm", I, [I64, P, P, P]),
+    ("pp_gmres_create", I, [I64, I, D, I, P]),
+    ("pp_gmres_destroy", I, [P]),
+    ("pp_gmres_solve", I, [P, OP_FN, P, P, P, P]),
+    ("pp_gmres_stats", I, [P, P, P, P]),
+    ("pp_gmres_error_list", I, [P, P]),
+    ("pp_pamp_create", I, [I64, I, I, I, P]),
+    ("pp_pamp_destroy", I, [P]),
+    ("pp_pamp_propagate", I, [P, OP_FN, P, P, PPCplx, P]),
+    ("pp_pamp_stats", I, [P, P, P, P]),
+    ("pp_precond_create", I, [I, I, I, I, P, P, P, I, P, I, P, PPCplx, PPCplx, P]),
+    ("pp_precond_destroy", I, [P]),
+    ("pp_precond_solve", I, [P, P, P]),
+    ("pp_cayley_create", I, [P, P, I, D, I, P]),
+    ("pp_cayley_destroy", I, [P]),
+    ("pp_cayley_step", I, [P, P, D, P, P]),
+    ("pp_cayley_stats", I, [P, P, P]),
+    ("pp_pamp_step_balanced", I, [P, P, P, PPCplx, P, P]),
+]
+
+_lib = None
+
+
+def load(path: Path | None = None):
+    """Load (once) and return the native library; raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise NativeError("%s is missing: build it with `python -m pyprop_amd.build` "
+                          "(there is no CPU fallback)" % p)
+    lib = C.CDLL(str(p))
+    for name, res, args in SIGNATURES:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().pp_last_error().decode()
+        raise NativeError("pyprop_amd error %d: %s" % (rc, msg))
+
+
+def cplx(z) -> PPCplx:
+    z = complex(z)
+    return PPCplx(z.real, z.imag)
