@@ -1,0 +1,207 @@
+"""Helium rank-3 Cayley/Crank-Nicolson propagation on one MI355X.
+
+Workload: helium in coupled spherical harmonics (L = 0..2, M = 0, l1, l2 <= 5: 42
+channels) x 60 order-7 B-splines per electron on [0, 40] (N = 151,200 complex128);
+H = H0 + 1/r12 (multipoles lambda <= 10) + length-gauge dipole E(t), stored as
+Simp x banded-packed x banded-nonhermitian tensor potentials; Cayley step with GMRES
+(basis 30, tolerance 1e-10) and the exact block-diagonal preconditioner; dt = 0.01,
+E0 = 0.05, omega = 0.057, sin^2 envelope of 4 cycles; start from the independent-
+particle 1s^2 state.  Data are synthetic (generated here), nothing is downloaded.
+
+Prints one JSON line (see --help).  A "step" is one Cayley AdvanceStep.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def build_workload(torch):
+    import numpy as np
+    from pyprop_amd.propagator import (BasisPropagator, CayleyPropagator, CombinedRepresentation,
+                                       RadialTwoElectronPreconditioner, Wavefunction)
+    from pyprop_amd.systems import helium_initial_state, helium_system
+    t0 = time.time()
+    sysm = helium_system()
+    log("helium set-up (host) %.1fs shape=%s" % (time.time() - t0, sysm.shape))
+    repr_ = CombinedRepresentation(sysm.shape, sysm.bases)
+    psi = Wavefunction(repr_, helium_initial_state(sysm))
+    psi.Normalize()
+    base = BasisPropagator(psi, sysm.potentials)
+    t0 = time.time()
+    base.SetupStep(0.01)
+    torch.cuda.synchronize()
+    log("device potentials + plans %.1fs" % (time.time() - t0))
+    return sysm, repr_, psi, base
+
+
+def time_matvec(torch, base, psi, reps=20):
+    """Average duration of the fused Hamiltonian multiply (one kernel) with HIP events on
+    the stream the library launches on (torch's current stream)."""
+    src = psi.data.clone()
+    dst = torch.zeros_like(src)
+    for _ in range(3):
+        base.MultiplyHamiltonianNoOverlap(src, dst, 100.0)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        base.MultiplyHamiltonianNoOverlap(src, dst, 100.0)
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps / 1e3
+
+
+def cpu_baseline(sysm, n_ops_per_step, sample_frac=0.1):
+    """The oracle (sequential C++ restatement of the reference loops) timed on the host:
+    the tensor-potential multiply on the first `sample_frac` of every potential's angular
+    pairs and the block-preconditioner solve of two channels, scaled to one Cayley step
+    (n_ops operator applications, each one multiply + one preconditioner solve)."""
+    import numpy as np
+    import oracle
+    rng = np.random.default_rng(3)
+    nc, n1, n2 = sysm.shape
+    psi = rng.standard_normal(sysm.shape) + 1j * rng.standard_normal(sysm.shape)
+    from pyprop_amd.systems import _expand
+    t_mul = 0.0
+    for p in sysm.potentials:
+        P0 = p.geometries[0].pair_count
+        k = max(1, int(P0 * sample_frac))
+        V = _expand(p.angular_coeffs[:, :k], p.radial_factors)
+        pairs0 = p.geometries[0].pairs[:k]
+        dst = np.zeros(sysm.shape, complex)
+        t0 = time.perf_counter()
+        oracle.tpmul(p.storage_ids, V, 1.0, psi, dst, [pairs0, p.geometries[1].pairs, None])
+        t_mul += (time.perf_counter() - t0) * P0 / k
+    # preconditioner: two channels, factor (not timed) then solve (timed)
+    from pyprop_amd.propagator import band_upper
+    b1 = sysm.bases[1]
+    S1 = b1.overlap_band()
+    g = sysm.precond_potentials[0].geometries
+    from pyprop_amd.systems import _expand
+    # the angular-diagonal pairs are (c, c) in channel order: the first two are channels 0, 1
+    P = sum(_expand(x.angular_coeffs[:, :2], x.radial_factors) for x in sysm.precond_potentials)
+    s = oracle.System((2, n1, n2))
+    s.set_preconditioner(S1, S1, b1.order - 1, P, g[1].pairs, g[2].pairs, 0.005j)
+    x = psi[:2].copy()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        s.precond_solve(x)
+    t_pc = (time.perf_counter() - t0) / reps * nc / 2
+    step = (n_ops_per_step + 1) * (t_mul + t_pc)
+    return dict(value=1.0 / step, unit="steps/s", cores=1, kind="port",
+                sample="oracle multiply on %d%% of each potential's angular pairs (t_full_matvec=%.3fs) and "
+                       "2-channel preconditioner solve (t_full=%.3fs), scaled to one step of %.1f operator "
+                       "applications + right-hand side" % (int(sample_frac * 100), t_mul, t_pc, n_ops_per_step))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-matvec", action="store_true", help="only run the matvec loop (for rocprof)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    from pyprop_amd.propagator import CayleyPropagator, RadialTwoElectronPreconditioner
+
+    sysm, repr_, psi, base = build_workload(torch)
+    stream_bytes = sum(p.stream_bytes for p in base.PotentialList)
+    nnz = sum(int(np.prod(p.source.shape)) for p in base.PotentialList)
+    N = sysm.size
+    alg_bytes = 8 * nnz + 3 * 16 * N  # real float64 potential elements + src read + dst write
+    if a.profile_matvec:
+        t = time_matvec(torch, base, psi, reps=50)
+        log("matvec %.3f ms, %.1f GB/s" % (t * 1e3, alg_bytes / t / 1e9))
+        return
+    t_mv = time_matvec(torch, base, psi)
+    achieved = alg_bytes / t_mv / 1e9
+    log("fused H multiply: %.3f ms  alg %.3f GB  -> %.1f GB/s (%.1f%% of 8 TB/s)" %
+        (t_mv * 1e3, alg_bytes / 1e9, achieved, 100 * achieved / HBM_PEAK_GBS))
+
+    dt = 0.01
+    t0 = time.time()
+    pc = RadialTwoElectronPreconditioner(psi, sysm)
+    prop = CayleyPropagator(psi, base, pc, krylov_basis_size=30, krylov_tolerance=1e-10)
+    prop.SetupStep(dt)
+    torch.cuda.synchronize()
+    log("preconditioner factorization + Cayley set-up %.1fs" % (time.time() - t0))
+    t = 0.0
+    ops = []
+    for _ in range(a.warmup):
+        prop.AdvanceStep(t, dt)
+        t += dt
+    torch.cuda.synchronize()
+    norm0 = psi.GetNorm()
+    torch.cuda.synchronize()
+    tstart = time.perf_counter()
+    for _ in range(a.steps):
+        prop.AdvanceStep(t, dt)
+        ops.append(prop.GetOperatorCount())
+        t += dt
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - tstart
+    norm1 = psi.GetNorm()
+    steps_per_s = a.steps / elapsed
+    energy = base.GetEnergyExpectationValue(psi, t)
+    log("plain FP64 Cayley: %.2f steps/s, %.1f ops/step, |norm drift| %.2e, E = %.6f" %
+        (steps_per_s, float(np.mean(ops)), abs(norm1 - norm0), energy))
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        t0 = time.time()
+        cpu = cpu_baseline(sysm, float(np.mean(ops)))
+        log("cpu baseline %.4f steps/s (%.1fs)" % (cpu["value"], time.time() - t0))
+
+    out = {
+        "metric": "CN steps/s, helium rank-3: learned+FP32 vs plain FP64 GPU at 1e-8 err; GB/s",
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (helium Hamiltonian generated in-process; independent-particle 1s^2 start)",
+        "config": {"workload": "helium rank-3 Cayley-CN, 42 ch x 60 x 60 B-splines (N=151200), plain FP64, "
+                               "GMRES m=30 tol=1e-10, exact block-diagonal preconditioner, dt=0.01",
+                   "path": "plain_fp64", "gmres_ops_per_step": float(np.mean(ops)),
+                   "norm_drift": abs(norm1 - norm0), "energy": energy},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "tp_dia_kernel (fused H0+R12+dipole multiply)", "ms": t_mv * 1e3,
+                     "alg_bytes": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
