@@ -119,6 +119,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-matvec", action="store_true", help="only run the matvec loop (for rocprof)")
+    ap.add_argument("--profile-cayley", type=int, default=0, help="only run this many Cayley steps (for rocprof)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -137,6 +138,18 @@ def main():
     if a.profile_matvec:
         t = time_matvec(torch, base, psi, reps=50)
         log("matvec %.3f ms, %.1f GB/s" % (t * 1e3, alg_bytes / t / 1e9))
+        return
+    if a.profile_cayley:
+        pc = RadialTwoElectronPreconditioner(psi, sysm)
+        prop = CayleyPropagator(psi, base, pc, krylov_basis_size=30, krylov_tolerance=1e-10)
+        prop.SetupStep(0.01)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(a.profile_cayley):
+            prop.AdvanceStep(0.01 * k, 0.01)
+        torch.cuda.synchronize()
+        log("%d Cayley steps %.2f ms/step, %d ops" % (a.profile_cayley, 1e3 * (time.perf_counter() - t0) /
+                                                      a.profile_cayley, prop.GetOperatorCount()))
         return
     t_mv = time_matvec(torch, base, psi)
     achieved = alg_bytes / t_mv / 1e9

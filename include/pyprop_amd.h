@@ -264,6 +264,96 @@ int pp_cayley_stats(const pp_cayley* c, double* error_estimate, int* operator_co
 int pp_pamp_step_balanced(pp_pamp* p, pp_hamiltonian* ham, pp_cplx* psi, pp_cplx dt,
                           const double* scalings, pp_stream stream);
 
+/* ------------------------------------------------------------------------------
+ * Kronecker (separable) preconditioner.  No reference counterpart: a replacement for
+ * the block-diagonal RadialTwoElectronPreconditioner
+ * (examples/tensor/helium_stabilization/preconditioner.py:101-198) that inverts the
+ * separable part of each channel block exactly,
+ *   K_c = S (x) S + a (h_l1 (x) S + S (x) h_l2),
+ * from the generalized eigenpairs h_l E_l = S E_l diag(lam_l), E_l^T S E_l = I:
+ *   K_c^-1 B = E_l1 [(E_l1^T B E_l2) / (1 + a (lam_l1,i + lam_l2,j))] E_l2^T.
+ * S: host [n][n] radial overlap, E: host [nl][n][n] row-major (column j = eigenvector
+ * j), lam: host [nl][n], l1, l2: host [nchan] (the l index of each electron in channel
+ * c).  psi: [nchan][n][n].
+ * ------------------------------------------------------------------------------ */
+typedef struct pp_kron pp_kron;
+int pp_kron_create(int nchan, int n, int nl, const double* S, const double* E, const double* lam,
+                   const int* l1, const int* l2, pp_kron** out);
+int pp_kron_destroy(pp_kron* k);
+/* psi <- K^-1 psi (complex128 / complex64) */
+int pp_kron_solve(const pp_kron* k, pp_cplx a, pp_cplx* psi, pp_stream stream);
+/* psi <- K psi (complex128) */
+int pp_kron_multiply(const pp_kron* k, pp_cplx a, pp_cplx* psi, pp_stream stream);
+int pp_kron_solve_f32(const pp_kron* k, pp_cplx a, pp_cplxf* psi, pp_stream stream);
+/* use K (a = i dt/2) as the left preconditioner of a Cayley solver created without a
+ * block preconditioner (NULL detaches) */
+int pp_cayley_set_kron(pp_cayley* c, const pp_kron* k);
+
+/* ------------------------------------------------------------------------------
+ * Learned preconditioner.  No reference counterpart.  An MLP shared by all channel
+ * blocks (in = 2 n^2 reals, hidden width `hidden`, GELU, per-channel embedding added to
+ * the first hidden layer), BF16 MFMA products with FP32 accumulation, FP32 master
+ * weights trained by Adam on the device.  As a preconditioner it applies
+ * K_c^-1 (v_c + sigma |v| net(v_c / |v|)), |v| the norm of the whole vector.
+ * Parameter vector layout (FP32): W1 [hidden][in], b1 [hidden], emb [nchan][hidden],
+ * W2 [hidden][hidden], b2 [hidden], W3 [in][hidden], b3 [in].
+ * x, t, y: device FP32 [rows][in] (block c as n*n interleaved re, im); chan: device
+ * int32 [rows].
+ * ------------------------------------------------------------------------------ */
+typedef struct pp_learned pp_learned;
+int pp_learned_create(int nchan, int n, int hidden, unsigned long long seed, pp_learned** out);
+int pp_learned_destroy(pp_learned* l);
+int pp_learned_param_count(const pp_learned* l, long long* count);
+int pp_learned_get_params(const pp_learned* l, float* host);
+int pp_learned_set_params(pp_learned* l, const float* host);
+int pp_learned_get_grads(const pp_learned* l, float* host);
+int pp_learned_set_scale(pp_learned* l, float sigma);
+/* y = net(x) (before sigma) */
+int pp_learned_forward(pp_learned* l, int rows, const float* x, const int* chan, float* y,
+                       pp_stream stream);
+/* gradient of loss = sum |net(x) - t / sigma|^2 / rows into the parameter gradient;
+ * loss (host, may be NULL) */
+int pp_learned_gradient(pp_learned* l, int rows, const float* x, const int* chan, const float* t,
+                        double* loss, pp_stream stream);
+/* gradient + one Adam step (beta 0.9, 0.999, eps 1e-8) */
+int pp_learned_train_step(pp_learned* l, int rows, const float* x, const int* chan, const float* t,
+                          float lr, double* loss, pp_stream stream);
+/* training rows from one solved system A d = r (complex128 psi-shaped r, d): per channel c
+ * x_c = r_c / |r|, t_c = (K_c d_c - r_c) / |r|, nrm_c = |r_c| / |r| (device FP32 outputs,
+ * x and t [nchan][in], nrm [nchan]) */
+int pp_learned_prepare(pp_learned* l, const pp_kron* k, double dt, const pp_cplx* r, const pp_cplx* d,
+                       float* x_out, float* t_out, float* nrm_out, pp_stream stream);
+/* out = K^-1 (in + sigma |in| net(in_c / |in|)) (complex64, psi-shaped) */
+int pp_learned_apply(pp_learned* l, const pp_kron* k, double dt, const pp_cplxf* in, pp_cplxf* out,
+                     pp_stream stream);
+/* C[M][N] = sum_k A(m,k) B(k,n), A(m,k) = a[m*sam + k*sak], B(k,n) = b[k*sbk + n*sbn]
+ * (BF16 bit patterns, FP32 result, MFMA) */
+int pp_gemm_bf16(int M, int N, int K, const uint16_t* a, long long sam, long long sak,
+                 const uint16_t* b, long long sbk, long long sbn, float* c, pp_stream stream);
+
+/* ------------------------------------------------------------------------------
+ * Mixed-precision Cayley step.  No reference counterpart; solves the system of
+ * pp_cayley_step (CayleyPropagator.py:71-150) by FP64 iterative refinement around a
+ * complex64 flexible GMRES with the Kronecker preconditioner (plus the learned
+ * correction on the first direction when attached):
+ *   stop when |b - A x| <= tol_rel |b| (FP64 residual); inner solves stop at
+ *   inner_rtol relative to their right-hand side.
+ * With a fallback solver attached, a step whose inner iterations exceed
+ * fallback_iterations (or that does not converge in max_outer corrections) is redone
+ * by the plain solver from the unchanged psi.
+ * counts (7 ints): outer corrections, inner iterations, FP64 multiplies, FP32
+ * multiplies, preconditioner applications, learned applications, fallback flag;
+ * residuals (2 doubles): |b - A x|, |b - A x| / |b|.
+ * ------------------------------------------------------------------------------ */
+typedef struct pp_mixed pp_mixed;
+int pp_mixed_create(pp_hamiltonian* ham, const pp_kron* k, int basis_size, double tol_rel,
+                    double inner_rtol, int max_outer, pp_mixed** out);
+int pp_mixed_destroy(pp_mixed* m);
+int pp_mixed_set_learned(pp_mixed* m, pp_learned* l);
+int pp_mixed_set_fallback(pp_mixed* m, pp_cayley* plain, int fallback_iterations);
+int pp_mixed_step(pp_mixed* m, pp_cplx* psi, double dt, const double* scalings, pp_stream stream);
+int pp_mixed_stats(const pp_mixed* m, int* counts, double* residuals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,31 @@ SIGNATURES = [
     ("pp_cayley_step", I, [P, P, D, P, P]),
     ("pp_cayley_stats", I, [P, P, P]),
     ("pp_pamp_step_balanced", I, [P, P, P, PPCplx, P, P]),
+    ("pp_kron_create", I, [I, I, I, P, P, P, P, P, P]),
+    ("pp_kron_multiply", I, [P, PPCplx, P, P]),
+    ("pp_kron_destroy", I, [P]),
+    ("pp_kron_solve", I, [P, PPCplx, P, P]),
+    ("pp_kron_solve_f32", I, [P, PPCplx, P, P]),
+    ("pp_cayley_set_kron", I, [P, P]),
+    ("pp_learned_create", I, [I, I, I, C.c_ulonglong, P]),
+    ("pp_learned_destroy", I, [P]),
+    ("pp_learned_param_count", I, [P, P]),
+    ("pp_learned_get_params", I, [P, P]),
+    ("pp_learned_set_params", I, [P, P]),
+    ("pp_learned_get_grads", I, [P, P]),
+    ("pp_learned_set_scale", I, [P, C.c_float]),
+    ("pp_learned_forward", I, [P, I, P, P, P, P]),
+    ("pp_learned_gradient", I, [P, I, P, P, P, P, P]),
+    ("pp_learned_train_step", I, [P, I, P, P, P, C.c_float, P, P]),
+    ("pp_learned_prepare", I, [P, P, D, P, P, P, P, P, P]),
+    ("pp_learned_apply", I, [P, P, D, P, P, P]),
+    ("pp_gemm_bf16", I, [I, I, I, P, I64, I64, P, I64, I64, P, P]),
+    ("pp_mixed_create", I, [P, P, I, D, D, I, P]),
+    ("pp_mixed_destroy", I, [P]),
+    ("pp_mixed_set_learned", I, [P, P]),
+    ("pp_mixed_set_fallback", I, [P, P, I]),
+    ("pp_mixed_step", I, [P, P, D, P, P]),
+    ("pp_mixed_stats", I, [P, P, P]),
 ]
 
 _lib = None

@@ -1,0 +1,778 @@
+// Mixed-precision Cayley step: complex64 inner solves (FP32 vectors and arithmetic, FP64
+// accumulation of inner products), FP64 outer iterative refinement.
+//
+// No reference counterpart; it solves the system of
+// pyprop/propagator/CayleyPropagator.py:71-150,
+//   (S + i dt/2 H(t+dt/2)) psi(t+dt) = (S - i dt/2 H(t+dt/2)) psi(t),
+// by
+//   b = S psi - i a H psi,  x = psi,  r = b - A x = -2 i a H psi        (a = dt/2, one FP64 multiply)
+//   repeat:  stop if |r| <= tol_rel |b|
+//            d ~= A^-1 r by flexible GMRES in FP32, right preconditioner P_j
+//            x += d;  r = b - A x                                       (FP64)
+// Flexible GMRES stores the preconditioned directions z_j, so P_j may change from one
+// iteration to the next and may be nonlinear (the learned correction is used on the first
+// direction only).
+//
+// Kronecker preconditioner: the exact inverse of the separable part of each channel block,
+//   K_c = S x S + a (h_l1 x S + S x h_l2),  h_l E_l = S E_l diag(lam_l),  E_l^T S E_l = I
+//   K_c^-1 B = E_l1 [ (E_l1^T B E_l2) / (1 + a (lam_l1,i + lam_l2,j)) ] E_l2^T,
+// four n x n products per channel, one workgroup per channel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "pp_common.h"
+#include "pp_internal.h"
+#include "pp_krylov.h"
+#include "pp_mixed.h"
+#include "pp_learned.h"
+
+namespace pp {
+
+static int grid_n(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// ------------------------------------------------------------------------------------
+// FP32 vector kernels (FP64 accumulation in reductions)
+// ------------------------------------------------------------------------------------
+__global__ void f64_to_f32_kernel(long long n, const double2* __restrict__ x, float2* __restrict__ y) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = make_float2((float)x[i].x, (float)x[i].y);
+}
+
+// y += x (FP32 into FP64)
+__global__ void add_f32_to_f64_kernel(long long n, const float2* __restrict__ x, double2* __restrict__ y) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    double2 v = y[i];
+    v.x += (double)x[i].x;
+    v.y += (double)x[i].y;
+    y[i] = v;
+  }
+}
+
+// r = b - r (r holds A x on entry), with |r|^2 block partials
+__global__ __launch_bounds__(256) void residual_kernel(long long n, const double2* __restrict__ b,
+                                                       double2* __restrict__ r, double2* __restrict__ partial) {
+  __shared__ double sh[4];
+  double acc = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double2 v = make_double2(b[i].x - r[i].x, b[i].y - r[i].y);
+    r[i] = v;
+    acc = fma(v.x, v.x, acc);
+    acc = fma(v.y, v.y, acc);
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sh[0] + sh[1] + sh[2] + sh[3], 0.0);
+}
+
+// partial[b][j] over blocks -> out[j]
+__global__ void finish64_kernel(const double2* __restrict__ partial, int blocks, int width, int count,
+                                double2* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  double2 s = make_double2(0, 0);
+  for (int b = 0; b < blocks; ++b) {
+    s.x += partial[(long long)b * width + j].x;
+    s.y += partial[(long long)b * width + j].y;
+  }
+  out[j] = s;
+}
+
+// partial[b][j] = sum over the block's slice of conj(V_j) w (j < m); partial[b][m] = |w|^2
+__global__ __launch_bounds__(256) void block_dot32_kernel(long long n, int m, const float2* __restrict__ V, long long ld,
+                                                          const float2* __restrict__ w, double2* __restrict__ partial,
+                                                          int width) {
+  __shared__ double2 sh[4];
+  const long long chunk = (n + gridDim.x - 1) / gridDim.x;
+  const long long beg = chunk * blockIdx.x;
+  const long long end = min(n, beg + chunk);
+  for (int j0 = 0; j0 <= m; j0 += 8) {
+    double2 acc[8];
+#pragma unroll
+    for (int g = 0; g < 8; ++g) acc[g] = make_double2(0, 0);
+    for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
+      const float2 wv = w[i];
+#pragma unroll
+      for (int g = 0; g < 8; ++g) {
+        const int j = j0 + g;
+        if (j < m) {
+          const float2 v = V[j * ld + i];
+          acc[g].x += (double)v.x * wv.x + (double)v.y * wv.y;
+          acc[g].y += (double)v.x * wv.y - (double)v.y * wv.x;
+        } else if (j == m) {
+          acc[g].x += (double)wv.x * wv.x + (double)wv.y * wv.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      const int j = j0 + g;
+      if (j > m) break;  // uniform over the block
+      double2 v = acc[g];
+      for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_down(v.x, off, 64);
+        v.y += __shfl_down(v.y, off, 64);
+      }
+      __syncthreads();
+      if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double2 s = sh[0];
+        for (int q = 1; q < 4; ++q) {
+          s.x += sh[q].x;
+          s.y += sh[q].y;
+        }
+        partial[(long long)blockIdx.x * width + j] = s;
+      }
+    }
+  }
+}
+
+// w -= sum_j V_j h_j (coefficients FP64 on the device, applied in FP32); |w|^2 block partials
+__global__ __launch_bounds__(256) void block_update32_kernel(long long n, int m, const float2* __restrict__ V,
+                                                             long long ld, const double2* __restrict__ h,
+                                                             float2* __restrict__ w, double2* __restrict__ partial) {
+  __shared__ float2 hs[128];
+  __shared__ double sh[4];
+  for (int j = threadIdx.x; j < m; j += blockDim.x) hs[j] = make_float2((float)h[j].x, (float)h[j].y);
+  __syncthreads();
+  const long long chunk = (n + gridDim.x - 1) / gridDim.x;
+  const long long beg = chunk * blockIdx.x;
+  const long long end = min(n, beg + chunk);
+  double acc = 0;
+  for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
+    float2 t = make_float2(0.f, 0.f);
+    for (int j = 0; j < m; ++j) {
+      const float2 v = V[j * ld + i];
+      t.x = fmaf(v.x, hs[j].x, t.x);
+      t.x = fmaf(-v.y, hs[j].y, t.x);
+      t.y = fmaf(v.x, hs[j].y, t.y);
+      t.y = fmaf(v.y, hs[j].x, t.y);
+    }
+    float2 wv = w[i];
+    wv.x -= t.x;
+    wv.y -= t.y;
+    w[i] = wv;
+    acc += (double)wv.x * wv.x + (double)wv.y * wv.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sh[0] + sh[1] + sh[2] + sh[3], 0.0);
+}
+
+__global__ void scale32_kernel(long long n, float s, const float2* __restrict__ x, float2* __restrict__ y) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = make_float2(s * x[i].x, s * x[i].y);
+}
+
+// out = sum_j c_j Z_j (FP32 vectors, coefficients FP64 on the device)
+__global__ void lincomb32_kernel(long long n, int m, const float2* __restrict__ Z, long long ld,
+                                 const double2* __restrict__ c, float2* __restrict__ out) {
+  __shared__ float2 cs[128];
+  for (int j = threadIdx.x; j < m; j += blockDim.x) cs[j] = make_float2((float)c[j].x, (float)c[j].y);
+  __syncthreads();
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    float2 t = make_float2(0.f, 0.f);
+    for (int j = 0; j < m; ++j) {
+      const float2 v = Z[j * ld + i];
+      t.x = fmaf(v.x, cs[j].x, t.x);
+      t.x = fmaf(-v.y, cs[j].y, t.x);
+      t.y = fmaf(v.x, cs[j].y, t.y);
+      t.y = fmaf(v.y, cs[j].x, t.y);
+    }
+    out[i] = t;
+  }
+}
+
+// One FP32 pass of the banded overlap factor along one rank, out-of-place:
+// MODE 0: y = U x, MODE 1: y = U^T x (U real upper band, [j][sd + i - j]).
+template <int SD, int MODE>
+__global__ __launch_bounds__(256) void overlap_elem32_kernel(const float* __restrict__ u, int n, long long post,
+                                                             long long total, const float2* __restrict__ x,
+                                                             float2* __restrict__ y) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int i = (int)((idx / post) % n);
+  const long long base = idx - (long long)i * post;
+  float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int d = 0; d <= SD; ++d) {
+    const int j = MODE == 0 ? i + d : i - d;
+    if (j >= 0 && j < n) {
+      const float a = MODE == 0 ? u[j * (SD + 1) + SD - d] : u[i * (SD + 1) + SD - d];
+      const float2 v = x[base + (long long)j * post];
+      acc.x = fmaf(a, v.x, acc.x);
+      acc.y = fmaf(a, v.y, acc.y);
+    }
+  }
+  y[idx] = acc;
+}
+
+void MixedWork::overlap_multiply32(const Repr& r, const float2* in, float2* out, float2* scratch, hipStream_t s) {
+  const float2* cur = in;
+  bool any = false;
+  for (int rank = 0; rank < r.ndim; ++rank) {
+    const OverlapFactor* ov = r.ov[rank];
+    if (!ov) continue;
+    if (u32[rank].count == 0) {
+      std::vector<float> h(ov->u_host.begin(), ov->u_host.end());
+      u32[rank].upload(h.data(), h.size(), s);
+    }
+    long long post = 1;
+    for (int q = rank + 1; q < r.ndim; ++q) post *= r.extents[q];
+    const long long total = n;
+    const int eb = (int)ceil_div(total, 256);
+    auto pass = [&](int mode, const float2* x, float2* y) {
+      switch (ov->sd * 2 + mode) {
+#define PP_EL(K)                                                                                                      \
+  case 2 * K:                                                                                                         \
+    hipLaunchKernelGGL((overlap_elem32_kernel<K, 0>), dim3(eb), dim3(256), 0, s, u32[rank].ptr, ov->n, post, total, x, y); \
+    break;                                                                                                            \
+  case 2 * K + 1:                                                                                                     \
+    hipLaunchKernelGGL((overlap_elem32_kernel<K, 1>), dim3(eb), dim3(256), 0, s, u32[rank].ptr, ov->n, post, total, x, y); \
+    break;
+        PP_EL(1) PP_EL(2) PP_EL(3) PP_EL(4) PP_EL(5) PP_EL(6) PP_EL(7) PP_EL(8)
+#undef PP_EL
+        default:
+          throw Error(PP_ERR_INVALID, "FP32 overlap supports 1..8 superdiagonals");
+      }
+    };
+    pass(0, cur, scratch);
+    pass(1, scratch, out);
+    cur = out;
+    any = true;
+  }
+  if (!any && in != out) PP_HIP(hipMemcpyAsync(out, in, n * sizeof(float2), hipMemcpyDeviceToDevice, s));
+  PP_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------
+// Kronecker preconditioner
+// ------------------------------------------------------------------------------------
+// One stage of K_c^-1 on an n x n complex block held in LDS, 2 x 2 outputs per thread:
+//   LEFT:  out(i, j) = sum_k m(i, k) in(k, j)
+//   RIGHT: out(i, j) = sum_k in(i, k) m(k, j)
+// m(p, q) = M[p*n + q], or M[q*n + p] when TRANS.  SCALE 1 divides by, SCALE 2 multiplies
+// by, d_ij = 1 + a (lam1_i + lam2_j).
+template <class R, class C, bool LEFT, bool TRANS, int SCALE>
+__device__ inline void kron_stage(const R* __restrict__ M, const C* in, C* out, int n, const R* lam1,
+                                  const R* lam2, C a) {
+  const int h = (n + 1) >> 1;
+  for (int t = threadIdx.x; t < h * h; t += blockDim.x) {
+    const int i0 = (t / h) * 2, j0 = (t % h) * 2;
+    const bool i1ok = i0 + 1 < n, j1ok = j0 + 1 < n;
+    C acc[2][2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        acc[p][q].x = 0;
+        acc[p][q].y = 0;
+      }
+    for (int k = 0; k < n; ++k) {
+      if (LEFT) {
+        // m(i, k) for i = i0, i0+1 ; in(k, j) for j = j0, j0+1
+        const R m0 = TRANS ? M[k * n + i0] : M[i0 * n + k];
+        const R m1 = i1ok ? (TRANS ? M[k * n + i0 + 1] : M[(i0 + 1) * n + k]) : R(0);
+        const C v0 = in[k * n + j0];
+        const C v1 = j1ok ? in[k * n + j0 + 1] : v0;
+        acc[0][0].x = fma(m0, v0.x, acc[0][0].x); acc[0][0].y = fma(m0, v0.y, acc[0][0].y);
+        acc[0][1].x = fma(m0, v1.x, acc[0][1].x); acc[0][1].y = fma(m0, v1.y, acc[0][1].y);
+        acc[1][0].x = fma(m1, v0.x, acc[1][0].x); acc[1][0].y = fma(m1, v0.y, acc[1][0].y);
+        acc[1][1].x = fma(m1, v1.x, acc[1][1].x); acc[1][1].y = fma(m1, v1.y, acc[1][1].y);
+      } else {
+        // in(i, k) for i = i0, i0+1 ; m(k, j) for j = j0, j0+1
+        const C v0 = in[i0 * n + k];
+        const C v1 = i1ok ? in[(i0 + 1) * n + k] : v0;
+        const R m0 = TRANS ? M[j0 * n + k] : M[k * n + j0];
+        const R m1 = j1ok ? (TRANS ? M[(j0 + 1) * n + k] : M[k * n + j0 + 1]) : R(0);
+        acc[0][0].x = fma(m0, v0.x, acc[0][0].x); acc[0][0].y = fma(m0, v0.y, acc[0][0].y);
+        acc[0][1].x = fma(m1, v0.x, acc[0][1].x); acc[0][1].y = fma(m1, v0.y, acc[0][1].y);
+        acc[1][0].x = fma(m0, v1.x, acc[1][0].x); acc[1][0].y = fma(m0, v1.y, acc[1][0].y);
+        acc[1][1].x = fma(m1, v1.x, acc[1][1].x); acc[1][1].y = fma(m1, v1.y, acc[1][1].y);
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int i = i0 + p, j = j0 + q;
+        if (i >= n || j >= n) continue;
+        C v = acc[p][q];
+        if (SCALE == 1) {
+          const R sl = lam1[i] + lam2[j];
+          const R dr = R(1) + a.x * sl, di = a.y * sl;
+          const R inv = R(1) / (dr * dr + di * di);
+          C qv;
+          qv.x = (v.x * dr + v.y * di) * inv;
+          qv.y = (v.y * dr - v.x * di) * inv;
+          v = qv;
+        } else if (SCALE == 2) {
+          const R sl = lam1[i] + lam2[j];
+          const R dr = R(1) + a.x * sl, di = a.y * sl;
+          C qv;
+          qv.x = v.x * dr - v.y * di;
+          qv.y = v.x * di + v.y * dr;
+          v = qv;
+        }
+        out[i * n + j] = v;
+      }
+  }
+}
+
+// One workgroup per channel c.  E_IN_LDS stages both eigenvector matrices in LDS (FP32);
+// the FP64 variant reads them through the cache to stay inside 160 KiB of LDS.
+// FWD applies K_c itself: with F_l = S E_l (= E_l^-T),  K_c B = F_l1 [(F_l1^T B F_l2) d] F_l2^T,
+// E then holds F.
+template <class R, class C, bool E_IN_LDS, bool FWD>
+__global__ __launch_bounds__(1024) void kron_kernel(const R* __restrict__ E, const R* __restrict__ lam,
+                                                    const int* __restrict__ l1, const int* __restrict__ l2, int n,
+                                                    C a, C* __restrict__ psi) {
+  extern __shared__ unsigned char smem[];
+  const int c = blockIdx.x;
+  const int n2 = n * n;
+  C* B = reinterpret_cast<C*>(smem);
+  C* T = B + n2;
+  const R* E1 = E + (long long)l1[c] * n2;
+  const R* E2 = E + (long long)l2[c] * n2;
+  if (E_IN_LDS) {
+    R* e1 = reinterpret_cast<R*>(T + n2);
+    R* e2 = e1 + n2;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+      e1[i] = E1[i];
+      e2[i] = E2[i];
+    }
+    E1 = e1;
+    E2 = e2;
+  }
+  const R* lam1 = lam + (long long)l1[c] * n;
+  const R* lam2 = lam + (long long)l2[c] * n;
+  C* x = psi + (long long)c * n2;
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) B[i] = x[i];
+  __syncthreads();
+  kron_stage<R, C, true, true, 0>(E1, B, T, n, lam1, lam2, a);              // T = E1^T B
+  __syncthreads();
+  kron_stage<R, C, false, false, FWD ? 2 : 1>(E2, T, B, n, lam1, lam2, a);  // B = (T E2) / d  (or * d)
+  __syncthreads();
+  kron_stage<R, C, true, false, 0>(E1, B, T, n, lam1, lam2, a);             // T = E1 B
+  __syncthreads();
+  kron_stage<R, C, false, true, 0>(E2, T, B, n, lam1, lam2, a);             // B = T E2^T
+  __syncthreads();
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) x[i] = B[i];
+}
+
+static size_t kron_lds64(int n) { return 2 * (size_t)n * n * sizeof(double2); }
+static size_t kron_lds32(int n) { return 2 * (size_t)n * n * (sizeof(float2) + sizeof(float)); }
+
+void Kron::setup_attributes() {
+  PP_REQUIRE(kron_lds64(n) <= 160 * 1024 && kron_lds32(n) <= 160 * 1024, "Kronecker block too large for LDS");
+  PP_HIP(hipFuncSetAttribute((const void*)kron_kernel<double, double2, false, false>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kron_lds64(n)));
+  PP_HIP(hipFuncSetAttribute((const void*)kron_kernel<double, double2, false, true>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kron_lds64(n)));
+  PP_HIP(hipFuncSetAttribute((const void*)kron_kernel<float, float2, true, false>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kron_lds32(n)));
+}
+
+void kron_solve(const Kron& k, double2 a, double2* psi, hipStream_t s) { k.apply64(a, psi, s); }
+
+void Kron::apply64(double2 a, double2* psi, hipStream_t s) const {
+  hipLaunchKernelGGL((kron_kernel<double, double2, false, false>), dim3(nchan), dim3(1024), kron_lds64(n), s, E64.ptr,
+                     lam64.ptr, l1.ptr, l2.ptr, n, a, psi);
+  PP_HIP(hipGetLastError());
+}
+
+void Kron::multiply64(double2 a, double2* psi, hipStream_t s) const {
+  hipLaunchKernelGGL((kron_kernel<double, double2, false, true>), dim3(nchan), dim3(1024), kron_lds64(n), s, F64.ptr,
+                     lam64.ptr, l1.ptr, l2.ptr, n, a, psi);
+  PP_HIP(hipGetLastError());
+}
+
+void Kron::apply32(double2 a, float2* psi, hipStream_t s) const {
+  hipLaunchKernelGGL((kron_kernel<float, float2, true, false>), dim3(nchan), dim3(1024), kron_lds32(n), s, E32.ptr,
+                     lam32.ptr, l1.ptr, l2.ptr, n, make_float2((float)a.x, (float)a.y), psi);
+  PP_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------
+// flexible GMRES in FP32
+// ------------------------------------------------------------------------------------
+// min ||A y - b|| by Householder QR (A column-major rows x cols, full column rank)
+static void least_squares_small(int rows, int cols, std::vector<std::complex<double>> A,
+                                std::vector<std::complex<double>> b, std::vector<std::complex<double>>& x) {
+  using cd = std::complex<double>;
+  for (int k = 0; k < cols; ++k) {
+    double nrm = 0;
+    for (int i = k; i < rows; ++i) nrm += std::norm(A[(size_t)k * rows + i]);
+    nrm = std::sqrt(nrm);
+    if (nrm == 0) continue;
+    const cd akk = A[(size_t)k * rows + k];
+    const cd alpha = std::abs(akk) == 0 ? cd(-nrm, 0) : -akk / std::abs(akk) * nrm;
+    std::vector<cd> v(rows - k);
+    v[0] = akk - alpha;
+    for (int i = k + 1; i < rows; ++i) v[i - k] = A[(size_t)k * rows + i];
+    double vn = 0;
+    for (auto& z : v) vn += std::norm(z);
+    if (vn == 0) continue;
+    for (int c = k; c < cols; ++c) {
+      cd d = 0;
+      for (int i = k; i < rows; ++i) d += std::conj(v[i - k]) * A[(size_t)c * rows + i];
+      d *= 2.0 / vn;
+      for (int i = k; i < rows; ++i) A[(size_t)c * rows + i] -= d * v[i - k];
+    }
+    cd d = 0;
+    for (int i = k; i < rows; ++i) d += std::conj(v[i - k]) * b[i];
+    d *= 2.0 / vn;
+    for (int i = k; i < rows; ++i) b[i] -= d * v[i - k];
+  }
+  x.assign(cols, 0);
+  for (int k = cols - 1; k >= 0; --k) {
+    cd sum = b[k];
+    for (int c = k + 1; c < cols; ++c) sum -= A[(size_t)c * rows + k] * x[c];
+    x[k] = A[(size_t)k * rows + k] == cd(0) ? cd(0) : sum / A[(size_t)k * rows + k];
+  }
+}
+
+void MixedWork::ensure(long long n_, int m_) {
+  if (n == n_ && m >= m_) return;
+  n = n_;
+  m = m_;
+  V.resize((size_t)(m + 1) * n);
+  Z.resize((size_t)m * n);
+  w.resize(n);
+  tmp.resize(n);
+  rhs.resize(n);
+  d.resize(n);
+  small.resize(4 * (m + 4));
+  coef.resize(m + 1);
+  b64.resize(n);
+  r64.resize(n);
+  x64.resize(n);
+  t64.resize(n);
+  h64.resize(n);
+}
+
+void MixedWork::read_small(int count, hipStream_t s) {
+  host_small.resize(count);
+  PP_HIP(hipMemcpyAsync(host_small.data(), small.ptr, count * sizeof(double2), hipMemcpyDeviceToHost, s));
+  PP_HIP(hipStreamSynchronize(s));
+}
+
+void MixedWork::block_dot32(int k, const float2* w_, double2* out, hipStream_t s) {
+  const int blocks = reduction_blocks(n);
+  const int width = k + 1;
+  red.ensure(blocks, width);
+  hipLaunchKernelGGL(block_dot32_kernel, dim3(blocks), dim3(256), 0, s, n, k, V.ptr, n, w_, red.partial.ptr, width);
+  hipLaunchKernelGGL(finish64_kernel, dim3((width + 63) / 64), dim3(64), 0, s, red.partial.ptr, blocks, width, width,
+                     out);
+}
+
+void MixedWork::block_update32(int k, const double2* h, float2* w_, double2* out, hipStream_t s) {
+  const int blocks = reduction_blocks(n);
+  red.ensure(blocks, 1);
+  hipLaunchKernelGGL(block_update32_kernel, dim3(blocks), dim3(256), 0, s, n, k, V.ptr, n, h, w_, red.partial.ptr);
+  hipLaunchKernelGGL(finish64_kernel, dim3(1), dim3(64), 0, s, red.partial.ptr, blocks, 1, 1, out);
+}
+
+int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
+                        const std::function<void(int, const float2*, float2*)>& apply_P, const float2* b, float2* x,
+                        double rtol, int max_it, double* rel_out, hipStream_t s) {
+  using cd = std::complex<double>;
+  const int mm = std::min(max_it, m);
+  PP_REQUIRE(mm >= 1 && mm + 3 <= 120, "inner basis size out of range");
+  block_dot32(0, b, small.ptr, s);
+  read_small(1, s);
+  const double beta = std::sqrt(host_small[0].x);
+  if (!(beta > 0)) {
+    PP_HIP(hipMemsetAsync(x, 0, n * sizeof(float2), s));
+    if (rel_out) *rel_out = 0;
+    return 0;
+  }
+  hipLaunchKernelGGL(scale32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, (float)(1.0 / beta), b, V.ptr);
+  std::vector<cd> H((size_t)(mm + 1) * mm, 0.0);  // column-major (mm+1) x mm
+  auto Hc = [&](int row, int col) -> cd& { return H[(size_t)col * (mm + 1) + row]; };
+  std::vector<cd> y;
+  int k = 0;
+  double rel = 1.0;
+  for (int j = 0; j < mm; ++j) {
+    float2* vj = V.ptr + (size_t)j * n;
+    float2* zj = Z.ptr + (size_t)j * n;
+    float2* vn = V.ptr + (size_t)(j + 1) * n;
+    apply_P(j, vj, zj);
+    apply_A(zj, vn);
+    // classical Gram-Schmidt, two passes, one host read
+    const int w1 = j + 2;  // h (j+1) + |w|^2
+    double2* h1 = small.ptr;
+    double2* h2 = small.ptr + w1;
+    double2* nr = small.ptr + 2 * w1;
+    block_dot32(j + 1, vn, h1, s);
+    block_update32(j + 1, h1, vn, nr, s);
+    block_dot32(j + 1, vn, h2, s);
+    block_update32(j + 1, h2, vn, nr, s);
+    read_small(2 * w1 + 1, s);
+    for (int i = 0; i <= j; ++i)
+      Hc(i, j) = cd(host_small[i].x + host_small[w1 + i].x, host_small[i].y + host_small[w1 + i].y);
+    const double hn = std::sqrt(std::max(0.0, host_small[2 * w1].x));
+    Hc(j + 1, j) = hn;
+    k = j + 1;
+    std::vector<cd> A((size_t)(k + 1) * k), rhs_ls(k + 1, 0.0);
+    for (int c = 0; c < k; ++c)
+      for (int r = 0; r <= k; ++r) A[(size_t)c * (k + 1) + r] = Hc(r, c);
+    rhs_ls[0] = beta;
+    least_squares_small(k + 1, k, A, rhs_ls, y);
+    double e2 = 0;
+    for (int r = 0; r <= k; ++r) {
+      cd acc = r == 0 ? -beta : 0.0;
+      for (int c = 0; c < k; ++c) acc += Hc(r, c) * y[c];
+      e2 += std::norm(acc);
+    }
+    rel = std::sqrt(e2) / beta;
+    if (rel <= rtol || !(hn > 1e-30 * beta)) break;
+    hipLaunchKernelGGL(scale32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, (float)(1.0 / hn), vn, vn);
+  }
+  std::vector<double2> yc(k);
+  for (int i = 0; i < k; ++i) yc[i] = make_double2(y[i].real(), y[i].imag());
+  coef.upload(yc.data(), yc.size(), s);
+  hipLaunchKernelGGL(lincomb32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, k, Z.ptr, n, coef.ptr, x);
+  PP_HIP(hipGetLastError());
+  if (rel_out) *rel_out = rel;
+  return k;
+}
+
+// ------------------------------------------------------------------------------------
+// mixed Cayley step
+// ------------------------------------------------------------------------------------
+void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s) {
+  PP_REQUIRE(ham && kron, "mixed Cayley needs a Hamiltonian and a Kronecker preconditioner");
+  Hamiltonian& H = *ham;
+  const Repr& R = *H.repr;
+  const long long n = H.n;
+  work.ensure(n, m);
+  scal.assign(sc, sc + H.terms.size());
+  stats = MixedStats{};
+  const double a = 0.5 * dt;
+  const Kron& K = *kron;
+  const double2 ka = make_double2(0.0, a);
+  // h = H psi ; b = S psi - i a h ; r = -2 i a h ; x = psi
+  vec_copy(n, psi, work.t64.ptr, s);
+  R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
+  H.apply(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), s);
+  ++stats.outer_ops;
+  vec_combine(n, make_double2(1.0, 0.0), work.t64.ptr, make_double2(0.0, -a), work.h64.ptr, work.b64.ptr, s);
+  vec_combine(n, make_double2(0.0, 0.0), work.h64.ptr, make_double2(0.0, -2.0 * a), work.h64.ptr, work.r64.ptr, s);
+  vec_copy(n, psi, work.x64.ptr, s);
+  vec_norm2(n, work.b64.ptr, work.small.ptr, work.red, s);
+  vec_norm2(n, work.r64.ptr, work.small.ptr + 1, work.red, s);
+  work.read_small(2, s);
+  const double bnorm = std::sqrt(work.host_small[0].x);
+  double rn = std::sqrt(work.host_small[1].x);
+
+  auto apply_A32 = [&](const float2* in, float2* out) {
+    // out = S in + i a H in
+    work.overlap_multiply32(R, in, work.tmp.ptr, work.w.ptr, s);
+    H.apply_f32(scal.data(), in, out, work.tmp.ptr, make_float2(0.f, (float)a), s);
+    ++stats.inner_ops;
+  };
+  auto apply_P32 = [&](int j, const float2* in, float2* out) {
+    if (j == 0 && learned) {
+      learned_apply(*learned, K, ka, in, out, s);
+      ++stats.learned_applies;
+    } else {
+      PP_HIP(hipMemcpyAsync(out, in, n * sizeof(float2), hipMemcpyDeviceToDevice, s));
+      K.apply32(ka, out, s);
+    }
+    ++stats.precond_applies;
+  };
+  bool converged = false;
+  for (int outer = 0;; ++outer) {
+    stats.residual = rn;
+    stats.rel_residual = bnorm > 0 ? rn / bnorm : 0.0;
+    if (rn <= tol_rel * bnorm) {
+      converged = true;
+      break;
+    }
+    if (outer >= max_outer || stats.inner_iterations > fallback_iterations) break;
+    ++stats.outer_iterations;
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, work.r64.ptr, work.rhs.ptr);
+    double rel = 0;
+    stats.inner_iterations += work.fgmres32(apply_A32, apply_P32, work.rhs.ptr, work.d.ptr, inner_rtol, m, &rel, s);
+    hipLaunchKernelGGL(add_f32_to_f64_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, work.d.ptr, work.x64.ptr);
+    // r = b - (S x + i a H x)
+    vec_copy(n, work.x64.ptr, work.t64.ptr, s);
+    R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
+    H.apply(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
+    ++stats.outer_ops;
+    const int blocks = reduction_blocks(n);
+    work.red.ensure(blocks, 1);
+    hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(256), 0, s, n, work.b64.ptr, work.r64.ptr,
+                       work.red.partial.ptr);
+    hipLaunchKernelGGL(finish64_kernel, dim3(1), dim3(64), 0, s, work.red.partial.ptr, blocks, 1, 1, work.small.ptr);
+    PP_HIP(hipGetLastError());
+    work.read_small(1, s);
+    rn = std::sqrt(work.host_small[0].x);
+  }
+  if (!converged) {
+    if (!fallback) throw Error(PP_ERR_NOT_CONVERGED, "mixed-precision Cayley step did not converge");
+    // psi is untouched: the plain FP64 solver with the exact block preconditioner redoes the step
+    fallback->step(psi, dt, sc, s);
+    stats.fallback = 1;
+    return;
+  }
+  vec_copy(n, work.x64.ptr, psi, s);
+}
+
+}  // namespace pp
+
+// ------------------------------------------------------------------------------------
+// C-ABI
+// ------------------------------------------------------------------------------------
+using namespace pp;
+
+extern "C" int pp_kron_create(int nchan, int n, int nl, const double* S, const double* E, const double* lam,
+                              const int* l1, const int* l2, pp_kron** out) {
+  return guarded([&] {
+    PP_REQUIRE(out && S && E && lam && l1 && l2 && nchan > 0 && n > 0 && nl > 0, "invalid Kronecker arguments");
+    for (int c = 0; c < nchan; ++c)
+      PP_REQUIRE(l1[c] >= 0 && l1[c] < nl && l2[c] >= 0 && l2[c] < nl, "channel l index out of range");
+    auto* p = new pp_kron();
+    Kron& k = p->k;
+    k.nchan = nchan;
+    k.n = n;
+    k.nl = nl;
+    try {
+      k.setup_attributes();
+      hipStream_t s = nullptr;
+      const size_t ne = (size_t)nl * n * n, nlam = (size_t)nl * n;
+      k.E64.upload(E, ne, s);
+      k.lam64.upload(lam, nlam, s);
+      std::vector<double> F(ne, 0.0);  // F_l = S E_l
+      for (int l = 0; l < nl; ++l)
+        for (int i = 0; i < n; ++i)
+          for (int q = 0; q < n; ++q) {
+            const double sv = S[(size_t)i * n + q];
+            if (sv == 0.0) continue;
+            for (int j = 0; j < n; ++j) F[((size_t)l * n + i) * n + j] += sv * E[((size_t)l * n + q) * n + j];
+          }
+      k.F64.upload(F.data(), ne, s);
+      std::vector<float> ef(E, E + ne), lf(lam, lam + nlam);
+      k.E32.upload(ef.data(), ne, s);
+      k.lam32.upload(lf.data(), nlam, s);
+      k.l1.upload(l1, nchan, s);
+      k.l2.upload(l2, nchan, s);
+      PP_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+
+extern "C" int pp_kron_destroy(pp_kron* k) {
+  return guarded([&] { delete k; });
+}
+
+extern "C" int pp_kron_solve(const pp_kron* k, pp_cplx a, pp_cplx* psi, pp_stream stream) {
+  return guarded([&] {
+    PP_REQUIRE(k && psi, "null argument");
+    k->k.apply64(make_double2(a.re, a.im), reinterpret_cast<double2*>(psi), (hipStream_t)stream);
+  });
+}
+
+extern "C" int pp_kron_multiply(const pp_kron* k, pp_cplx a, pp_cplx* psi, pp_stream stream) {
+  return guarded([&] {
+    PP_REQUIRE(k && psi, "null argument");
+    k->k.multiply64(make_double2(a.re, a.im), reinterpret_cast<double2*>(psi), (hipStream_t)stream);
+  });
+}
+
+extern "C" int pp_kron_solve_f32(const pp_kron* k, pp_cplx a, pp_cplxf* psi, pp_stream stream) {
+  return guarded([&] {
+    PP_REQUIRE(k && psi, "null argument");
+    k->k.apply32(make_double2(a.re, a.im), reinterpret_cast<float2*>(psi), (hipStream_t)stream);
+  });
+}
+
+extern "C" int pp_cayley_set_kron(pp_cayley* c, const pp_kron* k) {
+  return guarded([&] {
+    PP_REQUIRE(c, "null argument");
+    PP_REQUIRE(!k || (long long)k->k.nchan * k->k.n * k->k.n == c->c.n, "Kronecker shape does not match psi");
+    c->c.kron = k ? &k->k : nullptr;
+  });
+}
+
+extern "C" int pp_mixed_create(pp_hamiltonian* ham, const pp_kron* k, int basis_size, double tol_rel,
+                               double inner_rtol, int max_outer, pp_mixed** out) {
+  return guarded([&] {
+    PP_REQUIRE(ham && k && out, "null argument");
+    PP_REQUIRE(basis_size >= 1 && basis_size <= 100, "inner basis size must be in 1..100");
+    PP_REQUIRE(tol_rel > 0 && inner_rtol > 0 && max_outer >= 1, "invalid tolerances");
+    PP_REQUIRE((long long)k->k.nchan * k->k.n * k->k.n == ham->h.n, "Kronecker shape does not match psi");
+    PP_REQUIRE(ham->h.fused, "the mixed solver needs the fused banded Hamiltonian");
+    auto* p = new pp_mixed();
+    p->c.ham = &ham->h;
+    p->c.kron = &k->k;
+    p->c.m = basis_size;
+    p->c.tol_rel = tol_rel;
+    p->c.inner_rtol = inner_rtol;
+    p->c.max_outer = max_outer;
+    *out = p;
+  });
+}
+
+extern "C" int pp_mixed_destroy(pp_mixed* m) {
+  return guarded([&] { delete m; });
+}
+
+extern "C" int pp_mixed_set_learned(pp_mixed* m, pp_learned* l) {
+  return guarded([&] {
+    PP_REQUIRE(m, "null argument");
+    if (l) PP_REQUIRE(l->L.nchan == m->c.kron->nchan && l->L.n == m->c.kron->n, "network shape does not match");
+    m->c.learned = l ? &l->L : nullptr;
+  });
+}
+
+extern "C" int pp_mixed_set_fallback(pp_mixed* m, pp_cayley* plain, int fallback_iterations) {
+  return guarded([&] {
+    PP_REQUIRE(m && fallback_iterations >= 1, "invalid argument");
+    PP_REQUIRE(!plain || plain->c.n == m->c.ham->n, "fallback solver size does not match");
+    m->c.fallback = plain ? &plain->c : nullptr;
+    m->c.fallback_iterations = fallback_iterations;
+  });
+}
+
+extern "C" int pp_mixed_step(pp_mixed* m, pp_cplx* psi, double dt, const double* scalings, pp_stream stream) {
+  return guarded([&] {
+    PP_REQUIRE(m && psi && scalings, "null argument");
+    m->c.step(reinterpret_cast<double2*>(psi), dt, scalings, (hipStream_t)stream);
+  });
+}
+
+extern "C" int pp_mixed_stats(const pp_mixed* m, int* counts, double* residuals) {
+  return guarded([&] {
+    PP_REQUIRE(m, "null argument");
+    const MixedStats& st = m->c.stats;
+    if (counts) {
+      counts[0] = st.outer_iterations;
+      counts[1] = st.inner_iterations;
+      counts[2] = st.outer_ops;
+      counts[3] = st.inner_ops;
+      counts[4] = st.precond_applies;
+      counts[5] = st.learned_applies;
+      counts[6] = st.fallback;
+    }
+    if (residuals) {
+      residuals[0] = st.residual;
+      residuals[1] = st.rel_residual;
+    }
+  });
+}

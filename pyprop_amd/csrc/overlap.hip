@@ -92,6 +92,158 @@ __global__ void overlap_fibre_kernel(const double* __restrict__ u, int n, int sd
   else { if (CONJ) solve_uh(); else solve_u(); }
 }
 
+// Same operations with the band width fixed at compile time: each thread keeps a sliding
+// window of SD+1 fibre values in registers, so every element of the fibre is loaded and
+// stored exactly once per pass and the recurrences run out of registers.  The band
+// coefficients of row/column i are uniform across the wave (scalar loads).
+template <int OP, bool CONJ, int SD>
+__global__ __launch_bounds__(128) void overlap_window_kernel(const double* __restrict__ u, int n, long long pre,
+                                                            long long post, double2* __restrict__ data) {
+  const long long fib = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (fib >= pre * post) return;
+  const long long p = fib / post, q = fib % post;
+  double2* x = data + p * n * post + q;
+  const long long st = post;
+  auto U = [&](int i, int j) { return u[j * (SD + 1) + SD + i - j]; };
+  auto ld = [&](int i) { return (i >= 0 && i < n) ? x[i * st] : make_double2(0, 0); };
+  // x <- U x, ascending: x_i = sum_{d=0..SD} U(i, i+d) x_{i+d} (old values)
+  auto mul_u = [&]() {
+    double2 w[SD + 1];
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) w[d] = ld(d);
+    for (int i = 0; i < n; ++i) {
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int d = 0; d <= SD; ++d) {
+        if (i + d < n) {
+          const double a = U(i, i + d);
+          acc.x = fma(a, w[d].x, acc.x);
+          acc.y = fma(a, w[d].y, acc.y);
+        }
+      }
+      x[i * st] = acc;
+#pragma unroll
+      for (int d = 0; d < SD; ++d) w[d] = w[d + 1];
+      w[SD] = ld(i + SD + 1);
+    }
+  };
+  // x <- U^H x, descending: x_i = sum_{d=0..SD} U(i-d, i) x_{i-d} (old values)
+  auto mul_uh = [&]() {
+    double2 w[SD + 1];
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) w[d] = ld(n - 1 - d);
+    for (int i = n - 1; i >= 0; --i) {
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int d = 0; d <= SD; ++d) {
+        if (i - d >= 0) {
+          const double a = U(i - d, i);
+          acc.x = fma(a, w[d].x, acc.x);
+          acc.y = fma(a, w[d].y, acc.y);
+        }
+      }
+      x[i * st] = acc;
+#pragma unroll
+      for (int d = 0; d < SD; ++d) w[d] = w[d + 1];
+      w[SD] = ld(i - SD - 1);
+    }
+  };
+  // x <- U^-1 x, descending; window holds the SD most recently solved values x_{i+1..i+SD}
+  auto solve_u = [&]() {
+    double2 w[SD + 1];
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) w[d] = make_double2(0, 0);
+    for (int i = n - 1; i >= 0; --i) {
+      double2 acc = x[i * st];
+#pragma unroll
+      for (int d = 1; d <= SD; ++d) {
+        if (i + d < n) {
+          const double a = U(i, i + d);
+          acc.x = fma(-a, w[d - 1].x, acc.x);
+          acc.y = fma(-a, w[d - 1].y, acc.y);
+        }
+      }
+      const double dg = U(i, i);
+      acc = make_double2(acc.x / dg, acc.y / dg);
+      x[i * st] = acc;
+#pragma unroll
+      for (int d = SD; d > 0; --d) w[d] = w[d - 1];
+      w[0] = acc;
+    }
+  };
+  // x <- U^-H x, ascending; window holds x_{i-1..i-SD}
+  auto solve_uh = [&]() {
+    double2 w[SD + 1];
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) w[d] = make_double2(0, 0);
+    for (int i = 0; i < n; ++i) {
+      double2 acc = x[i * st];
+#pragma unroll
+      for (int d = 1; d <= SD; ++d) {
+        if (i - d >= 0) {
+          const double a = U(i - d, i);
+          acc.x = fma(-a, w[d - 1].x, acc.x);
+          acc.y = fma(-a, w[d - 1].y, acc.y);
+        }
+      }
+      const double dg = U(i, i);
+      acc = make_double2(acc.x / dg, acc.y / dg);
+      x[i * st] = acc;
+#pragma unroll
+      for (int d = SD; d > 0; --d) w[d] = w[d - 1];
+      w[0] = acc;
+    }
+  };
+  if (OP == PP_OVERLAP_MULTIPLY) { mul_u(); mul_uh(); }
+  else if (OP == PP_OVERLAP_SOLVE) { solve_uh(); solve_u(); }
+  else if (OP == PP_OVERLAP_SQRT_MULTIPLY) { if (CONJ) mul_uh(); else mul_u(); }
+  else { if (CONJ) solve_uh(); else solve_u(); }
+}
+
+// Element-parallel triangular multiplies (no recurrence): one thread per tensor element,
+// out-of-place.  mode 0: y = U x, mode 1: y = U^H x along the rank of extent n with
+// stride `post`.  Used for the multiply ops, which need no sequential sweep.
+template <int SD, int MODE>
+__global__ __launch_bounds__(256) void overlap_elem_kernel(const double* __restrict__ u, int n, long long post,
+                                                           long long total, const double2* __restrict__ x,
+                                                           double2* __restrict__ y) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const long long q = idx % post;
+  const int i = (int)((idx / post) % n);
+  const long long base = idx - (long long)i * post;  // element 0 of this fibre
+  double2 acc = make_double2(0, 0);
+#pragma unroll
+  for (int d = 0; d <= SD; ++d) {
+    const int j = MODE == 0 ? i + d : i - d;
+    if (j >= 0 && j < n) {
+      const double a = MODE == 0 ? u[j * (SD + 1) + SD - d] : u[i * (SD + 1) + SD - d];
+      const double2 v = x[base + (long long)j * post];
+      acc.x = fma(a, v.x, acc.x);
+      acc.y = fma(a, v.y, acc.y);
+    }
+  }
+  (void)q;
+  y[idx] = acc;
+}
+
+template <int SD>
+static void launch_window(int op, bool conj, const double* u, int n, long long pre, long long post, double2* data,
+                          int blocks, int threads, hipStream_t s) {
+#define PP_OW(OPC, CJ) \
+  hipLaunchKernelGGL((overlap_window_kernel<OPC, CJ, SD>), dim3(blocks), dim3(threads), 0, s, u, n, pre, post, data)
+  switch (op) {
+    case PP_OVERLAP_MULTIPLY: PP_OW(PP_OVERLAP_MULTIPLY, false); break;
+    case PP_OVERLAP_SOLVE: PP_OW(PP_OVERLAP_SOLVE, false); break;
+    case PP_OVERLAP_SQRT_MULTIPLY:
+      if (conj) PP_OW(PP_OVERLAP_SQRT_MULTIPLY, true); else PP_OW(PP_OVERLAP_SQRT_MULTIPLY, false);
+      break;
+    default:
+      if (conj) PP_OW(PP_OVERLAP_SQRT_SOLVE, true); else PP_OW(PP_OVERLAP_SQRT_SOLVE, false);
+  }
+#undef PP_OW
+}
+
 void overlap_apply(const OverlapFactor& ov, int op, bool conj, int ndim, const int* extents, int rank,
                    double2* data, hipStream_t s) {
   PP_REQUIRE(rank >= 0 && rank < ndim, "rank out of range");
@@ -103,6 +255,40 @@ void overlap_apply(const OverlapFactor& ov, int op, bool conj, int ndim, const i
   const int threads = 128;
   const int blocks = ceil_div(fibres, threads);
   if (blocks == 0) return;
+  PP_REQUIRE(op >= 0 && op <= 3, "unknown overlap op");
+  if (ov.sd >= 1 && ov.sd <= 8 && (op == PP_OVERLAP_MULTIPLY || op == PP_OVERLAP_SQRT_MULTIPLY)) {
+    const long long total = pre * ov.n * post;
+    if (ov.work.count < (size_t)total) ov.work.resize(total);
+    const int eb = ceil_div(total, 256);
+    double2* w = ov.work.ptr;
+    auto pass = [&](int mode, const double2* in, double2* out) {
+      switch (ov.sd * 2 + mode) {
+#define PP_EL(K) \
+  case 2 * K: hipLaunchKernelGGL((overlap_elem_kernel<K, 0>), dim3(eb), dim3(256), 0, s, ov.u.ptr, ov.n, post, total, in, out); break; \
+  case 2 * K + 1: hipLaunchKernelGGL((overlap_elem_kernel<K, 1>), dim3(eb), dim3(256), 0, s, ov.u.ptr, ov.n, post, total, in, out); break;
+        PP_EL(1) PP_EL(2) PP_EL(3) PP_EL(4) PP_EL(5) PP_EL(6) PP_EL(7) PP_EL(8)
+#undef PP_EL
+      }
+    };
+    if (op == PP_OVERLAP_MULTIPLY) {
+      pass(0, data, w);     // U x
+      pass(1, w, data);     // U^H (U x)
+    } else {
+      pass(conj ? 1 : 0, data, w);
+      PP_HIP(hipMemcpyAsync(data, w, total * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    }
+    PP_HIP(hipGetLastError());
+    return;
+  }
+  if (ov.sd >= 1 && ov.sd <= 8) {
+    switch (ov.sd) {
+#define PP_SD(K) case K: launch_window<K>(op, conj, ov.u.ptr, ov.n, pre, post, data, blocks, threads, s); break;
+      PP_SD(1) PP_SD(2) PP_SD(3) PP_SD(4) PP_SD(5) PP_SD(6) PP_SD(7) PP_SD(8)
+#undef PP_SD
+    }
+    PP_HIP(hipGetLastError());
+    return;
+  }
 #define PP_OV_LAUNCH(OPC, CJ) \
   hipLaunchKernelGGL((overlap_fibre_kernel<OPC, CJ>), dim3(blocks), dim3(threads), 0, s, ov.u.ptr, ov.n, ov.sd, pre, post, data)
   switch (op) {

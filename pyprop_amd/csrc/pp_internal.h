@@ -102,6 +102,7 @@ struct OverlapFactor {
   int sd = 0;                    // superdiagonals
   std::vector<double> u_host;    // (n, sd+1) upper band, [j][sd + i - j]
   DeviceBuffer<double> u;        // same layout on device
+  mutable DeviceBuffer<double2> work;  // out-of-place passes of the multiply ops (grown on demand)
 };
 void overlap_apply(const OverlapFactor& ov, int op, bool conj, int ndim, const int* extents, int rank,
                    double2* data, hipStream_t s);

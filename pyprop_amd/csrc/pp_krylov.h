@@ -90,9 +90,14 @@ struct Precond {
 };
 void precond_solve(const Precond& pc, double2* psi, hipStream_t s);
 
+struct Kron;  // separable (Kronecker) preconditioner, mixed.hip
+// psi <- K^-1 psi with K_c = S x S + a (h_l1 x S + S x h_l2) per channel (FP64)
+void kron_solve(const Kron& k, double2 a, double2* psi, hipStream_t s);
+
 struct Cayley {
   Hamiltonian* ham = nullptr;
   const Precond* pc = nullptr;
+  const Kron* kron = nullptr;  // used when pc is null
   long long n = 0;
   Gmres gmres;
   DeviceBuffer<double2> tmp_s, rhs;

@@ -1,0 +1,92 @@
+// Mixed-precision Cayley solve: Kronecker preconditioner, FP32 flexible GMRES, FP64
+// iterative refinement, and the hook for the learned preconditioner.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <functional>
+#include <vector>
+
+#include "pp_common.h"
+#include "pp_internal.h"
+#include "pp_krylov.h"
+
+namespace pp {
+
+// Exact inverse of the separable part of every channel block of S (x) S + a H0:
+//   K_c = S x S + a (h_l1 x S + S x h_l2),   h_l E_l = S E_l diag(lam_l),  E_l^T S E_l = I.
+struct Kron {
+  int nchan = 0, n = 0, nl = 0;
+  DeviceBuffer<double> E64, lam64;   // [l][n][n] row-major, [l][n]
+  DeviceBuffer<double> F64;          // F_l = S E_l (= E_l^-T), for the forward product
+  DeviceBuffer<float> E32, lam32;
+  DeviceBuffer<int> l1, l2;          // per channel
+  void setup_attributes();
+  // psi <- K^-1 psi, a = complex factor of the one-electron part (i dt / 2)
+  void apply64(double2 a, double2* psi, hipStream_t s) const;
+  void apply32(double2 a, float2* psi, hipStream_t s) const;
+  // psi <- K psi (FP64)
+  void multiply64(double2 a, double2* psi, hipStream_t s) const;
+};
+
+struct Learned;  // learned.hip
+
+struct MixedStats {
+  int outer_iterations = 0;   // FP32 corrections applied
+  int inner_iterations = 0;   // FP32 FGMRES iterations, summed
+  int outer_ops = 0;          // FP64 Hamiltonian multiplies
+  int inner_ops = 0;          // FP32 Hamiltonian multiplies
+  int precond_applies = 0;
+  int learned_applies = 0;
+  int fallback = 0;           // 1 when the step was redone by the plain solver
+  double residual = 0;        // final |b - A x| (FP64)
+  double rel_residual = 0;    // final |b - A x| / |b|
+};
+
+struct MixedWork {
+  long long n = -1;
+  int m = 0;
+  DeviceBuffer<float2> V, Z, w, tmp, rhs, d;
+  DeviceBuffer<double2> small, coef, b64, r64, x64, t64, h64;
+  Reducer red;
+  std::vector<double2> host_small;
+  void ensure(long long n, int m);
+  void read_small(int count, hipStream_t s);
+  // out[0..k-1] = V_i^H w (FP64 accumulation), out[k] = |w|^2
+  void block_dot32(int k, const float2* w, double2* out, hipStream_t s);
+  // w -= sum_i V_i h_i, out[0] = |w|^2
+  void block_update32(int k, const double2* h, float2* w, double2* out, hipStream_t s);
+  // out = S x S in (FP32, out-of-place; scratch has n elements)
+  void overlap_multiply32(const Repr& r, const float2* in, float2* out, float2* scratch, hipStream_t s);
+  std::vector<DeviceBuffer<float>> u32 = std::vector<DeviceBuffer<float>>(kMaxRank);
+  // flexible GMRES(m) on A d = rhs, x0 = 0, right preconditioner P_j (j = iteration)
+  int fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
+               const std::function<void(int, const float2*, float2*)>& apply_P, const float2* rhs, float2* d,
+               double rtol, int max_it, double* rel_out, hipStream_t s);
+};
+
+struct MixedCayley {
+  Hamiltonian* ham = nullptr;
+  const Kron* kron = nullptr;        // FP32 base preconditioner (required)
+  Learned* learned = nullptr;        // optional correction on the first direction of each inner solve
+  Cayley* fallback = nullptr;        // optional plain FP64 solver (exact block preconditioner)
+  int m = 20;                        // inner FGMRES basis
+  double tol_rel = 1e-10;            // outer stop: |b - A x| <= tol_rel |b|
+  double inner_rtol = 1e-5;          // inner stop: relative to the current residual
+  int max_outer = 8;
+  int fallback_iterations = 40;      // inner iterations of one step beyond which the plain solver redoes it
+  MixedWork work;
+  MixedStats stats;
+  std::vector<double> scal;
+  void step(double2* psi, double dt, const double* scalings, hipStream_t s);
+};
+
+}  // namespace pp
+
+struct pp_kron {
+  pp::Kron k;
+};
+struct pp_mixed {
+  pp::MixedCayley c;
+};

@@ -192,7 +192,7 @@ __global__ __launch_bounds__(512) void pc_diag_kernel(const double2* __restrict_
   }
 }
 
-// sequential block sweep over one channel held in LDS
+// Sequential block sweep over one channel held in LDS (generic fallback, any n2).
 __global__ __launch_bounds__(512) void pc_sweep_kernel(const double2* __restrict__ Mp, double2* __restrict__ psi, int n1,
                                                        int n2, int b, int forward) {
   extern __shared__ double2 y[];
@@ -228,14 +228,90 @@ __global__ __launch_bounds__(512) void pc_sweep_kernel(const double2* __restrict
   for (int i = threadIdx.x; i < n1 * n2; i += blockDim.x) g[i] = y[i];
 }
 
+// Block sweep for n2 <= 64: 16 threads per row, 1024 threads; all B*4 matrix elements a
+// thread needs for a block step are loaded before any is used, so the step costs one
+// memory latency plus the transfer of the block row instead of a chain of dependent
+// loads.  The matrix is streamed with non-temporal loads (read once per solve).
+typedef double pc_d2v __attribute__((ext_vector_type(2)));
+template <int B>
+__global__ __launch_bounds__(1024) void pc_sweep16_kernel(const double2* __restrict__ Mp, double2* __restrict__ psi,
+                                                          int n1, int n2, int forward) {
+  constexpr int JPER = 4;  // ceil(64 / 16)
+  extern __shared__ double2 y[];
+  const int c = blockIdx.x;
+  double2* g = psi + (long long)c * n1 * n2;
+  for (int i = threadIdx.x; i < n1 * n2; i += blockDim.x) y[i] = g[i];
+  __syncthreads();
+  const int part = threadIdx.x & 15;
+  const int row = threadIdx.x >> 4;
+  const bool active = row < n2;
+  const int width = B * n2;
+  for (int step = 0; step < n1; ++step) {
+    const int I = forward ? step : n1 - 1 - step;
+    const double2* mrow = Mp + ((long long)c * n1 + I) * (long long)B * n2 * n2 + (long long)row * width;
+    double2 m[B][JPER];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int J = forward ? I - B + q : I + 1 + q;
+#pragma unroll
+      for (int u = 0; u < JPER; ++u) {
+        const int j = part + 16 * u;
+        if (active && J >= 0 && J < n1 && j < n2) {
+          const pc_d2v v = __builtin_nontemporal_load(reinterpret_cast<const pc_d2v*>(mrow + q * n2 + j));
+          m[q][u] = make_double2(v.x, v.y);
+        } else {
+          m[q][u] = make_double2(0, 0);
+        }
+      }
+    }
+    double2 acc = make_double2(0, 0);
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int J = forward ? I - B + q : I + 1 + q;
+      if (J < 0 || J >= n1) continue;
+      const double2* yj = y + J * n2;
+#pragma unroll
+      for (int u = 0; u < JPER; ++u) {
+        const int j = part + 16 * u;
+        if (j < n2) cfma(acc, m[q][u], yj[j]);
+      }
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+      acc.x += __shfl_xor(acc.x, off, 16);
+      acc.y += __shfl_xor(acc.y, off, 16);
+    }
+    if (active && part == 0) {
+      const double2 v = y[I * n2 + row];
+      y[I * n2 + row] = make_double2(v.x - acc.x, v.y - acc.y);
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < n1 * n2; i += blockDim.x) g[i] = y[i];
+}
+
+static void launch_sweep(const Precond& pc, const double2* M, double2* psi, int forward, hipStream_t s) {
+  const size_t lds = (size_t)pc.n1 * pc.n2 * sizeof(double2);
+  if (pc.n2 <= 64 && pc.b >= 1 && pc.b <= 8) {
+    switch (pc.b) {
+#define PP_SW(BB) case BB: hipLaunchKernelGGL(pc_sweep16_kernel<BB>, dim3(pc.nchan), dim3(1024), lds, s, M, psi, pc.n1, pc.n2, forward); break;
+      PP_SW(1) PP_SW(2) PP_SW(3) PP_SW(4) PP_SW(5) PP_SW(6) PP_SW(7) PP_SW(8)
+#undef PP_SW
+    }
+  } else {
+    hipLaunchKernelGGL(pc_sweep_kernel, dim3(pc.nchan), dim3(512), lds, s, M, psi, pc.n1, pc.n2, pc.b, forward);
+  }
+}
+
 void precond_solve(const Precond& pc, double2* psi, hipStream_t s) {
   const int nblk = pc.nchan * pc.n1;
   const size_t lds_diag = pc.n2 * sizeof(double2);
   const size_t lds_sweep = (size_t)pc.n1 * pc.n2 * sizeof(double2);
+  (void)lds_sweep;
   hipLaunchKernelGGL(pc_diag_kernel, dim3(nblk), dim3(512), lds_diag, s, pc.linv.ptr, psi, pc.n2);
-  hipLaunchKernelGGL(pc_sweep_kernel, dim3(pc.nchan), dim3(512), lds_sweep, s, pc.lp.ptr, psi, pc.n1, pc.n2, pc.b, 1);
+  launch_sweep(pc, pc.lp.ptr, psi, 1, s);
   hipLaunchKernelGGL(pc_diag_kernel, dim3(nblk), dim3(512), lds_diag, s, pc.uinv.ptr, psi, pc.n2);
-  hipLaunchKernelGGL(pc_sweep_kernel, dim3(pc.nchan), dim3(512), lds_sweep, s, pc.up.ptr, psi, pc.n1, pc.n2, pc.b, 0);
+  launch_sweep(pc, pc.up.ptr, psi, 0, s);
   PP_HIP(hipGetLastError());
 }
 

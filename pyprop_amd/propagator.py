@@ -297,11 +297,14 @@ class CayleyPropagator:
             self.Preconditioner.SetHamiltonianScaling(1.0j * dt / 2.0)
             self.Preconditioner.SetOverlapScaling(1.0)
             self.Preconditioner.Setup(self)
+        kron = getattr(self.Preconditioner, "kind", None) == "kron"
         h = C.c_void_p()
         N.check(N.load().pp_cayley_create(self.BasePropagator.ham,
-                                          None if self.Preconditioner is None else self.Preconditioner.h,
+                                          None if self.Preconditioner is None or kron else self.Preconditioner.h,
                                           self.basis_size, self.tol, int(self.dorth), C.byref(h)))
         self.h = h
+        if kron:
+            N.check(N.load().pp_cayley_set_kron(h, self.Preconditioner.h))
         self.dt = dt
 
     def AdvanceStep(self, t, dt):
