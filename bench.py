@@ -3,10 +3,17 @@
 Workload: helium in coupled spherical harmonics (L = 0..2, M = 0, l1, l2 <= 5: 42
 channels) x 60 order-7 B-splines per electron on [0, 40] (N = 151,200 complex128);
 H = H0 + 1/r12 (multipoles lambda <= 10) + length-gauge dipole E(t), stored as
-Simp x banded-packed x banded-nonhermitian tensor potentials; Cayley step with GMRES
-(basis 30, tolerance 1e-10) and the exact block-diagonal preconditioner; dt = 0.01,
-E0 = 0.05, omega = 0.057, sin^2 envelope of 4 cycles; start from the independent-
-particle 1s^2 state.  Data are synthetic (generated here), nothing is downloaded.
+Simp x banded-packed x banded-nonhermitian tensor potentials; dt = 0.01, E0 = 0.05,
+omega = 0.057, sin^2 envelope of 4 cycles; start from the independent-particle 1s^2
+state.  Data are synthetic (generated here), nothing is downloaded.
+
+Three solvers of the same Cayley system are timed on the same steps:
+  plain FP64      GMRES(30, 1e-10) with the exact block-diagonal preconditioner
+  FP32 + Kron     complex64 flexible GMRES with the Kronecker preconditioner inside FP64
+                  iterative refinement to |b - A x| <= 1e-10 |b|
+  FP32 + learned  the same with the learned correction (trained here first, on pairs
+                  from the plain solver under seeded random pulses)
+`value` is the learned + FP32 rate; the errors are against the plain FP64 trajectory.
 
 Prints one JSON line (see --help).  A "step" is one Cayley AdvanceStep.
 """
@@ -112,14 +119,36 @@ def cpu_baseline(sysm, n_ops_per_step, sample_frac=0.1):
                        "applications + right-hand side" % (int(sample_frac * 100), t_mul, t_pc, n_ops_per_step))
 
 
+def run_steps(torch, prop, psi, t0, dt, warmup, steps, stats=None):
+    """W untimed steps then K timed steps from t0; returns (seconds, t_end)."""
+    t = t0
+    for _ in range(warmup):
+        prop.AdvanceStep(t, dt)
+        t += dt
+    torch.cuda.synchronize()
+    tstart = time.perf_counter()
+    for _ in range(steps):
+        prop.AdvanceStep(t, dt)
+        if stats is not None:
+            stats.append(prop.GetStats() if hasattr(prop, "GetStats") else {"ops": prop.GetOperatorCount()})
+        t += dt
+    torch.cuda.synchronize()
+    return time.perf_counter() - tstart, t
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--train-trajectories", type=int, default=64)
+    ap.add_argument("--train-steps", type=int, default=24)
+    ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--profile-matvec", action="store_true", help="only run the matvec loop (for rocprof)")
-    ap.add_argument("--profile-cayley", type=int, default=0, help="only run this many Cayley steps (for rocprof)")
+    ap.add_argument("--profile-cayley", type=int, default=0, help="only run this many plain Cayley steps (for rocprof)")
+    ap.add_argument("--profile-mixed", type=int, default=0,
+                    help="only run this many mixed-precision steps with the Kronecker preconditioner (for rocprof)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -128,13 +157,15 @@ def main():
     import numpy as np
     import torch
     assert torch.cuda.is_available(), "bench needs a GPU"
-    from pyprop_amd.propagator import CayleyPropagator, RadialTwoElectronPreconditioner
+    from pyprop_amd.mixed import (KroneckerPreconditioner, LearnedPreconditioner, MixedCayleyPropagator,
+                                  generate_training_pairs)
+    from pyprop_amd.propagator import CayleyPropagator, RadialTwoElectronPreconditioner, Wavefunction
 
     sysm, repr_, psi, base = build_workload(torch)
-    stream_bytes = sum(p.stream_bytes for p in base.PotentialList)
     nnz = sum(int(np.prod(p.source.shape)) for p in base.PotentialList)
     N = sysm.size
     alg_bytes = 8 * nnz + 3 * 16 * N  # real float64 potential elements + src read + dst write
+    dt = 0.01
     if a.profile_matvec:
         t = time_matvec(torch, base, psi, reps=50)
         log("matvec %.3f ms, %.1f GB/s" % (t * 1e3, alg_bytes / t / 1e9))
@@ -142,74 +173,119 @@ def main():
     if a.profile_cayley:
         pc = RadialTwoElectronPreconditioner(psi, sysm)
         prop = CayleyPropagator(psi, base, pc, krylov_basis_size=30, krylov_tolerance=1e-10)
-        prop.SetupStep(0.01)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(a.profile_cayley):
-            prop.AdvanceStep(0.01 * k, 0.01)
-        torch.cuda.synchronize()
-        log("%d Cayley steps %.2f ms/step, %d ops" % (a.profile_cayley, 1e3 * (time.perf_counter() - t0) /
-                                                      a.profile_cayley, prop.GetOperatorCount()))
+        prop.SetupStep(dt)
+        sec, _ = run_steps(torch, prop, psi, 0.0, dt, 0, a.profile_cayley)
+        log("%d Cayley steps %.2f ms/step, %d ops" % (a.profile_cayley, 1e3 * sec / a.profile_cayley,
+                                                      prop.GetOperatorCount()))
         return
+    if a.profile_mixed:
+        prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-10)
+        prop.SetupStep(dt)
+        sec, _ = run_steps(torch, prop, psi, 0.0, dt, 0, a.profile_mixed)
+        log("%d mixed steps %.2f ms/step, last %s" % (a.profile_mixed, 1e3 * sec / a.profile_mixed, prop.GetStats()))
+        return
+
     t_mv = time_matvec(torch, base, psi)
     achieved = alg_bytes / t_mv / 1e9
     log("fused H multiply: %.3f ms  alg %.3f GB  -> %.1f GB/s (%.1f%% of 8 TB/s)" %
         (t_mv * 1e3, alg_bytes / 1e9, achieved, 100 * achieved / HBM_PEAK_GBS))
+    psi0 = psi.data.clone()
 
-    dt = 0.01
+    # 1. plain FP64 path: GMRES(30, 1e-10) with the exact block preconditioner
     t0 = time.time()
     pc = RadialTwoElectronPreconditioner(psi, sysm)
-    prop = CayleyPropagator(psi, base, pc, krylov_basis_size=30, krylov_tolerance=1e-10)
-    prop.SetupStep(dt)
+    plain = CayleyPropagator(psi, base, pc, krylov_basis_size=30, krylov_tolerance=1e-10)
+    plain.SetupStep(dt)
     torch.cuda.synchronize()
-    log("preconditioner factorization + Cayley set-up %.1fs" % (time.time() - t0))
-    t = 0.0
-    ops = []
-    for _ in range(a.warmup):
-        prop.AdvanceStep(t, dt)
-        t += dt
-    torch.cuda.synchronize()
+    log("block preconditioner factorization + Cayley set-up %.1fs" % (time.time() - t0))
     norm0 = psi.GetNorm()
+    ops = []
+    sec_plain, t_end = run_steps(torch, plain, psi, 0.0, dt, a.warmup, a.steps, ops)
+    plain_sps = a.steps / sec_plain
+    ref = psi.data.clone()
+    e_plain = base.GetEnergyExpectationValue(psi, t_end)
+    n_plain = psi.GetNorm()
+    ops_plain = float(np.mean([o["ops"] for o in ops]))
+    log("plain FP64 Cayley: %.2f steps/s (%.2f ms/step), %.1f ops/step, |norm drift| %.2e, E = %.8f" %
+        (plain_sps, 1e3 * sec_plain / a.steps, ops_plain, abs(n_plain - norm0), e_plain))
+
+    # 2. learned correction: pairs from the plain solve under seeded random pulses, trained on the device
+    psi.data.copy_(psi0)
+    kron = KroneckerPreconditioner(psi, sysm)
+    learned = LearnedPreconditioner(kron, hidden=512, seed=1)
+    t0 = time.time()
+    x, tt, ch = generate_training_pairs(learned, base, plain, psi, dt, a.train_trajectories, a.train_steps, seed=2024)
     torch.cuda.synchronize()
-    tstart = time.perf_counter()
-    for _ in range(a.steps):
-        prop.AdvanceStep(t, dt)
-        ops.append(prop.GetOperatorCount())
-        t += dt
+    t_gen = time.time() - t0
+    t0 = time.time()
+    loss = learned.fit(x, ch, tt, epochs=a.train_epochs, batch=256, lr=1e-3, seed=1)
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - tstart
-    norm1 = psi.GetNorm()
-    steps_per_s = a.steps / elapsed
-    energy = base.GetEnergyExpectationValue(psi, t)
-    log("plain FP64 Cayley: %.2f steps/s, %.1f ops/step, |norm drift| %.2e, E = %.6f" %
-        (steps_per_s, float(np.mean(ops)), abs(norm1 - norm0), energy))
+    t_fit = time.time() - t0
+    log("learned: %d training rows from %d plain steps (%.1fs), %d epochs (%.1fs), loss %.3e -> %.3e" %
+        (x.shape[0], a.train_trajectories * a.train_steps, t_gen, a.train_epochs, t_fit,
+         learned.loss_history[0], learned.loss_history[-1]))
+    n_rows = int(x.shape[0])
+    del x, tt, ch
+
+    def mixed_run(with_learned):
+        psi.data.copy_(psi0)
+        prop = MixedCayleyPropagator(psi, base, kron, learned if with_learned else None, fallback=plain,
+                                     tolerance=1e-10)
+        prop.SetupStep(dt)
+        st = []
+        sec, t_e = run_steps(torch, prop, psi, 0.0, dt, a.warmup, a.steps, st)
+        err = (torch.linalg.norm(psi.data - ref) / torch.linalg.norm(ref)).item()
+        e = base.GetEnergyExpectationValue(psi, t_e)
+        nn = psi.GetNorm()
+        agg = {k: float(np.mean([s[k] for s in st])) for k in ("outer", "inner", "fp64_ops", "fp32_ops")}
+        agg["fallbacks"] = int(sum(s["fallback"] for s in st))
+        agg["max_rel_residual"] = float(max(s["rel_residual"] for s in st))
+        return a.steps / sec, sec, err, abs(nn - n_plain), abs(e - e_plain), agg
+
+    kron_sps, kron_sec, kron_err, kron_dn, kron_de, kron_st = mixed_run(False)
+    log("mixed FP32 + Kronecker: %.2f steps/s (%.2f ms/step), err vs plain %.2e, %s" %
+        (kron_sps, 1e3 * kron_sec / a.steps, kron_err, kron_st))
+    lrn_sps, lrn_sec, lrn_err, lrn_dn, lrn_de, lrn_st = mixed_run(True)
+    log("mixed FP32 + learned: %.2f steps/s (%.2f ms/step), err vs plain %.2e, |dnorm| %.1e |dE| %.1e, %s" %
+        (lrn_sps, 1e3 * lrn_sec / a.steps, lrn_err, lrn_dn, lrn_de, lrn_st))
 
     cpu = None
     if not a.no_cpu_baseline:
         t0 = time.time()
-        cpu = cpu_baseline(sysm, float(np.mean(ops)))
+        cpu = cpu_baseline(sysm, ops_plain)
         log("cpu baseline %.4f steps/s (%.1fs)" % (cpu["value"], time.time() - t0))
 
     out = {
         "metric": "CN steps/s, helium rank-3: learned+FP32 vs plain FP64 GPU at 1e-8 err; GB/s",
-        "value": steps_per_s,
+        "value": lrn_sps,
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * elapsed / a.steps,
+        "ms_per_step": 1e3 * lrn_sec / a.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128",
-        "data": "synthetic (helium Hamiltonian generated in-process; independent-particle 1s^2 start)",
-        "config": {"workload": "helium rank-3 Cayley-CN, 42 ch x 60 x 60 B-splines (N=151200), plain FP64, "
-                               "GMRES m=30 tol=1e-10, exact block-diagonal preconditioner, dt=0.01",
-                   "path": "plain_fp64", "gmres_ops_per_step": float(np.mean(ops)),
-                   "norm_drift": abs(norm1 - norm0), "energy": energy},
+        "dtype": "c64 inner / c128 outer",
+        "data": "synthetic (helium Hamiltonian generated in-process; independent-particle 1s^2 start; "
+                "network trained in-process on plain-solve pairs from seeded random pulses)",
+        "config": {"workload": "helium rank-3 Cayley-CN, 42 ch x 60 x 60 B-splines (N=151200), dt=0.01, "
+                               "E0=0.05 w=0.057 4 cycles from t=0; learned+FP32: complex64 flexible GMRES(20) "
+                               "with Kronecker + learned MLP (hidden 512, BF16 MFMA) preconditioner inside FP64 "
+                               "refinement to |b-Ax| <= 1e-10 |b|, fallback to plain after 40 inner iterations",
+                   "plain_fp64_steps_per_s": plain_sps, "speedup_vs_plain_fp64": lrn_sps / plain_sps,
+                   "plain_fp64_ops_per_step": ops_plain,
+                   "wavefunction_error_vs_plain_fp64": lrn_err, "norm_error": lrn_dn, "energy_error": lrn_de,
+                   "learned_stats_per_step": lrn_st,
+                   "fp32_kron_steps_per_s": kron_sps, "fp32_kron_error": kron_err,
+                   "fp32_kron_stats_per_step": kron_st,
+                   "training": {"rows": n_rows, "plain_steps": a.train_trajectories * a.train_steps,
+                                "epochs": a.train_epochs, "gen_s": t_gen, "fit_s": t_fit,
+                                "loss_first": learned.loss_history[0], "loss_last": learned.loss_history[-1]},
+                   "energy": e_plain, "norm_drift_plain": abs(n_plain - norm0)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "tp_dia_kernel (fused H0+R12+dipole multiply)", "ms": t_mv * 1e3,
+                     "kernel": "tp_dia_kernel (fused H0+R12+dipole FP64 multiply)", "ms": t_mv * 1e3,
                      "alg_bytes": alg_bytes},
         "cpu_baseline": cpu,
     }

@@ -332,6 +332,29 @@ int pp_gemm_bf16(int M, int N, int K, const uint16_t* a, long long sam, long lon
                  const uint16_t* b, long long sbk, long long sbn, float* c, pp_stream stream);
 
 /* ------------------------------------------------------------------------------
+ * Factored Hamiltonian multiply.  No reference counterpart for the evaluation order: the
+ * operator is the sum of the tensor potentials (BasisPropagator.MultiplyHamiltonianNoOverlap,
+ * BasisPropagator.py:116-159) written as sum_t f_p(t) (A_t (x) R_t), one angular matrix A_t
+ * (nchan x nchan, host [T][nchan][nchan]) and one radial operator R_t per separable term
+ * (the multipole sum of 1/r12, coupledspherical.cpp:60-111, is 11 such terms), R_t given
+ * diagonal-blocked as host dia[t][i1][jj][d][i2] = element (row (i1, i2), column
+ * (i1-bw+jj, i2-bw+d)).  term_potential[t] indexes the per-potential scalings.  The angular
+ * contraction W_t = f A_t X and the radial products both run on MFMA (FP64 and FP32).
+ * psi: [nchan][n][n].
+ * ------------------------------------------------------------------------------ */
+typedef struct pp_fham pp_fham;
+int pp_fham_create(int nchan, int n, int nterms, int bw, const double* A, const int* term_potential,
+                   const double* dia, int npotentials, pp_fham** out);
+int pp_fham_destroy(pp_fham* f);
+/* dst = addend + alpha * sum_p scalings[p] V_p src (addend may be NULL; src may alias dst) */
+int pp_fham_multiply(const pp_fham* f, const double* scalings, const pp_cplx* src, pp_cplx* dst,
+                     const pp_cplx* addend, pp_cplx alpha, pp_stream stream);
+int pp_fham_multiply_f32(const pp_fham* f, const double* scalings, const pp_cplxf* src, pp_cplxf* dst,
+                         const pp_cplxf* addend, pp_cplx alpha, pp_stream stream);
+/* use the factored multiply inside a Cayley solver (NULL detaches) */
+int pp_cayley_set_factored(pp_cayley* c, const pp_fham* f);
+
+/* ------------------------------------------------------------------------------
  * Mixed-precision Cayley step.  No reference counterpart; solves the system of
  * pp_cayley_step (CayleyPropagator.py:71-150) by FP64 iterative refinement around a
  * complex64 flexible GMRES with the Kronecker preconditioner (plus the learned
@@ -351,6 +374,8 @@ int pp_mixed_create(pp_hamiltonian* ham, const pp_kron* k, int basis_size, doubl
 int pp_mixed_destroy(pp_mixed* m);
 int pp_mixed_set_learned(pp_mixed* m, pp_learned* l);
 int pp_mixed_set_fallback(pp_mixed* m, pp_cayley* plain, int fallback_iterations);
+/* use the factored multiply for the FP64 residuals and the FP32 inner products (NULL detaches) */
+int pp_mixed_set_factored(pp_mixed* m, const pp_fham* f);
 int pp_mixed_step(pp_mixed* m, pp_cplx* psi, double dt, const double* scalings, pp_stream stream);
 int pp_mixed_stats(const pp_mixed* m, int* counts, double* residuals);
 

@@ -98,6 +98,12 @@ SIGNATURES = [
     ("pp_mixed_set_fallback", I, [P, P, I]),
     ("pp_mixed_step", I, [P, P, D, P, P]),
     ("pp_mixed_stats", I, [P, P, P]),
+    ("pp_mixed_set_factored", I, [P, P]),
+    ("pp_cayley_set_factored", I, [P, P]),
+    ("pp_fham_create", I, [I, I, I, I, P, P, P, I, P]),
+    ("pp_fham_destroy", I, [P]),
+    ("pp_fham_multiply", I, [P, P, P, P, P, PPCplx, P]),
+    ("pp_fham_multiply_f32", I, [P, P, P, P, P, PPCplx, P]),
 ]
 
 _lib = None

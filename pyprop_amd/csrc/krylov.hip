@@ -503,7 +503,8 @@ int Cayley::operator_cb(void* ctx, const pp_cplx* in_, pp_cplx* out_, pp_stream 
     // SolverCallback: MultiplySpH(+i dt/2) then left preconditioner (CayleyPropagator.py:145-150)
     vec_copy(c->n, in, c->tmp_s.ptr, s);
     c->ham->repr->overlap(PP_OVERLAP_MULTIPLY, false, c->tmp_s.ptr, s);
-    c->ham->apply(c->scalings.data(), in, out, c->tmp_s.ptr, make_double2(0.0, 0.5 * c->dt), s);
+    if (c->fham) fham_apply64(*c->fham, c->scalings.data(), in, out, c->tmp_s.ptr, make_double2(0.0, 0.5 * c->dt), s);
+    else c->ham->apply(c->scalings.data(), in, out, c->tmp_s.ptr, make_double2(0.0, 0.5 * c->dt), s);
     if (c->pc) precond_solve(*c->pc, out, s);
     else if (c->kron) kron_solve(*c->kron, make_double2(0.0, 0.5 * c->dt), out, s);
     return 0;
@@ -519,7 +520,8 @@ void Cayley::step(double2* psi, double dt_, const double* sc, hipStream_t s) {
   // TempPsi = (S - i dt/2 H) psi   (AdvanceStep, CayleyPropagator.py:81-85)
   vec_copy(n, psi, tmp_s.ptr, s);
   ham->repr->overlap(PP_OVERLAP_MULTIPLY, false, tmp_s.ptr, s);
-  ham->apply(scalings.data(), psi, rhs.ptr, tmp_s.ptr, make_double2(0.0, -0.5 * dt), s);
+  if (fham) fham_apply64(*fham, scalings.data(), psi, rhs.ptr, tmp_s.ptr, make_double2(0.0, -0.5 * dt), s);
+  else ham->apply(scalings.data(), psi, rhs.ptr, tmp_s.ptr, make_double2(0.0, -0.5 * dt), s);
   if (pc) precond_solve(*pc, rhs.ptr, s);
   else if (kron) kron_solve(*kron, make_double2(0.0, 0.5 * dt), rhs.ptr, s);
   gmres.solve(&Cayley::operator_cb, this, rhs.ptr, psi, s);

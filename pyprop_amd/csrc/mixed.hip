@@ -28,6 +28,7 @@
 #include "pp_krylov.h"
 #include "pp_mixed.h"
 #include "pp_learned.h"
+#include "pp_factored.h"
 
 namespace pp {
 
@@ -565,7 +566,8 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   // h = H psi ; b = S psi - i a h ; r = -2 i a h ; x = psi
   vec_copy(n, psi, work.t64.ptr, s);
   R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
-  H.apply(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), s);
+  if (fham) fham->apply64(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), s);
+  else H.apply(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), s);
   ++stats.outer_ops;
   vec_combine(n, make_double2(1.0, 0.0), work.t64.ptr, make_double2(0.0, -a), work.h64.ptr, work.b64.ptr, s);
   vec_combine(n, make_double2(0.0, 0.0), work.h64.ptr, make_double2(0.0, -2.0 * a), work.h64.ptr, work.r64.ptr, s);
@@ -579,11 +581,15 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   auto apply_A32 = [&](const float2* in, float2* out) {
     // out = S in + i a H in
     work.overlap_multiply32(R, in, work.tmp.ptr, work.w.ptr, s);
-    H.apply_f32(scal.data(), in, out, work.tmp.ptr, make_float2(0.f, (float)a), s);
+    if (fham) fham->apply32(scal.data(), in, out, work.tmp.ptr, make_double2(0.0, a), s);
+    else H.apply_f32(scal.data(), in, out, work.tmp.ptr, make_float2(0.f, (float)a), s);
     ++stats.inner_ops;
   };
+  int correction = 0;
   auto apply_P32 = [&](int j, const float2* in, float2* out) {
-    if (j == 0 && learned) {
+    // the learned correction acts on the first direction of the first correction, whose
+    // right-hand side r = -2 i a H psi is the kind of vector it was trained on
+    if (j == 0 && correction == 0 && learned) {
       learned_apply(*learned, K, ka, in, out, s);
       ++stats.learned_applies;
     } else {
@@ -605,11 +611,13 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     hipLaunchKernelGGL(f64_to_f32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, work.r64.ptr, work.rhs.ptr);
     double rel = 0;
     stats.inner_iterations += work.fgmres32(apply_A32, apply_P32, work.rhs.ptr, work.d.ptr, inner_rtol, m, &rel, s);
+    ++correction;
     hipLaunchKernelGGL(add_f32_to_f64_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, work.d.ptr, work.x64.ptr);
     // r = b - (S x + i a H x)
     vec_copy(n, work.x64.ptr, work.t64.ptr, s);
     R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
-    H.apply(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
+    if (fham) fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
+    else H.apply(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
     ++stats.outer_ops;
     const int blocks = reduction_blocks(n);
     work.red.ensure(blocks, 1);
@@ -747,6 +755,24 @@ extern "C" int pp_mixed_set_fallback(pp_mixed* m, pp_cayley* plain, int fallback
     PP_REQUIRE(!plain || plain->c.n == m->c.ham->n, "fallback solver size does not match");
     m->c.fallback = plain ? &plain->c : nullptr;
     m->c.fallback_iterations = fallback_iterations;
+  });
+}
+
+extern "C" int pp_mixed_set_factored(pp_mixed* m, const pp_fham* f) {
+  return guarded([&] {
+    PP_REQUIRE(m, "null argument");
+    PP_REQUIRE(!f || (long long)f->f.nc * f->f.n * f->f.n == m->c.ham->n, "factored operator shape does not match");
+    PP_REQUIRE(!f || f->f.npot == (int)m->c.ham->terms.size(), "factored operator has a different potential count");
+    m->c.fham = f ? &f->f : nullptr;
+  });
+}
+
+extern "C" int pp_cayley_set_factored(pp_cayley* c, const pp_fham* f) {
+  return guarded([&] {
+    PP_REQUIRE(c, "null argument");
+    PP_REQUIRE(!f || (long long)f->f.nc * f->f.n * f->f.n == c->c.n, "factored operator shape does not match");
+    PP_REQUIRE(!f || f->f.npot == (int)c->c.ham->terms.size(), "factored operator has a different potential count");
+    c->c.fham = f ? &f->f : nullptr;
   });
 }
 

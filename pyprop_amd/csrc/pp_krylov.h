@@ -91,6 +91,10 @@ struct Precond {
 void precond_solve(const Precond& pc, double2* psi, hipStream_t s);
 
 struct Kron;  // separable (Kronecker) preconditioner, mixed.hip
+struct FHam;  // factored Hamiltonian, factored.hip
+// dst = addend + alpha * H src through the factored operator (FP64)
+void fham_apply64(const FHam& f, const double* scalings, const double2* src, double2* dst, const double2* addend,
+                  double2 alpha, hipStream_t s);
 // psi <- K^-1 psi with K_c = S x S + a (h_l1 x S + S x h_l2) per channel (FP64)
 void kron_solve(const Kron& k, double2 a, double2* psi, hipStream_t s);
 
@@ -98,6 +102,7 @@ struct Cayley {
   Hamiltonian* ham = nullptr;
   const Precond* pc = nullptr;
   const Kron* kron = nullptr;  // used when pc is null
+  const FHam* fham = nullptr;  // factored multiply in place of the tensor potentials
   long long n = 0;
   Gmres gmres;
   DeviceBuffer<double2> tmp_s, rhs;

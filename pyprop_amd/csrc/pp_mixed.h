@@ -68,6 +68,7 @@ struct MixedWork {
 
 struct MixedCayley {
   Hamiltonian* ham = nullptr;
+  const FHam* fham = nullptr;        // factored multiply for both precisions when set
   const Kron* kron = nullptr;        // FP32 base preconditioner (required)
   Learned* learned = nullptr;        // optional correction on the first direction of each inner solve
   Cayley* fallback = nullptr;        // optional plain FP64 solver (exact block preconditioner)

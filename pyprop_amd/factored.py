@@ -1,0 +1,101 @@
+"""Factored form of a rank-3 (angular x radial x radial) Hamiltonian for the MFMA multiply.
+
+Every potential the set-up builds is a sum of terms a_t(angular pair) x R_t(radial pairs)
+(systems.py; for 1/r12 this is the multipole sum over lambda, the identity the reference's
+CustomPotentialEvaluationR12 evaluates term by term,
+examples/tensor/helium_stabilization/coupledspherical.cpp:60-111).  Hence
+    H psi = sum_t f_p(t) (A_t (x) R_t) psi,
+with A_t the dense nchan x nchan angular matrix of term t and R_t one radial operator
+shared by all channels.  The device multiply (pp_fham_*) contracts the angular index first
+on MFMA (W_t = f A_t X) and then applies every R_t to all channels at once (a banded
+radial product, also on MFMA), instead of streaming one radial block per angular pair as
+the reference storage does (generator.py:302-762).
+
+R_t is stored diagonal-blocked: dia[t][i1][jj][d][i2] is the element
+(row (i1, i2), column (j1, j2)) with j1 = i1 - bw + jj, j2 = i2 - bw + d.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import _native as N
+from . import device as D
+
+
+def factor_potentials(potentials: Sequence, nchan: int, n: int):
+    """(A [T][nchan][nchan], term_potential [T], dia [T][n][2bw+1][2bw+1][n], bw) from
+    TensorPotentialData with (Simp angular) x (Simp banded r1) x (BandNH r2) geometries and
+    separable factors (angular_coeffs, radial_factors).  term_potential[t] is the index of
+    the potential (time scaling) term t belongs to."""
+    A_list, tp_list, R_list = [], [], []
+    bw = None
+    for ip, p in enumerate(potentials):
+        src = getattr(p, "source", p)
+        if src.angular_coeffs is None or src.radial_factors is None:
+            raise ValueError("potential %s has no separable factors" % src.name)
+        g0, g1, g2 = src.geometries
+        if g0.storage_id != "Simp" or g1.storage_id != "Simp" or g2.storage_id != "BandNH":
+            raise ValueError("factored form needs Simp x Simp(banded) x BandNH storage")
+        b2 = (g2.pair_count // n - 1) // 2
+        p1 = g1.pairs
+        b1 = int(np.max(np.abs(p1[:, 0] - p1[:, 1])))
+        b = max(b1, b2)
+        bw = b if bw is None else max(bw, b)
+        for t in range(src.angular_coeffs.shape[0]):
+            A = np.zeros((nchan, nchan))
+            A[g0.pairs[:, 0], g0.pairs[:, 1]] = src.angular_coeffs[t]
+            A_list.append(A)
+            tp_list.append(ip)
+            R_list.append((src.radial_factors[t], g1.pairs, g2.pairs, b2))
+    w = 2 * bw + 1
+    T = len(A_list)
+    dia = np.zeros((T, n, w, w, n))
+    for t, (R, p1, p2, b2) in enumerate(R_list):
+        keep2 = np.abs(p2[:, 0] - p2[:, 1]) <= b2
+        i2 = p2[keep2, 0]
+        j2 = p2[keep2, 1]
+        d = j2 - i2 + bw
+        Rk = R[:, keep2]
+        for q1 in range(p1.shape[0]):
+            i1, j1 = int(p1[q1, 0]), int(p1[q1, 1])
+            jj = j1 - i1 + bw
+            dia[t, i1, jj, d, i2] = Rk[q1]
+    return np.stack(A_list), np.array(tp_list, dtype=np.int32), dia, bw
+
+
+class FactoredHamiltonian:
+    """Device factored Hamiltonian over a BasisPropagator's potentials (same scalings)."""
+
+    def __init__(self, base):
+        shape = base.psi.repr.shape
+        if len(shape) != 3 or shape[1] != shape[2]:
+            raise ValueError("the factored multiply needs psi shaped (channels, n, n)")
+        nchan, n, _ = shape
+        A, tpot, dia, bw = factor_potentials(base.PotentialList, nchan, n)
+        self.nterms = A.shape[0]
+        self.bw = bw
+        self.shape = shape
+        self.base = base
+        A = np.ascontiguousarray(A)
+        dia = np.ascontiguousarray(dia)
+        h = C.c_void_p()
+        N.check(N.load().pp_fham_create(nchan, n, self.nterms, bw, D.hptr(A), D.hptr(tpot), D.hptr(dia),
+                                        len(base.PotentialList), C.byref(h)))
+        self.h = h
+
+    def Multiply(self, src, dst, t, addend=None, alpha=1.0):
+        """dst = addend + alpha * H(t) src (complex128 or complex64 device tensors)."""
+        sc = self.base.scalings(t)
+        s = getattr(src, "data", src)
+        d = getattr(dst, "data", dst)
+        a = None if addend is None else getattr(addend, "data", addend)
+        fn = N.load().pp_fham_multiply_f32 if s.dtype == D.torch.complex64 else N.load().pp_fham_multiply
+        N.check(fn(self.h, D.hptr(sc), D.ptr(s), D.ptr(d), D.ptr(a), N.cplx(alpha), D.stream_handle()))
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and N._lib is not None:
+            N._lib.pp_fham_destroy(self.h)
+            self.h = None

@@ -313,12 +313,15 @@ def generate_training_pairs(learned: LearnedPreconditioner, base, plain, psi0, d
 class MixedCayleyPropagator:
     """Cayley step by FP64 iterative refinement around complex64 flexible GMRES with the
     Kronecker preconditioner (and the learned correction when given).  Same AdvanceStep(t,
-    dt) contract as CayleyPropagator (H at t + |dt|/2)."""
+    dt) contract as CayleyPropagator (H at t + |dt|/2).  With `factored` (a
+    pyprop_amd.factored.FactoredHamiltonian over the same potentials) every multiply uses
+    the factored MFMA form."""
 
     def __init__(self, psi, base, kron: KroneckerPreconditioner, learned: Optional[LearnedPreconditioner] = None,
                  fallback=None, fallback_iterations: int = 40, krylov_basis_size: int = 20,
-                 tolerance: float = 1e-10, inner_tolerance: float = 1e-5, max_outer: int = 8):
+                 tolerance: float = 1e-10, inner_tolerance: float = 1e-5, max_outer: int = 8, factored=None):
         self.psi = psi
+        self.factored = factored
         self.BasePropagator = base
         self.kron = kron
         self.learned = learned
@@ -343,6 +346,8 @@ class MixedCayleyPropagator:
             N.check(N.load().pp_mixed_set_learned(h, self.learned.h))
         if self.fallback is not None:
             N.check(N.load().pp_mixed_set_fallback(h, self.fallback.h, self.fallback_iterations))
+        if self.factored is not None:
+            N.check(N.load().pp_mixed_set_factored(h, self.factored.h))
 
     def SetLearned(self, learned):
         self.learned = learned

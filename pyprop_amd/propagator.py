@@ -305,7 +305,16 @@ class CayleyPropagator:
         self.h = h
         if kron:
             N.check(N.load().pp_cayley_set_kron(h, self.Preconditioner.h))
+        if getattr(self, "factored", None) is not None:
+            N.check(N.load().pp_cayley_set_factored(h, self.factored.h))
         self.dt = dt
+
+    def SetFactoredHamiltonian(self, factored):
+        """Multiply through a pyprop_amd.factored.FactoredHamiltonian (same operator, MFMA
+        evaluation order) instead of the tensor potentials; None restores them."""
+        self.factored = factored
+        if self.h is not None:
+            N.check(N.load().pp_cayley_set_factored(self.h, None if factored is None else factored.h))
 
     def AdvanceStep(self, t, dt):
         if self.h is None:

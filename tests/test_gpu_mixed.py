@@ -393,3 +393,92 @@ def test_helium_full_size_mixed_vs_plain(torch):
     err = rel(psi2.numpy(), psi.numpy())
     say("  wavefunction difference %.2e" % err)
     assert err < 1e-8
+
+
+def test_factored_multiply_vs_dense(torch, rng):
+    """Factored MFMA multiply (FP64 and FP32) against the dense operator, with an addend,
+    a complex alpha and src aliasing dst."""
+    from pyprop_amd.factored import FactoredHamiltonian
+    from tests.test_oracle import dense_operator
+    say("factored multiply vs dense (small helium)")
+    sysm = _small_helium()
+    repr_, psi, base = _device(sysm, torch)
+    base.SetupStep()
+    fh = FactoredHamiltonian(base)
+    t = 123.0
+    sc = base.scalings(t)
+    H, S = dense_operator(sysm, sc)
+    x = crandn(rng, sysm.shape)
+    y0 = crandn(rng, sysm.shape)
+    alpha = 0.3 - 0.7j
+    ref = y0.ravel() + alpha * (H @ x.ravel())
+    src = torch.from_numpy(x).to("cuda")
+    add = torch.from_numpy(y0).to("cuda")
+    dst = torch.empty_like(src)
+    fh.Multiply(src, dst, t, add, alpha)
+    assert rel(dst.cpu().numpy().ravel(), ref) < 1e-12
+    fh.Multiply(src, src, t, add, alpha)  # in place
+    assert rel(src.cpu().numpy().ravel(), ref) < 1e-12
+    s32 = torch.from_numpy(x.astype(np.complex64)).to("cuda")
+    a32 = torch.from_numpy(y0.astype(np.complex64)).to("cuda")
+    d32 = torch.empty_like(s32)
+    fh.Multiply(s32, d32, t, a32, alpha)
+    assert rel(d32.cpu().numpy().ravel(), ref) < 2e-6
+
+
+def test_factored_multiply_full_size_vs_tensor_potentials(torch, rng):
+    """Full helium: factored multiply against the fused tensor-potential multiply (itself
+    pinned to the oracle and to scipy.sparse at 1e-12)."""
+    from pyprop_amd.factored import FactoredHamiltonian
+    from pyprop_amd.propagator import BasisPropagator, CombinedRepresentation, Wavefunction
+    from pyprop_amd.systems import helium_system
+    say("factored multiply full size")
+    sysm = helium_system()
+    repr_ = CombinedRepresentation(sysm.shape, sysm.bases)
+    psi = Wavefunction(repr_)
+    base = BasisPropagator(psi, sysm.potentials)
+    base.SetupStep()
+    fh = FactoredHamiltonian(base)
+    x = torch.from_numpy(crandn(rng, sysm.shape)).to("cuda")
+    ref = torch.empty_like(x)
+    got = torch.empty_like(x)
+    t = 210.0
+    base.MultiplyHamiltonianNoOverlap(x, ref, t)
+    fh.Multiply(x, got, t)
+    e64 = (torch.linalg.norm(got - ref) / torch.linalg.norm(ref)).item()
+    g32 = torch.empty_like(x, dtype=torch.complex64)
+    fh.Multiply(x.to(torch.complex64), g32, t)
+    e32 = (torch.linalg.norm(g32.to(torch.complex128) - ref) / torch.linalg.norm(ref)).item()
+    say("  FP64 %.2e  FP32 %.2e" % (e64, e32))
+    assert e64 < 1e-12 and e32 < 2e-6
+
+
+def test_mixed_cayley_factored_full_size_vs_plain(torch):
+    """Full helium, 10 steps: mixed FP32 + factored multiply vs the plain FP64 path."""
+    from pyprop_amd.factored import FactoredHamiltonian
+    from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
+    from pyprop_amd.propagator import (BasisPropagator, CayleyPropagator, CombinedRepresentation,
+                                       RadialTwoElectronPreconditioner, Wavefunction)
+    from pyprop_amd.systems import helium_initial_state, helium_system
+    say("full-size helium: mixed+factored vs plain FP64, 10 steps")
+    sysm = helium_system()
+    repr_ = CombinedRepresentation(sysm.shape, sysm.bases)
+    psi = Wavefunction(repr_, helium_initial_state(sysm))
+    psi.Normalize()
+    psi2 = psi.Copy()
+    base = BasisPropagator(psi, sysm.potentials)
+    dt = 0.01
+    plain = CayleyPropagator(psi, base, RadialTwoElectronPreconditioner(psi, sysm), krylov_basis_size=30,
+                             krylov_tolerance=1e-10)
+    plain.SetupStep(dt)
+    fh = FactoredHamiltonian(base)
+    mixed = MixedCayleyPropagator(psi2, base, KroneckerPreconditioner(psi2, sysm), tolerance=1e-10, factored=fh)
+    t = 100.0
+    for k in range(10):
+        plain.AdvanceStep(t, dt)
+        mixed.AdvanceStep(t, dt)
+        t += dt
+    say("  last step stats %s" % mixed.GetStats())
+    err = rel(psi2.numpy(), psi.numpy())
+    say("  wavefunction difference %.2e" % err)
+    assert err < 1e-8
