@@ -75,6 +75,40 @@ def time_matvec(torch, base, psi, reps=20):
     return e0.elapsed_time(e1) / reps / 1e3
 
 
+def time_factored(torch, fh, psi, dtype, reps=50):
+    """Average duration of the factored multiply (angular + radial launch) in `dtype`."""
+    src = psi.data.to(dtype).clone()
+    dst = torch.zeros_like(src)
+    for _ in range(3):
+        fh.Multiply(src, dst, 100.0)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        fh.Multiply(src, dst, 100.0)
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps / 1e3
+
+
+def factored_flops(fh, nc, n):
+    """(issued MFMA flops incl. padding, useful flops) of one factored multiply."""
+    T, bw = fh.nterms, fh.bw
+    w = 2 * bw + 1
+    kp = (nc + 3) // 4 * 4
+    nq = 2 * n * n
+    m = T * nc
+    ang_issued = math.ceil(nq / 64) * math.ceil(m / 64) * 4 * (kp // 4) * 4 * 2048
+    pairs1 = sum(min(n, i + bw + 1) - max(0, i - bw) for i in range(n))
+    ks = (16 + 2 * bw + 3) // 4
+    rad_issued = T * pairs1 * ks * math.ceil(n / 16) * math.ceil(nc / 8) * 2048
+    nnz_r = pairs1 * sum(min(n, i + bw + 1) - max(0, i - bw) for i in range(n))
+    useful = 2 * m * nc * nq + T * nnz_r * 4 * nc
+    return ang_issued + rad_issued, useful
+
+
 def cpu_baseline(sysm, n_ops_per_step, sample_frac=0.1):
     """The oracle (sequential C++ restatement of the reference loops) timed on the host:
     the tensor-potential multiply on the first `sample_frac` of every potential's angular
@@ -147,6 +181,7 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--profile-matvec", action="store_true", help="only run the matvec loop (for rocprof)")
     ap.add_argument("--profile-cayley", type=int, default=0, help="only run this many plain Cayley steps (for rocprof)")
+    ap.add_argument("--profile-factored", action="store_true", help="only time the factored multiply (for rocprof)")
     ap.add_argument("--profile-mixed", type=int, default=0,
                     help="only run this many mixed-precision steps with the Kronecker preconditioner (for rocprof)")
     a = ap.parse_args()
@@ -159,6 +194,7 @@ def main():
     assert torch.cuda.is_available(), "bench needs a GPU"
     from pyprop_amd.mixed import (KroneckerPreconditioner, LearnedPreconditioner, MixedCayleyPropagator,
                                   generate_training_pairs)
+    from pyprop_amd.factored import FactoredHamiltonian
     from pyprop_amd.propagator import CayleyPropagator, RadialTwoElectronPreconditioner, Wavefunction
 
     sysm, repr_, psi, base = build_workload(torch)
@@ -178,8 +214,15 @@ def main():
         log("%d Cayley steps %.2f ms/step, %d ops" % (a.profile_cayley, 1e3 * sec / a.profile_cayley,
                                                       prop.GetOperatorCount()))
         return
+    if a.profile_factored:
+        fh = FactoredHamiltonian(base)
+        for dtype in (torch.complex128, torch.complex64):
+            t = time_factored(torch, fh, psi, dtype, reps=20)
+            log("factored %s %.1f us" % (dtype, t * 1e6))
+        return
     if a.profile_mixed:
-        prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-10)
+        prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-10,
+                                     factored=FactoredHamiltonian(base))
         prop.SetupStep(dt)
         sec, _ = run_steps(torch, prop, psi, 0.0, dt, 0, a.profile_mixed)
         log("%d mixed steps %.2f ms/step, last %s" % (a.profile_mixed, 1e3 * sec / a.profile_mixed, prop.GetStats()))
@@ -189,6 +232,16 @@ def main():
     achieved = alg_bytes / t_mv / 1e9
     log("fused H multiply: %.3f ms  alg %.3f GB  -> %.1f GB/s (%.1f%% of 8 TB/s)" %
         (t_mv * 1e3, alg_bytes / 1e9, achieved, 100 * achieved / HBM_PEAK_GBS))
+    fh = FactoredHamiltonian(base)
+    nc, n, _ = sysm.shape
+    t_f64 = time_factored(torch, fh, psi, torch.complex128)
+    t_f32 = time_factored(torch, fh, psi, torch.complex64)
+    issued, useful = factored_flops(fh, nc, n)
+    fact = {"fp64_ms": t_f64 * 1e3, "fp32_ms": t_f32 * 1e3, "fp64_matvecs_per_s": 1 / t_f64,
+            "fp32_matvecs_per_s": 1 / t_f32, "mfma_flops_issued": issued, "useful_flops": useful,
+            "fp64_mfma_util": issued / t_f64 / 78.6e12, "fp32_mfma_util": issued / t_f32 / 157.3e12}
+    log("factored multiply: FP64 %.1f us (%.1f%% of FP64 MFMA peak), FP32 %.1f us (%.1f%% of FP32 MFMA peak)" %
+        (t_f64 * 1e6, 100 * fact["fp64_mfma_util"], t_f32 * 1e6, 100 * fact["fp32_mfma_util"]))
     psi0 = psi.data.clone()
 
     # 1. plain FP64 path: GMRES(30, 1e-10) with the exact block preconditioner
@@ -208,6 +261,14 @@ def main():
     ops_plain = float(np.mean([o["ops"] for o in ops]))
     log("plain FP64 Cayley: %.2f steps/s (%.2f ms/step), %.1f ops/step, |norm drift| %.2e, E = %.8f" %
         (plain_sps, 1e3 * sec_plain / a.steps, ops_plain, abs(n_plain - norm0), e_plain))
+    # the same plain FP64 solver with the factored multiply in place of the tensor potentials
+    psi.data.copy_(psi0)
+    plain.SetFactoredHamiltonian(fh)
+    sec_pf, _ = run_steps(torch, plain, psi, 0.0, dt, a.warmup, a.steps)
+    plainf_sps = a.steps / sec_pf
+    plainf_err = (torch.linalg.norm(psi.data - ref) / torch.linalg.norm(ref)).item()
+    log("plain FP64 Cayley, factored multiply: %.2f steps/s (%.2f ms/step), err vs plain %.2e" %
+        (plainf_sps, 1e3 * sec_pf / a.steps, plainf_err))
 
     # 2. learned correction: pairs from the plain solve under seeded random pulses, trained on the device
     psi.data.copy_(psi0)
@@ -230,7 +291,7 @@ def main():
     def mixed_run(with_learned):
         psi.data.copy_(psi0)
         prop = MixedCayleyPropagator(psi, base, kron, learned if with_learned else None, fallback=plain,
-                                     tolerance=1e-10)
+                                     tolerance=1e-10, factored=fh)
         prop.SetupStep(dt)
         st = []
         sec, t_e = run_steps(torch, prop, psi, 0.0, dt, a.warmup, a.steps, st)
@@ -272,8 +333,12 @@ def main():
         "config": {"workload": "helium rank-3 Cayley-CN, 42 ch x 60 x 60 B-splines (N=151200), dt=0.01, "
                                "E0=0.05 w=0.057 4 cycles from t=0; learned+FP32: complex64 flexible GMRES(20) "
                                "with Kronecker + learned MLP (hidden 512, BF16 MFMA) preconditioner inside FP64 "
-                               "refinement to |b-Ax| <= 1e-10 |b|, fallback to plain after 40 inner iterations",
+                               "refinement to |b-Ax| <= 1e-10 |b|, fallback to plain after 40 inner iterations; "
+                               "all multiplies of the FP32 paths through the factored MFMA operator",
                    "plain_fp64_steps_per_s": plain_sps, "speedup_vs_plain_fp64": lrn_sps / plain_sps,
+                   "plain_fp64_factored_steps_per_s": plainf_sps,
+                   "speedup_vs_plain_fp64_factored": lrn_sps / plainf_sps,
+                   "factored_multiply": fact,
                    "plain_fp64_ops_per_step": ops_plain,
                    "wavefunction_error_vs_plain_fp64": lrn_err, "norm_error": lrn_dn, "energy_error": lrn_de,
                    "learned_stats_per_step": lrn_st,

@@ -84,19 +84,30 @@ __global__ __launch_bounds__(kRedThreads) void block_dot_kernel(long long n, int
   }
 }
 
-// out[j] = sum_b partial[b * width + j] in block order
-__global__ void finish_kernel(const double2* __restrict__ partial, int blocks, int width, int count,
-                              double2* __restrict__ out, const int* cond) {
+// out[j] = sum_b partial[b * width + j]: one 256-thread workgroup per column j, strided loads
+// then a tree (fixed order, so results are reproducible)
+__global__ __launch_bounds__(256) void finish_kernel(const double2* __restrict__ partial, int blocks, int width,
+                                                     int count, double2* __restrict__ out, const int* cond) {
+  __shared__ double2 sh[256];
   if (cond && *cond == 0) return;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.x;
   if (j >= count) return;
   double2 s = make_double2(0, 0);
-  for (int b = 0; b < blocks; ++b) {
+  for (int b = threadIdx.x; b < blocks; b += 256) {
     const double2 p = partial[(long long)b * width + j];
     s.x += p.x;
     s.y += p.y;
   }
-  out[j] = s;
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      sh[threadIdx.x].x += sh[threadIdx.x + off].x;
+      sh[threadIdx.x].y += sh[threadIdx.x + off].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = sh[0];
 }
 
 void block_dot_if(long long n, int m, const double2* V, long long ld, const double2* w, double2* out_dev,
@@ -106,8 +117,8 @@ void block_dot_if(long long n, int m, const double2* V, long long ld, const doub
   red.ensure(blocks, width);
   hipLaunchKernelGGL(block_dot_kernel, dim3(blocks), dim3(kRedThreads), 0, s, n, m, V, ld, w, red.partial.ptr, width,
                      cond);
-  hipLaunchKernelGGL(finish_kernel, dim3((width + 63) / 64), dim3(64), 0, s, red.partial.ptr, blocks, width, width,
-                     out_dev, cond);
+  hipLaunchKernelGGL(finish_kernel, dim3(width), dim3(256), 0, s, red.partial.ptr, blocks, width, width, out_dev,
+                     cond);
   PP_HIP(hipGetLastError());
 }
 
@@ -151,7 +162,7 @@ void block_update_if(long long n, int m, const double2* V, long long ld, const d
   red.ensure(blocks, 1);
   hipLaunchKernelGGL(block_update_kernel, dim3(blocks), dim3(kRedThreads), 0, s, n, m, V, ld, h_dev, w, red.partial.ptr,
                      cond);
-  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, s, red.partial.ptr, blocks, 1, 1, norm2_dev, cond);
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, red.partial.ptr, blocks, 1, 1, norm2_dev, cond);
   PP_HIP(hipGetLastError());
 }
 
@@ -166,7 +177,7 @@ void vec_dot(long long n, const double2* x, const double2* y, double2* out_dev, 
   red.ensure(blocks, 2);
   hipLaunchKernelGGL(block_dot_kernel, dim3(blocks), dim3(kRedThreads), 0, s, n, 1, x, n, y, red.partial.ptr, 2,
                      (const int*)nullptr);
-  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, s, red.partial.ptr, blocks, 2, 1, out_dev, (const int*)nullptr);
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, red.partial.ptr, blocks, 2, 1, out_dev, (const int*)nullptr);
   PP_HIP(hipGetLastError());
 }
 
@@ -175,7 +186,7 @@ void vec_norm2(long long n, const double2* x, double2* out_dev, Reducer& red, hi
   red.ensure(blocks, 1);
   hipLaunchKernelGGL(block_dot_kernel, dim3(blocks), dim3(kRedThreads), 0, s, n, 0, x, n, x, red.partial.ptr, 1,
                      (const int*)nullptr);
-  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, s, red.partial.ptr, blocks, 1, 1, out_dev, (const int*)nullptr);
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, red.partial.ptr, blocks, 1, 1, out_dev, (const int*)nullptr);
   PP_HIP(hipGetLastError());
 }
 

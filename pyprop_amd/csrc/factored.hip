@@ -3,17 +3,18 @@
 //
 //   H X = sum_t f_t (A_t (x) R_t) X,   X: [nc][n][n] complex
 //
-// Stage 1, angular contraction (the multipole/dipole angular coupling as one GEMM):
-//   W[t][c'][q] = f_t sum_c A_t[c'][c] X[c][q]       q over the 2 n^2 reals of a channel
-//   M = T nc rows, K = nc, N = 2 n^2; v_mfma_{f32,f64}_16x16x4 with operands read straight
-//   from global memory (A is tiny and cache-resident, X rows are read 16 floats at a time).
-// Stage 2, radial products, all terms accumulated in registers:
-//   Y[c][i1][i2] = sum_t sum_jj sum_d R_t[i1][jj][d][i2] W[t][c][i1-bw+jj][i2-bw+d]
-//   One wave per (i1, 16 consecutive i2, 8 channels): per (t, jj) it multiplies the
-//   16 x (16+2bw) window of W (rows = 8 channels x re/im) by the dense band tile
+// Stage 1, radial products (one per term, all channels; the input X is L2-resident):
+//   Z[t][c][i1][i2] = sum_jj sum_d R_t[i1][jj][d][i2] X[c][i1-bw+jj][i2-bw+d]
+//   One wave per (t, i1, 16 consecutive i2, 8 channels): per jj it multiplies the
+//   16 x (16+2bw) window of X (rows = 8 channels x re/im) by the dense band tile
 //   B[k][m] = R_t[i1][jj][k-m][I+m] (zero outside the band) on MFMA, K = 16+2bw.
-//   Epilogue: out = addend + alpha * Y (complex alpha).  src may alias dst or addend: the
-//   second stage reads only W and the addend.
+//   Workgroups are renumbered so that the waves sharing (t, i1) -- and so one radial block --
+//   run on one XCD and share its L2.
+// Stage 2, angular contraction of all terms at once (the multipole and dipole angular
+// coupling as one GEMM, K = T nc):
+//   Y[c'][p] = addend + alpha sum_{t,c} f_t A_t[c'][c] Z[t][c][p]
+//   real A, complex Z: two accumulators (re, im) so each lane holds whole complex outputs.
+// Both stages use v_mfma_{f32,f64}_16x16x4.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -53,112 +54,166 @@ struct FScales {
   T s[kMaxFTerms];
 };
 
-// W[m][q] = scale[m / nc] * sum_k A[m][k] X[k][q]; A is [M][Kp] zero padded, 64 x 64 tile per
-// workgroup, four waves of 32 x 32 (2 x 2 MFMA tiles).
+// Channel-minor copy of the input: Xt[j1][j2][c][part] (channels padded to ncp with zeros),
+// so that the 16 operand rows of a wave (8 channels x re/im) at one j2 are 16 consecutive
+// values.
 template <class T>
-__global__ __launch_bounds__(256) void fham_angular_kernel(int M, int nc, int Kp, int Nq, const T* __restrict__ A,
-                                                           const T* __restrict__ X, FScales<T> sc, T* __restrict__ W) {
-  typedef typename MfmaT<T>::V4 V4;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m0 = blockIdx.y * 64 + (wave >> 1) * 32, n0 = blockIdx.x * 64 + (wave & 1) * 32;
-  V4 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = V4{0, 0, 0, 0};
-  const int lr = lane & 15, lk = lane >> 4;
-  for (int k0 = 0; k0 < Kp; k0 += 4) {
-    const int k = k0 + lk;
-    T a[2], b[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int row = m0 + i * 16 + lr;
-      a[i] = row < M ? A[(long long)row * Kp + k] : T(0);
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = n0 + j * 16 + lr;
-      b[j] = (k < nc && col < Nq) ? X[(long long)k * Nq + col] : T(0);
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = MfmaT<T>::mma(a[i], b[j], acc[i][j]);
+__global__ void fham_transpose_kernel(int nc, int ncp, int n, const T* __restrict__ X, T* __restrict__ Xt) {
+  const long long total = (long long)n * n * ncp * 2;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int part = (int)(i & 1);
+    const int c = (int)((i >> 1) % ncp);
+    const long long pos = (i >> 1) / ncp;  // j1 * n + j2
+    Xt[i] = c < nc ? X[((long long)c * n * n + pos) * 2 + part] : T(0);
   }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + i * 16 + (MfmaT<T>::kRowMajorOut ? lk * 4 + r : r * 4 + lk);
-        const int col = n0 + j * 16 + lr;
-        if (row < M && col < Nq) W[(long long)row * Nq + col] = sc.s[row / nc] * acc[i][j][r];
-      }
 }
 
-// One wave per (i2 tile of 16, channel group of 8, i1).  W: [T][nc][n][n] complex (as reals),
-// dia: [T][n][w][w][n].
-template <class T>
-__global__ __launch_bounds__(64) void fham_radial_kernel(int nc, int n, int bw, int nterms, int ngroups,
-                                                         const T* __restrict__ W, const T* __restrict__ dia,
-                                                         const typename MfmaT<T>::C2* __restrict__ addend,
-                                                         typename MfmaT<T>::C2 alpha,
-                                                         typename MfmaT<T>::C2* __restrict__ out) {
+// Stage 1.  Z: [T][nc][n][n] complex (as reals).  BT: the band tiles in MFMA lane order,
+// [T][n][w][tiles][KS][64] (element B[k][m] = R_t[i1][jj][k-m][I+m] at k = 4 ks + lane/16,
+// m = lane % 16), so every operand load is one coalesced wave-wide read.
+template <class T, int BW>
+__global__ __launch_bounds__(64) void fham_radial_kernel(int nc, int ncp, int n, int nterms, int ngroups, int tiles,
+                                                         long long nblocks, const T* __restrict__ Xt,
+                                                         const T* __restrict__ BT, T* __restrict__ Z) {
   typedef typename MfmaT<T>::V4 V4;
-  typedef typename MfmaT<T>::C2 C2;
+  constexpr int w = 2 * BW + 1;
+  constexpr int KW = 16 + 2 * BW;
+  constexpr int KS = (KW + 3) / 4;
+  // XCD-aware renumbering: hardware block h runs on XCD h % 8; logical blocks
+  // [x * per, (x+1) * per) go to XCD x, so consecutive logical blocks share an L2.
+  const long long per = (nblocks + 7) / 8;
+  const long long hw = blockIdx.x;
+  const long long L = (hw % 8) * per + hw / 8;
+  if (L >= nblocks) return;
+  const int grp = (int)(L % ngroups);
+  const int tile = (int)((L / ngroups) % tiles);
+  const int i1 = (int)((L / ((long long)ngroups * tiles)) % n);
+  const int t = (int)(L / ((long long)ngroups * tiles * n));
   const int lane = threadIdx.x;
-  const int tile = blockIdx.x / ngroups, grp = blockIdx.x % ngroups;
-  const int i1 = blockIdx.y;
   const int I = tile * 16;
-  const int w = 2 * bw + 1;
-  const int KW = 16 + 2 * bw;
-  const int KS = (KW + 3) >> 2;
   const int lr = lane & 15, lk = lane >> 4;
   // A operand row lr of the wave's 16 rows (8 channels x re/im).  FP32: row = 2 c + part;
   // FP64: row = 8 part + c, so that each lane's four results hold both parts of two
   // channels in either result layout.
-  const int ca = grp * 8 + (MfmaT<T>::kRowMajorOut ? (lr >> 1) : (lr & 7));
+  const int cl = MfmaT<T>::kRowMajorOut ? (lr >> 1) : (lr & 7);
   const int part = MfmaT<T>::kRowMajorOut ? (lr & 1) : (lr >> 3);
-  const bool row_ok = ca < nc;
-  const long long nn = (long long)n * n;
-  V4 acc = V4{0, 0, 0, 0};
-  for (int t = 0; t < nterms; ++t) {
-    const T* Wt = W + ((long long)t * nc + (row_ok ? ca : 0)) * nn * 2;
-    const T* Dt = dia + (long long)t * n * w * w * n + (long long)i1 * w * w * n;
-    for (int jj = 0; jj < w; ++jj) {
-      const int j1 = i1 - bw + jj;
-      if (j1 < 0 || j1 >= n) continue;  // uniform over the wave
-      const T* Wrow = Wt + (long long)j1 * n * 2;
-      const T* Dj = Dt + (long long)jj * w * n;
+  const int xoff = (grp * 8 + cl) * 2 + part;
+  const int i2 = I + lr;
+  const T* Bt = BT + (((long long)t * n + i1) * w * tiles + tile) * KS * 64 + lane;
+  const int jlo = max(0, BW - i1), jhi = min(2 * BW, n - 1 - i1 + BW);
+  // jj in batches of JB: all 2 KS JB operand loads of a batch are issued before its MFMAs
+  constexpr int JB = sizeof(T) == 8 ? 3 : 5;
+  V4 acc0 = V4{0, 0, 0, 0}, acc1 = V4{0, 0, 0, 0};
+  for (int jb = jlo; jb <= jhi; jb += JB) {
+    T av[JB][KS], bv[JB][KS];
+#pragma unroll
+    for (int u = 0; u < JB; ++u) {
+      const int jj = jb + u;
+      const bool jok = jj <= jhi;
+      const int j1 = i1 - BW + jj;
+      const T* Xrow = Xt + (long long)j1 * n * ncp * 2 + xoff;
+      const T* Bj = Bt + (long long)jj * tiles * KS * 64;
+#pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
         const int k = ks * 4 + lk;
-        const int j2 = I - bw + k;
-        const T a = (row_ok && k < KW && j2 >= 0 && j2 < n) ? Wrow[2 * j2 + part] : T(0);
-        const int d = k - lr;  // B[k][m] with m = lr
-        const int i2 = I + lr;
-        const T b = (d >= 0 && d < w && i2 < n) ? Dj[(long long)d * n + i2] : T(0);
-        acc = MfmaT<T>::mma(a, b, acc);
+        const int j2 = I - BW + k;
+        av[u][ks] = (jok && k < KW && j2 >= 0 && j2 < n) ? Xrow[(long long)j2 * ncp * 2] : T(0);
+        bv[u][ks] = jok ? Bj[ks * 64] : T(0);
       }
     }
+#pragma unroll
+    for (int u = 0; u < JB; ++u)
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        if ((u * KS + ks) & 1) acc1 = MfmaT<T>::mma(av[u][ks], bv[u][ks], acc1);
+        else acc0 = MfmaT<T>::mma(av[u][ks], bv[u][ks], acc0);
+      }
   }
+  const V4 acc = acc0 + acc1;
+  if (i2 >= n) return;
   // FP32: registers r hold rows 4 lk + r = channels 2 lk, 2 lk + 1 as (re, im, re, im);
   // FP64: registers r hold rows 4 r + lk = channels lk, lk + 4 as (re, re, im, im).
-  const int i2 = I + lr;
-  if (i2 >= n) return;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int c = grp * 8 + (MfmaT<T>::kRowMajorOut ? lk * 2 + q : lk + 4 * q);
     if (c >= nc) continue;
-    const long long idx = ((long long)c * n + i1) * n + i2;
-    const T vr = MfmaT<T>::kRowMajorOut ? acc[2 * q] : acc[q];
-    const T vi = MfmaT<T>::kRowMajorOut ? acc[2 * q + 1] : acc[q + 2];
+    T* z = Z + ((((long long)t * nc + c) * n + i1) * n + i2) * 2;
+    z[0] = MfmaT<T>::kRowMajorOut ? acc[2 * q] : acc[q];
+    z[1] = MfmaT<T>::kRowMajorOut ? acc[2 * q + 1] : acc[q + 2];
+  }
+}
+
+// Stage 2.  Workgroup of kAngWaves waves per (16 rows c', 16 complex positions p); the waves
+// split K = T nc and reduce through LDS.  A: [Mp][K] (row c', column t*nc + c).
+constexpr int kAngWaves = 16;
+constexpr int kAngBatch = 12;
+template <class T>
+__global__ __launch_bounds__(64 * kAngWaves) void fham_angular_kernel(int nc, int K, int Kreal, long long P,
+                                                                      const T* __restrict__ A,
+                                                                      const typename MfmaT<T>::C2* __restrict__ Z,
+                                                                      FScales<T> sc,
+                                                                      const typename MfmaT<T>::C2* addend,
+                                                                      typename MfmaT<T>::C2 alpha,
+                                                                      typename MfmaT<T>::C2* out) {
+  typedef typename MfmaT<T>::V4 V4;
+  typedef typename MfmaT<T>::C2 C2;
+  __shared__ V4 red[kAngWaves - 1][2][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int m0 = blockIdx.y * 16;
+  const long long p0 = (long long)blockIdx.x * 16;
+  const int nks = K >> 2;
+  const int per = (nks + kAngWaves - 1) / kAngWaves;
+  const int ks0 = wave * per, ks1 = min(nks, ks0 + per);
+  const long long pc = p0 + lr;
+  const bool col_ok = pc < P;
+  V4 are = V4{0, 0, 0, 0}, aim = V4{0, 0, 0, 0};
+  for (int kb = ks0; kb < ks1; kb += kAngBatch) {
+    T a[kAngBatch];
+    C2 z[kAngBatch];
+#pragma unroll
+    for (int q = 0; q < kAngBatch; ++q) {
+      const int ks = kb + q;
+      const int k = ks * 4 + lk;
+      const bool ok = ks < ks1 && k < Kreal;
+      a[q] = ok ? A[(long long)(m0 + lr) * K + k] * sc.s[k / nc] : T(0);
+      if (ok && col_ok) z[q] = Z[(long long)k * P + pc];
+      else {
+        z[q].x = 0;
+        z[q].y = 0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kAngBatch; ++q) {
+      are = MfmaT<T>::mma(a[q], z[q].x, are);
+      aim = MfmaT<T>::mma(a[q], z[q].y, aim);
+    }
+  }
+  if (wave > 0) {
+    red[wave - 1][0][lane] = are;
+    red[wave - 1][1][lane] = aim;
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int q = 0; q < kAngWaves - 1; ++q) {
+    are += red[q][0][lane];
+    aim += red[q][1][lane];
+  }
+  if (!col_ok) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int c = m0 + (MfmaT<T>::kRowMajorOut ? lk * 4 + r : r * 4 + lk);
+    if (c >= nc) continue;
+    const long long idx = (long long)c * P + pc;
     C2 o;
     if (addend) o = addend[idx];
     else {
       o.x = 0;
       o.y = 0;
     }
+    const T vr = are[r], vi = aim[r];
     o.x += alpha.x * vr - alpha.y * vi;
     o.y += alpha.x * vi + alpha.y * vr;
     out[idx] = o;
@@ -168,22 +223,39 @@ __global__ __launch_bounds__(64) void fham_radial_kernel(int nc, int n, int bw, 
 template <class T>
 static void fham_run(const FHam& F, const double* scalings, const typename MfmaT<T>::C2* src,
                      typename MfmaT<T>::C2* dst, const typename MfmaT<T>::C2* addend, double2 alpha,
-                     const DeviceBuffer<T>& Ab, const DeviceBuffer<T>& Db, DeviceBuffer<T>& Wb, hipStream_t s) {
+                     const DeviceBuffer<T>& Ab, const DeviceBuffer<T>& Db, DeviceBuffer<T>& Zb, DeviceBuffer<T>& Xb,
+                     hipStream_t s) {
   const int nc = F.nc, n = F.n, nterms = F.nterms;
-  const long long Nq = 2LL * n * n;
-  if (Wb.count < (size_t)nterms * nc * Nq) Wb.resize((size_t)nterms * nc * Nq);
+  const long long P = (long long)n * n;
+  if (Zb.count < (size_t)nterms * nc * P * 2) Zb.resize((size_t)nterms * nc * P * 2);
+  const int ngroups = (nc + 7) / 8;
+  const int ncp = ngroups * 8;
+  if (Xb.count < (size_t)P * ncp * 2) Xb.resize((size_t)P * ncp * 2);
+  hipLaunchKernelGGL(fham_transpose_kernel<T>, dim3((unsigned)std::min<long long>(4096, ceil_div(P * ncp * 2, 256))),
+                     dim3(256), 0, s, nc, ncp, n, reinterpret_cast<const T*>(src), Xb.ptr);
+  const int tiles = (n + 15) / 16;
+  const long long nblocks = (long long)nterms * n * tiles * ngroups;
+  const long long hw = (nblocks + 7) / 8 * 8;
+  switch (F.bw) {
+#define PP_RAD(B)                                                                                                    \
+  case B:                                                                                                            \
+    hipLaunchKernelGGL((fham_radial_kernel<T, B>), dim3((unsigned)hw), dim3(64), 0, s, nc, ncp, n, nterms, ngroups,   \
+                       tiles, nblocks, Xb.ptr, Db.ptr, Zb.ptr);                                                       \
+    break;
+    PP_RAD(1) PP_RAD(2) PP_RAD(3) PP_RAD(4) PP_RAD(5) PP_RAD(6) PP_RAD(7) PP_RAD(8)
+#undef PP_RAD
+    default:
+      throw Error(PP_ERR_INVALID, "factored multiply supports band half-widths 1..8");
+  }
   FScales<T> sc;
   for (int t = 0; t < kMaxFTerms; ++t) sc.s[t] = t < nterms ? (T)scalings[F.term_pot[t]] : T(0);
-  const int M = nterms * nc;
-  hipLaunchKernelGGL(fham_angular_kernel<T>, dim3((int)ceil_div(Nq, 64), (int)ceil_div(M, 64)), dim3(256), 0, s, M,
-                     nc, F.Kp, (int)Nq, Ab.ptr, reinterpret_cast<const T*>(src), sc, Wb.ptr);
-  const int ngroups = (nc + 7) / 8;
-  const int tiles = (n + 15) / 16;
   typename MfmaT<T>::C2 a;
   a.x = (T)alpha.x;
   a.y = (T)alpha.y;
-  hipLaunchKernelGGL(fham_radial_kernel<T>, dim3(tiles * ngroups, n), dim3(64), 0, s, nc, n, F.bw, nterms, ngroups,
-                     Wb.ptr, Db.ptr, addend, a, dst);
+  hipLaunchKernelGGL(fham_angular_kernel<T>, dim3((unsigned)ceil_div(P, 16), (unsigned)ceil_div(nc, 16)),
+                     dim3(64 * kAngWaves),
+                     0, s, nc, F.Kp, nterms * nc, P, Ab.ptr, reinterpret_cast<const typename MfmaT<T>::C2*>(Zb.ptr), sc, addend, a,
+                     dst);
   PP_HIP(hipGetLastError());
 }
 
@@ -194,12 +266,12 @@ void fham_apply64(const FHam& f, const double* scalings, const double2* src, dou
 
 void FHam::apply64(const double* scalings, const double2* src, double2* dst, const double2* addend, double2 alpha,
                    hipStream_t s) const {
-  fham_run<double>(*this, scalings, src, dst, addend, alpha, A64, dia64, W64, s);
+  fham_run<double>(*this, scalings, src, dst, addend, alpha, A64, bt64, W64, X64, s);
 }
 
 void FHam::apply32(const double* scalings, const float2* src, float2* dst, const float2* addend, double2 alpha,
                    hipStream_t s) const {
-  fham_run<float>(*this, scalings, src, dst, addend, alpha, A32, dia32, W32, s);
+  fham_run<float>(*this, scalings, src, dst, addend, alpha, A32, bt32, W32, X32, s);
 }
 
 }  // namespace pp
@@ -210,7 +282,8 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
                               const double* dia, int npotentials, pp_fham** out) {
   return guarded([&] {
     PP_REQUIRE(out && A && term_potential && dia, "null argument");
-    PP_REQUIRE(nchan > 0 && n > 0 && bw >= 0 && 2 * bw + 1 <= n, "invalid factored-operator shape");
+    PP_REQUIRE(nchan > 0 && nchan <= 64 && n > 0 && bw >= 1 && bw <= 8 && 2 * bw + 1 <= n,
+               "factored operator supports 1..64 channels and band half-widths 1..8");
     PP_REQUIRE(nterms > 0 && nterms <= kMaxFTerms, "factored operator supports 1..32 terms");
     for (int t = 0; t < nterms; ++t)
       PP_REQUIRE(term_potential[t] >= 0 && term_potential[t] < npotentials, "term potential index out of range");
@@ -221,22 +294,40 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
     F.nterms = nterms;
     F.bw = bw;
     F.npot = npotentials;
-    F.Kp = (nchan + 3) / 4 * 4;
+    F.Kp = (nterms * nchan + 3) / 4 * 4;  // angular GEMM K, zero-padded to the MFMA K step
     F.term_pot.assign(term_potential, term_potential + nterms);
-    const size_t M = (size_t)nterms * nchan;
-    std::vector<double> a(M * F.Kp, 0.0);
-    for (size_t r = 0; r < M; ++r)
-      for (int c = 0; c < nchan; ++c) a[r * F.Kp + c] = A[r * nchan + c];
+    const int Mp = (nchan + 15) / 16 * 16;
+    // A[c'][t*nc + c] = A_t[c'][c], rows padded to a multiple of 16 with zeros
+    std::vector<double> a((size_t)Mp * F.Kp, 0.0);
+    for (int t = 0; t < nterms; ++t)
+      for (int r = 0; r < nchan; ++r)
+        for (int c = 0; c < nchan; ++c) a[(size_t)r * F.Kp + (size_t)t * nchan + c] = A[((size_t)t * nchan + r) * nchan + c];
     const int w = 2 * bw + 1;
-    const size_t nd = (size_t)nterms * n * w * w * n;
+    const int tiles = (n + 15) / 16;
+    const int KS = (16 + 2 * bw + 3) / 4;
+    // band tiles in MFMA lane order: [t][i1][jj][tile][ks][lane]
+    const size_t nbt = (size_t)nterms * n * w * tiles * KS * 64;
+    std::vector<double> bt(nbt, 0.0);
+    for (int t = 0; t < nterms; ++t)
+      for (int i1 = 0; i1 < n; ++i1)
+        for (int jj = 0; jj < w; ++jj)
+          for (int tile = 0; tile < tiles; ++tile)
+            for (int ks = 0; ks < KS; ++ks)
+              for (int lane = 0; lane < 64; ++lane) {
+                const int k = ks * 4 + lane / 16, m = lane % 16;
+                const int d = k - m, i2 = tile * 16 + m;
+                if (d < 0 || d >= w || i2 >= n) continue;
+                bt[((((((size_t)t * n + i1) * w + jj) * tiles + tile) * KS + ks) * 64) + lane] =
+                    dia[((((size_t)t * n + i1) * w + jj) * w + d) * n + i2];
+              }
     try {
       hipStream_t s = nullptr;
       F.A64.upload(a.data(), a.size(), s);
       std::vector<float> af(a.begin(), a.end());
       F.A32.upload(af.data(), af.size(), s);
-      F.dia64.upload(dia, nd, s);
-      std::vector<float> df(dia, dia + nd);
-      F.dia32.upload(df.data(), nd, s);
+      F.bt64.upload(bt.data(), nbt, s);
+      std::vector<float> bf(bt.begin(), bt.end());
+      F.bt32.upload(bf.data(), nbt, s);
       PP_HIP(hipStreamSynchronize(s));
     } catch (...) {
       delete p;

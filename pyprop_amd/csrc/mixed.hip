@@ -74,17 +74,28 @@ __global__ __launch_bounds__(256) void residual_kernel(long long n, const double
   if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sh[0] + sh[1] + sh[2] + sh[3], 0.0);
 }
 
-// partial[b][j] over blocks -> out[j]
-__global__ void finish64_kernel(const double2* __restrict__ partial, int blocks, int width, int count,
-                                double2* __restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// out[j] = sum over blocks of partial[b][j]: one 256-thread workgroup per column j, strided
+// loads then a tree (deterministic order)
+__global__ __launch_bounds__(256) void finish64_kernel(const double2* __restrict__ partial, int blocks, int width,
+                                                       int count, double2* __restrict__ out) {
+  __shared__ double2 sh[256];
+  const int j = blockIdx.x;
   if (j >= count) return;
   double2 s = make_double2(0, 0);
-  for (int b = 0; b < blocks; ++b) {
+  for (int b = threadIdx.x; b < blocks; b += 256) {
     s.x += partial[(long long)b * width + j].x;
     s.y += partial[(long long)b * width + j].y;
   }
-  out[j] = s;
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      sh[threadIdx.x].x += sh[threadIdx.x + off].x;
+      sh[threadIdx.x].y += sh[threadIdx.x + off].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = sh[0];
 }
 
 // partial[b][j] = sum over the block's slice of conj(V_j) w (j < m); partial[b][m] = |w|^2
@@ -473,15 +484,14 @@ void MixedWork::block_dot32(int k, const float2* w_, double2* out, hipStream_t s
   const int width = k + 1;
   red.ensure(blocks, width);
   hipLaunchKernelGGL(block_dot32_kernel, dim3(blocks), dim3(256), 0, s, n, k, V.ptr, n, w_, red.partial.ptr, width);
-  hipLaunchKernelGGL(finish64_kernel, dim3((width + 63) / 64), dim3(64), 0, s, red.partial.ptr, blocks, width, width,
-                     out);
+  hipLaunchKernelGGL(finish64_kernel, dim3(width), dim3(256), 0, s, red.partial.ptr, blocks, width, width, out);
 }
 
 void MixedWork::block_update32(int k, const double2* h, float2* w_, double2* out, hipStream_t s) {
   const int blocks = reduction_blocks(n);
   red.ensure(blocks, 1);
   hipLaunchKernelGGL(block_update32_kernel, dim3(blocks), dim3(256), 0, s, n, k, V.ptr, n, h, w_, red.partial.ptr);
-  hipLaunchKernelGGL(finish64_kernel, dim3(1), dim3(64), 0, s, red.partial.ptr, blocks, 1, 1, out);
+  hipLaunchKernelGGL(finish64_kernel, dim3(1), dim3(256), 0, s, red.partial.ptr, blocks, 1, 1, out);
 }
 
 int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
@@ -623,7 +633,7 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     work.red.ensure(blocks, 1);
     hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(256), 0, s, n, work.b64.ptr, work.r64.ptr,
                        work.red.partial.ptr);
-    hipLaunchKernelGGL(finish64_kernel, dim3(1), dim3(64), 0, s, work.red.partial.ptr, blocks, 1, 1, work.small.ptr);
+    hipLaunchKernelGGL(finish64_kernel, dim3(1), dim3(256), 0, s, work.red.partial.ptr, blocks, 1, 1, work.small.ptr);
     PP_HIP(hipGetLastError());
     work.read_small(1, s);
     rn = std::sqrt(work.host_small[0].x);
