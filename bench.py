@@ -109,6 +109,51 @@ def factored_flops(fh, nc, n):
     return ang_issued + rad_issued, useful
 
 
+def bench_batch(torch, steps=1000, per_graph=10, natoms=64, seed=2024, dt=0.01):
+    """64 independent H atoms (N ~ U[80,200], lmax ~ U[4,15], seeded pulses): one
+    persistent launch per pAMP step (Krylov 20, complex64 multiply, FP64 state), steps
+    captured in a hipGraph, workgroup size from the autotuner.  The same steps with the FP64
+    multiply give the error of the FP32 multiply."""
+    import numpy as np
+    from pyprop_amd.batch import BatchedAtoms, random_specs
+    specs = random_specs(natoms, seed=seed)
+    t0 = time.time()
+    b32 = BatchedAtoms(specs, fp32_matvec=True)
+    b64 = BatchedAtoms(specs, fp32_matvec=False, packs=b32.packs)
+    log("batch: %d atoms set up (%.1fs), %d complex elements" % (natoms, time.time() - t0, b32.elements()))
+    best, tim = b32.autotune(dt, trial_steps=3)
+    b64.set_threads(best)
+    stream = torch.cuda.Stream()
+    res = {}
+    for name, b in (("fp32", b32), ("fp64", b64)):
+        with torch.cuda.stream(stream):
+            b.capture(dt, per_graph)
+            b.replay(1)  # warm-up
+            torch.cuda.synchronize()
+            b.set_state(b.psi0, np.zeros(natoms))
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            b.replay(steps // per_graph)
+            torch.cuda.synchronize()
+            res[name] = time.perf_counter() - t1
+    s32, tt, _ = b32.get_state()
+    s64, _, _ = b64.get_state()
+    err = max(float(np.linalg.norm(x - y) / np.linalg.norm(y)) for x, y in zip(s32, s64))
+    drift = 0.0
+    for a, sysm in enumerate(b32.systems):
+        S = sysm.bases[1].overlap_band()
+        drift = max(drift, abs(float(np.einsum("li,ij,lj->", s32[a].conj(), S, s32[a]).real) - 1.0))
+    sec = res["fp32"]
+    out = {"atoms": natoms, "steps": steps, "krylov": 20, "dt": dt, "threads": best,
+           "autotune_ms_per_step": tim, "elements": b32.elements(), "seconds": sec,
+           "steps_per_s": steps / sec, "atom_steps_per_s": natoms * steps / sec,
+           "matvecs_per_s": natoms * steps * 20 / sec, "fp64_multiply_steps_per_s": steps / res["fp64"],
+           "fp32_vs_fp64_max_rel_error": err, "max_norm_drift": drift, "t_end": float(tt.max())}
+    log("batch: %.1f steps/s (%.0f atom-steps/s, %.2e matvecs/s), threads %d, FP32 vs FP64 %.2e, norm drift %.1e" %
+        (out["steps_per_s"], out["atom_steps_per_s"], out["matvecs_per_s"], best, err, drift))
+    return out
+
+
 def cpu_baseline(sysm, n_ops_per_step, sample_frac=0.1):
     """The oracle (sequential C++ restatement of the reference loops) timed on the host:
     the tensor-potential multiply on the first `sample_frac` of every potential's angular
@@ -176,6 +221,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-atom batched pAMP section")
+    ap.add_argument("--profile-batch", type=int, default=0, help="only run this many batched steps (for rocprof)")
     ap.add_argument("--train-trajectories", type=int, default=64)
     ap.add_argument("--train-steps", type=int, default=24)
     ap.add_argument("--train-epochs", type=int, default=30)
@@ -197,6 +244,9 @@ def main():
     from pyprop_amd.factored import FactoredHamiltonian
     from pyprop_amd.propagator import CayleyPropagator, RadialTwoElectronPreconditioner, Wavefunction
 
+    if a.profile_batch:
+        bench_batch(torch, steps=a.profile_batch, per_graph=min(10, a.profile_batch))
+        return
     sysm, repr_, psi, base = build_workload(torch)
     nnz = sum(int(np.prod(p.source.shape)) for p in base.PotentialList)
     N = sysm.size
@@ -310,6 +360,8 @@ def main():
     log("mixed FP32 + learned: %.2f steps/s (%.2f ms/step), err vs plain %.2e, |dnorm| %.1e |dE| %.1e, %s" %
         (lrn_sps, 1e3 * lrn_sec / a.steps, lrn_err, lrn_dn, lrn_de, lrn_st))
 
+    batch = None if a.no_batch else bench_batch(torch)
+
     cpu = None
     if not a.no_cpu_baseline:
         t0 = time.time()
@@ -347,7 +399,8 @@ def main():
                    "training": {"rows": n_rows, "plain_steps": a.train_trajectories * a.train_steps,
                                 "epochs": a.train_epochs, "gen_s": t_gen, "fit_s": t_fit,
                                 "loss_first": learned.loss_history[0], "loss_last": learned.loss_history[-1]},
-                   "energy": e_plain, "norm_drift_plain": abs(n_plain - norm0)},
+                   "energy": e_plain, "norm_drift_plain": abs(n_plain - norm0),
+                   "batched_h_atoms": batch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "tp_dia_kernel (fused H0+R12+dipole FP64 multiply)", "ms": t_mv * 1e3,

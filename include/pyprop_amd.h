@@ -379,6 +379,43 @@ int pp_mixed_set_factored(pp_mixed* m, const pp_fham* f);
 int pp_mixed_step(pp_mixed* m, pp_cplx* psi, double dt, const double* scalings, pp_stream stream);
 int pp_mixed_stats(const pp_mixed* m, int* counts, double* residuals);
 
+/* ------------------------------------------------------------------------------
+ * Batched pAMP propagation of independent atoms.  No reference counterpart for the
+ * batching; each atom's step is PamPropagator.AdvanceStep in the balanced-overlap form
+ * (pyprop/propagator/PamPropagator.py:50-59, pAMP::PropagateVector pamp.h:270-670) with the
+ * reduced-spherical hydrogen Hamiltonian H0 (Diag x BandNH) + E(t) dipole (Simp x BandNH,
+ * couplings l -> l +- 1).  One persistent launch advances every atom by one step (one
+ * workgroup per atom, largest first); steps can be captured into a hipGraph and replayed.
+ * Per-atom host arrays are concatenated in atom order:
+ *   h0:     [nl][2b+1][N]   H0 in DIA form, element (i, i+d-b) at [l][d][i]
+ *   r:      [2b+1][N]       the dipole's radial matrix in DIA form
+ *   cpl:    [nl][2]         angular dipole couplings to l-1 and l+1
+ *   s_upper:[N][b+1]        radial overlap, upper band (element (i, j) at [j][b+i-j])
+ *   pulse:  [3]             E0, omega, duration of E0 sin^2(pi t/T) cos(omega t)
+ * fp32_matvec: the banded H multiply in complex64 with FP32 accumulation (state, Krylov
+ * basis, orthogonalisation, Pade and overlap solves stay FP64).
+ * ------------------------------------------------------------------------------ */
+typedef struct pp_batch pp_batch;
+int pp_batch_create(int natoms, const int* nl, const int* n, int band, const double* h0, const double* r,
+                    const double* cpl, const double* s_upper, const double* pulse, int krylov, int pade_order,
+                    int fp32_matvec, pp_batch** out);
+int pp_batch_destroy(pp_batch* b);
+/* host psi (concatenated [nl][N] complex128 per atom) and per-atom times */
+int pp_batch_set_state(pp_batch* b, const pp_cplx* psi, const double* t);
+/* any of psi, t, err (per-atom error estimate E(m-1, 0)) may be NULL */
+int pp_batch_get_state(const pp_batch* b, pp_cplx* psi, double* t, pp_cplx* err);
+/* time 64..1024-thread workgroups on trial_steps steps, restore the state, keep the fastest;
+ * timings_ms: 5 per-step times (may be NULL) */
+int pp_batch_autotune(pp_batch* b, double dt, int trial_steps, int* best_threads, double* timings_ms,
+                      pp_stream stream);
+int pp_batch_set_threads(pp_batch* b, int threads);
+/* nsteps persistent launches (one step each) */
+int pp_batch_step(pp_batch* b, double dt, int nsteps, pp_stream stream);
+/* capture steps_per_graph step launches into a hipGraph on a non-default stream; replay it */
+int pp_batch_capture(pp_batch* b, double dt, int steps_per_graph, pp_stream stream);
+int pp_batch_replay(pp_batch* b, int replays, pp_stream stream);
+int pp_batch_info(const pp_batch* b, int* threads, int* grid, long long* psi_elements, long long* lds_bytes);
+
 #ifdef __cplusplus
 }
 #endif

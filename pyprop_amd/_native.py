@@ -104,6 +104,16 @@ SIGNATURES = [
     ("pp_fham_destroy", I, [P]),
     ("pp_fham_multiply", I, [P, P, P, P, P, PPCplx, P]),
     ("pp_fham_multiply_f32", I, [P, P, P, P, P, PPCplx, P]),
+    ("pp_batch_create", I, [I, P, P, I, P, P, P, P, P, I, I, I, P]),
+    ("pp_batch_destroy", I, [P]),
+    ("pp_batch_set_state", I, [P, P, P]),
+    ("pp_batch_get_state", I, [P, P, P, P]),
+    ("pp_batch_autotune", I, [P, D, I, P, P, P]),
+    ("pp_batch_set_threads", I, [P, I]),
+    ("pp_batch_step", I, [P, D, I, P]),
+    ("pp_batch_capture", I, [P, D, I, P]),
+    ("pp_batch_replay", I, [P, I, P]),
+    ("pp_batch_info", I, [P, P, P, P, P]),
 ]
 
 _lib = None

@@ -308,7 +308,7 @@ void overlap_apply(const OverlapFactor& ov, int op, bool conj, int ndim, const i
 
 // Upper banded Cholesky S = U^T U of a real symmetric positive definite band matrix
 // (the arithmetic of zpbtrf('U') for a real matrix).
-static void band_cholesky_upper(int n, int sd, const double* s, std::vector<double>& u) {
+void band_cholesky_upper_host(int n, int sd, const double* s, std::vector<double>& u) {
   const int ld = sd + 1;
   u.assign(s, s + (size_t)n * ld);
   auto U = [&](int i, int j) -> double& { return u[(size_t)j * ld + sd + i - j]; };
@@ -339,7 +339,7 @@ extern "C" int pp_overlap_create(int n, int superdiagonals, const double* s_uppe
     try {
       o->f.n = n;
       o->f.sd = superdiagonals;
-      band_cholesky_upper(n, superdiagonals, s_upper, o->f.u_host);
+      band_cholesky_upper_host(n, superdiagonals, s_upper, o->f.u_host);
       o->f.u.upload(o->f.u_host.data(), o->f.u_host.size());
       PP_HIP(hipDeviceSynchronize());
     } catch (...) {
