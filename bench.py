@@ -56,6 +56,18 @@ def build_workload(torch):
     return sysm, repr_, psi, base
 
 
+def profiled_traffic():
+    """HBM bytes per launch of the fused multiply from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = ROOT / "profiles" / "tp_dia_kernel_helium_fp64_matvec.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def time_matvec(torch, base, psi, reps=20):
     """Average duration of the fused Hamiltonian multiply (one kernel) with HIP events on
     the stream the library launches on (torch's current stream)."""
@@ -94,19 +106,18 @@ def time_factored(torch, fh, psi, dtype, reps=50):
 
 
 def factored_flops(fh, nc, n):
-    """(issued MFMA flops incl. padding, useful flops) of one factored multiply."""
+    """(issued MFMA flops incl. padding, useful flops) of one factored multiply:
+    radial stage, one 16x16x4 MFMA per (term, i1, jj, i2 tile, channel group, K step);
+    angular stage, two MFMAs (re, im) per (16 channels, 16 positions, K step of 4)."""
     T, bw = fh.nterms, fh.bw
-    w = 2 * bw + 1
-    kp = (nc + 3) // 4 * 4
-    nq = 2 * n * n
-    m = T * nc
-    ang_issued = math.ceil(nq / 64) * math.ceil(m / 64) * 4 * (kp // 4) * 4 * 2048
     pairs1 = sum(min(n, i + bw + 1) - max(0, i - bw) for i in range(n))
     ks = (16 + 2 * bw + 3) // 4
     rad_issued = T * pairs1 * ks * math.ceil(n / 16) * math.ceil(nc / 8) * 2048
-    nnz_r = pairs1 * sum(min(n, i + bw + 1) - max(0, i - bw) for i in range(n))
-    useful = 2 * m * nc * nq + T * nnz_r * 4 * nc
-    return ang_issued + rad_issued, useful
+    kp = (T * nc + 3) // 4 * 4
+    ang_issued = math.ceil(n * n / 16) * math.ceil(nc / 16) * (kp // 4) * 2 * 2048
+    nnz_r = pairs1 * pairs1
+    useful = T * nnz_r * 4 * nc + 2 * nc * T * nc * 2 * n * n
+    return rad_issued + ang_issued, useful
 
 
 def bench_batch(torch, steps=1000, per_graph=10, natoms=64, seed=2024, dt=0.01):
@@ -227,6 +238,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=24)
     ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--profile-matvec", action="store_true", help="only run the matvec loop (for rocprof)")
+    ap.add_argument("--profile-copy", action="store_true",
+                    help="only copy 1 GiB device to device 20 times (HBM counter calibration for rocprof)")
     ap.add_argument("--profile-cayley", type=int, default=0, help="only run this many plain Cayley steps (for rocprof)")
     ap.add_argument("--profile-factored", action="store_true", help="only time the factored multiply (for rocprof)")
     ap.add_argument("--profile-mixed", type=int, default=0,
@@ -244,13 +257,21 @@ def main():
     from pyprop_amd.factored import FactoredHamiltonian
     from pyprop_amd.propagator import CayleyPropagator, RadialTwoElectronPreconditioner, Wavefunction
 
+    if a.profile_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(20):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied 20 x 1 GiB (read 1 GiB + write 1 GiB per copy kernel)")
+        return
     if a.profile_batch:
         bench_batch(torch, steps=a.profile_batch, per_graph=min(10, a.profile_batch))
         return
     sysm, repr_, psi, base = build_workload(torch)
     nnz = sum(int(np.prod(p.source.shape)) for p in base.PotentialList)
     N = sysm.size
-    alg_bytes = 8 * nnz + 3 * 16 * N  # real float64 potential elements + src read + dst write
+    alg_bytes = 8 * nnz + 2 * 16 * N  # real float64 potential elements + src read + dst write
     dt = 0.01
     if a.profile_matvec:
         t = time_matvec(torch, base, psi, reps=50)
@@ -402,7 +423,9 @@ def main():
                    "energy": e_plain, "norm_drift_plain": abs(n_plain - norm0),
                    "batched_h_atoms": batch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
+                     "traffic_source": "profiles/tp_dia_kernel_helium_fp64_matvec.md (rocprofv3 FETCH_SIZE + "
+                                       "WRITE_SIZE, scaled by a 1 GiB copy)",
                      "kernel": "tp_dia_kernel (fused H0+R12+dipole FP64 multiply)", "ms": t_mv * 1e3,
                      "alg_bytes": alg_bytes},
         "cpu_baseline": cpu,
