@@ -295,7 +295,7 @@ def main():
         prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-10,
                                      factored=FactoredHamiltonian(base))
         prop.SetupStep(dt)
-        sec, _ = run_steps(torch, prop, psi, 0.0, dt, 0, a.profile_mixed)
+        sec, _ = run_steps(torch, prop, psi, 0.0, dt, 2, a.profile_mixed)
         log("%d mixed steps %.2f ms/step, last %s" % (a.profile_mixed, 1e3 * sec / a.profile_mixed, prop.GetStats()))
         return
 
@@ -374,6 +374,47 @@ def main():
         agg["max_rel_residual"] = float(max(s["rel_residual"] for s in st))
         return a.steps / sec, sec, err, abs(nn - n_plain), abs(e - e_plain), agg
 
+    # what the network gains: residual left after one preconditioned direction on held-out
+    # steps (the benchmark pulse, not seen in training), Kronecker alone vs with the network
+    def first_direction_gain(nsteps=6):
+        from pyprop_amd import _native as NN, device as DD
+        psi.data.copy_(psi0)
+        cur = psi.data.clone()
+        h = torch.empty_like(cur)
+        res = {"kron": [], "learned": []}
+        tt = 0.0
+        for _ in range(nsteps):
+            sc = base.scalings(tt + dt / 2)
+            NN.check(NN.load().pp_ham_multiply_nooverlap(base.ham, DD.hptr(sc), DD.ptr(cur), DD.ptr(h),
+                                                         DD.stream_handle()))
+            r = h * (-1j * dt)
+            for name in ("kron", "learned"):
+                r32 = r.to(torch.complex64)
+                z = torch.empty_like(r32)
+                if name == "learned":
+                    learned.apply(r32, z, dt)
+                else:
+                    z.copy_(r32)
+                    kron.SetHamiltonianScaling(0.5j * dt)
+                    kron.Solve(z)
+                z64 = z.to(torch.complex128)
+                sz = z64.clone()
+                repr_._op(0, False, sz)
+                az = torch.empty_like(z64)
+                NN.check(NN.load().pp_ham_multiply_nooverlap(base.ham, DD.hptr(sc), DD.ptr(z64), DD.ptr(az),
+                                                             DD.stream_handle()))
+                az = sz + 0.5j * dt * az
+                al = torch.vdot(az.flatten(), r.flatten()) / torch.vdot(az.flatten(), az.flatten())
+                res[name].append(float(torch.linalg.norm(r - al * az) / torch.linalg.norm(r)))
+            plain.AdvanceStep(tt, dt)  # advance the reference state (psi aliases plain's state)
+            cur.copy_(psi.data)
+            tt += dt
+        return {k: float(np.mean(v)) for k, v in res.items()}
+
+    gain = first_direction_gain()
+    log("first-direction residual on held-out steps: Kronecker %.3e, learned %.3e (%.1f%% lower)" %
+        (gain["kron"], gain["learned"], 100 * (1 - gain["learned"] / gain["kron"])))
+
     kron_sps, kron_sec, kron_err, kron_dn, kron_de, kron_st = mixed_run(False)
     log("mixed FP32 + Kronecker: %.2f steps/s (%.2f ms/step), err vs plain %.2e, %s" %
         (kron_sps, 1e3 * kron_sec / a.steps, kron_err, kron_st))
@@ -415,6 +456,7 @@ def main():
                    "plain_fp64_ops_per_step": ops_plain,
                    "wavefunction_error_vs_plain_fp64": lrn_err, "norm_error": lrn_dn, "energy_error": lrn_de,
                    "learned_stats_per_step": lrn_st,
+                   "first_direction_residual": gain,
                    "fp32_kron_steps_per_s": kron_sps, "fp32_kron_error": kron_err,
                    "fp32_kron_stats_per_step": kron_st,
                    "training": {"rows": n_rows, "plain_steps": a.train_trajectories * a.train_steps,

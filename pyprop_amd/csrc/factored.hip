@@ -58,7 +58,9 @@ struct FScales {
 // so that the 16 operand rows of a wave (8 channels x re/im) at one j2 are 16 consecutive
 // values.
 template <class T>
-__global__ void fham_transpose_kernel(int nc, int ncp, int n, const T* __restrict__ X, T* __restrict__ Xt) {
+__global__ void fham_transpose_kernel(int nc, int ncp, int n, const T* __restrict__ X, T* __restrict__ Xt,
+                                      const int* __restrict__ skip) {
+  if (skip && *skip) return;
   const long long total = (long long)n * n * ncp * 2;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
        i += (long long)gridDim.x * blockDim.x) {
@@ -75,7 +77,9 @@ __global__ void fham_transpose_kernel(int nc, int ncp, int n, const T* __restric
 template <class T, int BW>
 __global__ __launch_bounds__(64) void fham_radial_kernel(int nc, int ncp, int n, int nterms, int ngroups, int tiles,
                                                          long long nblocks, const T* __restrict__ Xt,
-                                                         const T* __restrict__ BT, T* __restrict__ Z) {
+                                                         const T* __restrict__ BT, T* __restrict__ Z,
+                                                         const int* __restrict__ skip) {
+  if (skip && *skip) return;
   typedef typename MfmaT<T>::V4 V4;
   constexpr int w = 2 * BW + 1;
   constexpr int KW = 16 + 2 * BW;
@@ -224,7 +228,7 @@ template <class T>
 static void fham_run(const FHam& F, const double* scalings, const typename MfmaT<T>::C2* src,
                      typename MfmaT<T>::C2* dst, const typename MfmaT<T>::C2* addend, double2 alpha,
                      const DeviceBuffer<T>& Ab, const DeviceBuffer<T>& Db, DeviceBuffer<T>& Zb, DeviceBuffer<T>& Xb,
-                     hipStream_t s) {
+                     const int* skip_radial, hipStream_t s) {
   const int nc = F.nc, n = F.n, nterms = F.nterms;
   const long long P = (long long)n * n;
   if (Zb.count < (size_t)nterms * nc * P * 2) Zb.resize((size_t)nterms * nc * P * 2);
@@ -232,7 +236,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   const int ncp = ngroups * 8;
   if (Xb.count < (size_t)P * ncp * 2) Xb.resize((size_t)P * ncp * 2);
   hipLaunchKernelGGL(fham_transpose_kernel<T>, dim3((unsigned)std::min<long long>(4096, ceil_div(P * ncp * 2, 256))),
-                     dim3(256), 0, s, nc, ncp, n, reinterpret_cast<const T*>(src), Xb.ptr);
+                     dim3(256), 0, s, nc, ncp, n, reinterpret_cast<const T*>(src), Xb.ptr, skip_radial);
   const int tiles = (n + 15) / 16;
   const long long nblocks = (long long)nterms * n * tiles * ngroups;
   const long long hw = (nblocks + 7) / 8 * 8;
@@ -240,7 +244,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
 #define PP_RAD(B)                                                                                                    \
   case B:                                                                                                            \
     hipLaunchKernelGGL((fham_radial_kernel<T, B>), dim3((unsigned)hw), dim3(64), 0, s, nc, ncp, n, nterms, ngroups,   \
-                       tiles, nblocks, Xb.ptr, Db.ptr, Zb.ptr);                                                       \
+                       tiles, nblocks, Xb.ptr, Db.ptr, Zb.ptr, skip_radial);                                          \
     break;
     PP_RAD(1) PP_RAD(2) PP_RAD(3) PP_RAD(4) PP_RAD(5) PP_RAD(6) PP_RAD(7) PP_RAD(8)
 #undef PP_RAD
@@ -265,13 +269,32 @@ void fham_apply64(const FHam& f, const double* scalings, const double2* src, dou
 }
 
 void FHam::apply64(const double* scalings, const double2* src, double2* dst, const double2* addend, double2 alpha,
-                   hipStream_t s) const {
-  fham_run<double>(*this, scalings, src, dst, addend, alpha, A64, bt64, W64, X64, s);
+                   hipStream_t s, const int* skip_radial) const {
+  ++gen64;
+  fham_run<double>(*this, scalings, src, dst, addend, alpha, A64, bt64, W64, X64, skip_radial, s);
+}
+
+// flag = 0 when a and b hold the same values, 1 otherwise (flag preset to 0 by the first
+// block's thread 0 is not ordered with the others, so the caller presets it)
+__global__ void equal_check_kernel(long long n, const double2* __restrict__ a, const double2* __restrict__ b,
+                                   int* __restrict__ differs) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    if (a[i].x != b[i].x || a[i].y != b[i].y) *differs = 1;
+}
+__global__ void set_flag_kernel(int* __restrict__ f, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *f = v;
+}
+
+void fham_flag_differs(long long n, const double2* a, const double2* b, int* flag, hipStream_t s) {
+  hipLaunchKernelGGL(set_flag_kernel, dim3(1), dim3(64), 0, s, flag, 0);
+  hipLaunchKernelGGL(equal_check_kernel, dim3((unsigned)std::min<long long>(2048, ceil_div(n, 256))), dim3(256), 0, s, n,
+                     a, b, flag);
+  PP_HIP(hipGetLastError());
 }
 
 void FHam::apply32(const double* scalings, const float2* src, float2* dst, const float2* addend, double2 alpha,
                    hipStream_t s) const {
-  fham_run<float>(*this, scalings, src, dst, addend, alpha, A32, bt32, W32, X32, s);
+  fham_run<float>(*this, scalings, src, dst, addend, alpha, A32, bt32, W32, X32, nullptr, s);
 }
 
 }  // namespace pp

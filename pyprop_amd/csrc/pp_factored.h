@@ -21,11 +21,17 @@ struct FHam {
   mutable DeviceBuffer<double> X64;   // channel-minor copy of the input
   mutable DeviceBuffer<float> X32;
   // dst = addend + alpha * sum_t scalings[term_pot[t]] (A_t (x) R_t) src
+  // skip_radial (device int, may be NULL): when it reads non-zero, the radial products of
+  // the previous FP64 multiply are reused (src must then hold the same values as that call's)
   void apply64(const double* scalings, const double2* src, double2* dst, const double2* addend, double2 alpha,
-               hipStream_t s) const;
+               hipStream_t s, const int* skip_radial = nullptr) const;
+  mutable unsigned long long gen64 = 0;  // FP64 multiplies issued (invalidates reuse)
   void apply32(const double* scalings, const float2* src, float2* dst, const float2* addend, double2 alpha,
                hipStream_t s) const;
 };
+
+// *flag = 1 if a and b differ anywhere (n elements), else 0
+void fham_flag_differs(long long n, const double2* a, const double2* b, int* flag, hipStream_t s);
 
 }  // namespace pp
 

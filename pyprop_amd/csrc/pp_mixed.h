@@ -79,6 +79,11 @@ struct MixedCayley {
   int fallback_iterations = 40;      // inner iterations of one step beyond which the plain solver redoes it
   MixedWork work;
   MixedStats stats;
+  // reuse of the last FP64 radial products across steps (factored operator only)
+  bool z_valid = false;
+  unsigned long long z_gen = 0;
+  const double2* z_psi = nullptr;
+  DeviceBuffer<int> flags;
   std::vector<double> scal;
   void step(double2* psi, double dt, const double* scalings, hipStream_t s);
 };

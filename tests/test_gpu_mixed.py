@@ -482,3 +482,40 @@ def test_mixed_cayley_factored_full_size_vs_plain(torch):
     err = rel(psi2.numpy(), psi.numpy())
     say("  wavefunction difference %.2e" % err)
     assert err < 1e-8
+
+
+def test_mixed_factored_reuse_detects_modified_psi(torch, rng):
+    """The mixed solver reuses the radial products of its last accepted solution for the
+    next step's first multiply; when psi is changed between steps (here multiplied by a
+    phase) the device check must fall back to a full multiply: the step then equals that of
+    a fresh solver started from the modified state."""
+    from pyprop_amd.factored import FactoredHamiltonian
+    from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
+    say("mixed+factored: reuse across steps and modified psi")
+    sysm = _small_helium()
+    repr_, psi, base = _device(sysm, torch)
+    base.SetupStep()
+    fh = FactoredHamiltonian(base)
+    x0 = crandn(rng, sysm.shape)
+    psi.data.copy_(torch.from_numpy(x0).to("cuda"))
+    kp = KroneckerPreconditioner(psi, sysm)
+    prop = MixedCayleyPropagator(psi, base, kp, tolerance=1e-12, factored=fh)
+    dt, t = 0.01, 40.0
+    for k in range(3):
+        prop.AdvanceStep(t, dt)
+        t += dt
+    psi.data.mul_(complex(np.cos(0.3), np.sin(0.3)))
+    start = psi.numpy()
+    prop.AdvanceStep(t, dt)
+    got = psi.numpy()
+    # fresh solver on a copy of the modified state
+    from pyprop_amd.propagator import Wavefunction
+    psi2 = Wavefunction(repr_, start)
+    prop2 = MixedCayleyPropagator(psi2, base, kp, tolerance=1e-12, factored=fh)
+    prop2.AdvanceStep(t, dt)
+    assert rel(got, psi2.numpy()) < 1e-12
+    # and the unmodified reuse path agrees with the dense solve
+    sol, A, b = _dense_cayley(sysm, got, dt, base.scalings(t + dt + dt / 2), True)
+    psi.data.copy_(torch.from_numpy(got).to("cuda"))
+    prop.AdvanceStep(t + dt, dt)  # psi was overwritten with equal values: reuse is allowed
+    _check_solution(psi.numpy(), sol, A, b, 2e-11, 1e-8)
