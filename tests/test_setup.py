@@ -190,3 +190,31 @@ def test_helium_small_hamiltonian_bound_state():
     H, S = dense_operator(sysm, [1.0, 1.0, 0.0])
     w = scipy.linalg.eigh(H.real, S, eigvals_only=True)
     assert -2.91 < w[0] < -2.7
+
+
+def test_hydrogen_bound_states_and_yield():
+    """Field-free bound states of the hydrogen system: the lowest energies per l are
+    -1/(2 n^2); the ground state has zero ionisation yield and a state with equal weight on
+    the 1s state and a continuum (E > 0) eigenvector has yield 1/2."""
+    import scipy.linalg
+    from pyprop_amd.observables import bound_states, ionization_yield
+    from pyprop_amd.systems import hydrogen_system
+    sysm = hydrogen_system()
+    bound = bound_states(sysm)
+    for l, E, C in bound[:3]:
+        for k in range(2):
+            nq = l + 1 + k
+            assert abs(E[k] + 0.5 / nq ** 2) < 1e-6
+    nl, n = sysm.shape
+    psi = np.zeros((nl, n), complex)
+    psi[0] = bound[0][2][:, 0]
+    assert abs(ionization_yield(psi, sysm, bound)) < 1e-12
+    # a continuum eigenvector of l = 1
+    from pyprop_amd.observables import _band_dense
+    H0 = sysm.potentials[0]
+    H = _band_dense(H0.data.real[1], H0.geometries[1].pairs, n)
+    S = sysm.bases[1].overlap_band()
+    w, v = scipy.linalg.eigh(0.5 * (H + H.T), S)
+    kc = int(np.argmax(w > 0.5))
+    psi[1] = v[:, kc]
+    assert abs(ionization_yield(psi, sysm, bound) - 0.5) < 1e-10
