@@ -87,6 +87,21 @@ def time_matvec(torch, base, psi, reps=20):
     return e0.elapsed_time(e1) / reps / 1e3
 
 
+def tuned_factored(base):
+    """The factored operator with its radial kernel shape tuned for both precisions."""
+    from pyprop_amd.factored import FactoredHamiltonian
+    fh = FactoredHamiltonian(base)
+    tune = {}
+    for prec in (64, 32):
+        best, tim = fh.tune(prec, trials=5)
+        name = lambda k: "%dx%d%s" % (k[0], k[1], "p" if k[2] else "")
+        tune["fp%d" % prec] = {"best": name(fh.RADIAL_SHAPES[best]),
+                               "us": {name(k): round(1e3 * v, 1) for k, v in tim.items()}}
+        log("factored radial tune FP%d: best %s  (%s)" % (prec, tune["fp%d" % prec]["best"], ", ".join(
+            "%s %.1f us" % (k, v) for k, v in tune["fp%d" % prec]["us"].items())))
+    return fh, tune
+
+
 def time_factored(torch, fh, psi, dtype, reps=50):
     """Average duration of the factored multiply (angular + radial launch) in `dtype`."""
     src = psi.data.to(dtype).clone()
@@ -286,14 +301,14 @@ def main():
                                                       prop.GetOperatorCount()))
         return
     if a.profile_factored:
-        fh = FactoredHamiltonian(base)
+        fh, _ = tuned_factored(base)
         for dtype in (torch.complex128, torch.complex64):
             t = time_factored(torch, fh, psi, dtype, reps=20)
             log("factored %s %.1f us" % (dtype, t * 1e6))
         return
     if a.profile_mixed:
         prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-10,
-                                     factored=FactoredHamiltonian(base))
+                                     factored=tuned_factored(base)[0])
         prop.SetupStep(dt)
         sec, _ = run_steps(torch, prop, psi, 0.0, dt, 2, a.profile_mixed)
         log("%d mixed steps %.2f ms/step, last %s" % (a.profile_mixed, 1e3 * sec / a.profile_mixed, prop.GetStats()))
@@ -303,14 +318,15 @@ def main():
     achieved = alg_bytes / t_mv / 1e9
     log("fused H multiply: %.3f ms  alg %.3f GB  -> %.1f GB/s (%.1f%% of 8 TB/s)" %
         (t_mv * 1e3, alg_bytes / 1e9, achieved, 100 * achieved / HBM_PEAK_GBS))
-    fh = FactoredHamiltonian(base)
+    fh, fh_tune = tuned_factored(base)
     nc, n, _ = sysm.shape
     t_f64 = time_factored(torch, fh, psi, torch.complex128)
     t_f32 = time_factored(torch, fh, psi, torch.complex64)
     issued, useful = factored_flops(fh, nc, n)
     fact = {"fp64_ms": t_f64 * 1e3, "fp32_ms": t_f32 * 1e3, "fp64_matvecs_per_s": 1 / t_f64,
             "fp32_matvecs_per_s": 1 / t_f32, "mfma_flops_issued": issued, "useful_flops": useful,
-            "fp64_mfma_util": issued / t_f64 / 78.6e12, "fp32_mfma_util": issued / t_f32 / 157.3e12}
+            "fp64_mfma_util": issued / t_f64 / 78.6e12, "fp32_mfma_util": issued / t_f32 / 157.3e12,
+            "radial_tune": fh_tune}
     log("factored multiply: FP64 %.1f us (%.1f%% of FP64 MFMA peak), FP32 %.1f us (%.1f%% of FP32 MFMA peak)" %
         (t_f64 * 1e6, 100 * fact["fp64_mfma_util"], t_f32 * 1e6, 100 * fact["fp32_mfma_util"]))
     psi0 = psi.data.clone()

@@ -351,6 +351,27 @@ int pp_fham_multiply(const pp_fham* f, const double* scalings, const pp_cplx* sr
                      const pp_cplx* addend, pp_cplx alpha, pp_stream stream);
 int pp_fham_multiply_f32(const pp_fham* f, const double* scalings, const pp_cplxf* src, pp_cplxf* dst,
                          const pp_cplxf* addend, pp_cplx alpha, pp_stream stream);
+/* Radial-stage kernel shape of the factored multiply (no reference counterpart: the
+ * reference has no factored form).  Variants 0..6 compute (terms x groups of 8 channels) =
+ * (1,1), (2,2), (3,2), (3,3) per wave, and 4..6 the shapes of 1..3 with the next band step's
+ * operands prefetched into registers; precision is 64 (complex128 multiply) or 32
+ * (complex64).  pp_fham_tune times every variant (trials multiplies each, on stream) and keeps
+ * the fastest; timings_ms (may be NULL) receives PP_FHAM_RADIAL_VARIANTS entries. */
+#define PP_FHAM_RADIAL_VARIANTS 7
+int pp_fham_set_radial_variant(pp_fham* f, int precision, int variant);
+int pp_fham_get_radial_variant(const pp_fham* f, int precision, int* variant);
+int pp_fham_tune(pp_fham* f, int precision, int trials, int* best, double* timings_ms, pp_stream stream);
+/* Overlap term (no reference counterpart): appends S1 (x) S2 (dense n x n, row-major, zero
+ * outside the operator's band) as one more radial term whose angular part is the identity,
+ * so that one multiply forms the Cayley operator
+ *   dst = addend + alpha * sum_p scalings[p] V_p src + beta * (S1 (x) S2) src
+ * (CayleyPropagator.py:116-142 builds the same S + i dt/2 H from MultiplyOverlap and the
+ * potentials).  The overlap radial product is only formed when beta != 0. */
+int pp_fham_set_overlap(pp_fham* f, const double* S1, const double* S2);
+int pp_fham_multiply_overlap(const pp_fham* f, const double* scalings, const pp_cplx* src, pp_cplx* dst,
+                             const pp_cplx* addend, pp_cplx alpha, pp_cplx beta, pp_stream stream);
+int pp_fham_multiply_overlap_f32(const pp_fham* f, const double* scalings, const pp_cplxf* src, pp_cplxf* dst,
+                                 const pp_cplxf* addend, pp_cplx alpha, pp_cplx beta, pp_stream stream);
 /* use the factored multiply inside a Cayley solver (NULL detaches) */
 int pp_cayley_set_factored(pp_cayley* c, const pp_fham* f);
 

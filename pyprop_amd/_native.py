@@ -104,6 +104,12 @@ SIGNATURES = [
     ("pp_fham_destroy", I, [P]),
     ("pp_fham_multiply", I, [P, P, P, P, P, PPCplx, P]),
     ("pp_fham_multiply_f32", I, [P, P, P, P, P, PPCplx, P]),
+    ("pp_fham_set_radial_variant", I, [P, I, I]),
+    ("pp_fham_get_radial_variant", I, [P, I, P]),
+    ("pp_fham_tune", I, [P, I, I, P, P, P]),
+    ("pp_fham_set_overlap", I, [P, P, P]),
+    ("pp_fham_multiply_overlap", I, [P, P, P, P, P, PPCplx, PPCplx, P]),
+    ("pp_fham_multiply_overlap_f32", I, [P, P, P, P, P, PPCplx, PPCplx, P]),
     ("pp_batch_create", I, [I, P, P, I, P, P, P, P, P, I, I, I, P]),
     ("pp_batch_destroy", I, [P]),
     ("pp_batch_set_state", I, [P, P, P]),

@@ -27,6 +27,13 @@ HIPCC_FLAGS = [
 ]
 
 
+# Per-source additions.  factored.hip: MFMA accumulators in VGPRs (the default heuristic
+# puts them in AGPRs and copies them in and out on every iteration of the radial loop).
+EXTRA_FLAGS = {
+    "factored.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
+
 def hipcc() -> str:
     p = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not Path(p).exists():
@@ -39,7 +46,8 @@ def sources():
 
 
 def compile_one(src: Path, obj: Path) -> None:
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(HERE.parent / "include"), "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc(), *HIPCC_FLAGS, *EXTRA_FLAGS.get(src.name, []), "-I", str(HERE.parent / "include"), "-c", str(src),
+           "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))

@@ -17,6 +17,7 @@
 // Both stages use v_mfma_{f32,f64}_16x16x4.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <vector>
 
 #include "pp_common.h"
@@ -56,19 +57,25 @@ struct FScales {
 
 // Channel-minor copy of the input: Xt[j1][j2][c][part] (channels padded to ncp with zeros),
 // so that the 16 operand rows of a wave (8 channels x re/im) at one j2 are 16 consecutive
-// values.
+// values.  With ref (the input of the multiply whose radial products are held), any element
+// where X differs from ref sets *differs = 1; the radial stage then recomputes, and skips
+// when *differs stayed 0.
 template <class T>
 __global__ void fham_transpose_kernel(int nc, int ncp, int n, const T* __restrict__ X, T* __restrict__ Xt,
-                                      const int* __restrict__ skip) {
-  if (skip && *skip) return;
+                                      const T* __restrict__ ref, int* __restrict__ differs) {
   const long long total = (long long)n * n * ncp * 2;
+  bool diff = false;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
        i += (long long)gridDim.x * blockDim.x) {
     const int part = (int)(i & 1);
     const int c = (int)((i >> 1) % ncp);
     const long long pos = (i >> 1) / ncp;  // j1 * n + j2
-    Xt[i] = c < nc ? X[((long long)c * n * n + pos) * 2 + part] : T(0);
+    const long long src = ((long long)c * n * n + pos) * 2 + part;
+    const T v = c < nc ? X[src] : T(0);
+    Xt[i] = v;
+    if (ref && c < nc && v != ref[src]) diff = true;
   }
+  if (diff) *differs = 1;
 }
 
 // Stage 1.  Z: [T][nc][n][n] complex (as reals).  BT: the band tiles in MFMA lane order,
@@ -78,8 +85,8 @@ template <class T, int BW>
 __global__ __launch_bounds__(64) void fham_radial_kernel(int nc, int ncp, int n, int nterms, int ngroups, int tiles,
                                                          long long nblocks, const T* __restrict__ Xt,
                                                          const T* __restrict__ BT, T* __restrict__ Z,
-                                                         const int* __restrict__ skip) {
-  if (skip && *skip) return;
+                                                         const int* __restrict__ differs) {
+  if (differs && *differs == 0) return;  // input equals the one whose products Z holds
   typedef typename MfmaT<T>::V4 V4;
   constexpr int w = 2 * BW + 1;
   constexpr int KW = 16 + 2 * BW;
@@ -148,25 +155,196 @@ __global__ __launch_bounds__(64) void fham_radial_kernel(int nc, int ncp, int n,
   }
 }
 
+// Stage 1 with register reuse: one wave computes TG terms x GG channel groups of one
+// (i1, 16-column tile).  Per jj it loads the X windows of its GG groups once and reuses them
+// for the band tiles of its TG terms, and each band tile for its GG groups: (GG + TG) KS
+// operand loads feed TG GG KS MFMAs (the one-term, one-group form above loads two operands
+// per MFMA).  PIPE: the operands of jj + 1 are loaded into a second register set while the
+// MFMAs of jj run.  Padding terms (t >= nterms) reuse the last term's tile and are not
+// stored; padding groups read zeros and are not stored.
+template <class T, int BW, int TG, int GG, bool PIPE>
+__global__ __launch_bounds__(64) void fham_radial_reuse_kernel(int nc, int ncp, int n, int nterms, int ngroups,
+                                                               int tiles, long long nblocks, const T* __restrict__ Xt,
+                                                               const T* __restrict__ BT, T* __restrict__ Z,
+                                                               const int* __restrict__ differs) {
+  if (differs && *differs == 0) return;  // input equals the one whose products Z holds
+  typedef typename MfmaT<T>::V4 V4;
+  constexpr int w = 2 * BW + 1;
+  constexpr int KW = 16 + 2 * BW;
+  constexpr int KS = (KW + 3) / 4;
+  const long long per = (nblocks + 7) / 8;
+  const long long hw = blockIdx.x;
+  const long long L = (hw % 8) * per + hw / 8;
+  if (L >= nblocks) return;
+  const int ngc = (ngroups + GG - 1) / GG, ntc = (nterms + TG - 1) / TG;
+  const int gc = (int)(L % ngc);
+  const int tc = (int)((L / ngc) % ntc);
+  const int tile = (int)((L / ((long long)ngc * ntc)) % tiles);
+  const int i1 = (int)(L / ((long long)ngc * ntc * tiles));
+  const int lane = threadIdx.x;
+  const int I = tile * 16;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int cl = MfmaT<T>::kRowMajorOut ? (lr >> 1) : (lr & 7);
+  const int part = MfmaT<T>::kRowMajorOut ? (lr & 1) : (lr >> 3);
+  const int jlo = max(0, BW - i1), jhi = min(2 * BW, n - 1 - i1 + BW);
+  V4 acc[TG][GG];
+#pragma unroll
+  for (int u = 0; u < TG; ++u)
+#pragma unroll
+    for (int g = 0; g < GG; ++g) acc[u][g] = V4{0, 0, 0, 0};
+  const T* Bt[TG];
+#pragma unroll
+  for (int u = 0; u < TG; ++u) {
+    const int t = min(tc * TG + u, nterms - 1);
+    Bt[u] = BT + (((long long)t * n + i1) * w * tiles + tile) * KS * 64 + lane;
+  }
+  // X window of the GG groups at jj: unconditional loads from clamped (in-bounds) addresses,
+  // then a select, so no operand load sits behind a branch
+  auto load_x = [&](int jj, T (&av)[GG][KS]) {
+    const T* Xrow = Xt + (long long)(i1 - BW + jj) * n * ncp * 2 + part;
+#pragma unroll
+    for (int g = 0; g < GG; ++g) {
+      const int grp = gc * GG + g;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int k = ks * 4 + lk;
+        const int j2 = I - BW + k;
+        const bool ok = grp < ngroups && k < KW && j2 >= 0 && j2 < n;
+        const T v = Xrow[ok ? (long long)j2 * ncp * 2 + (grp * 8 + cl) * 2 : 0];
+        av[g][ks] = ok ? v : T(0);
+      }
+    }
+  };
+  auto load_b = [&](int jj, int u, T (&bv)[KS]) {
+    const T* Bj = Bt[u] + (long long)jj * tiles * KS * 64;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) bv[ks] = Bj[ks * 64];
+  };
+  auto mma_all = [&](const T (&av)[GG][KS], int u, const T (&bv)[KS]) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int g = 0; g < GG; ++g) acc[u][g] = MfmaT<T>::mma(av[g][ks], bv[ks], acc[u][g]);
+  };
+  if constexpr (PIPE) {
+    T av[GG][KS], bv[TG][KS];
+    load_x(jlo, av);
+#pragma unroll
+    for (int u = 0; u < TG; ++u) load_b(jlo, u, bv[u]);
+    for (int jj = jlo; jj <= jhi; ++jj) {
+      const int jn = min(jj + 1, jhi);  // the last pass reloads jhi (in bounds, unused)
+      T an[GG][KS], bn[TG][KS];
+      load_x(jn, an);
+#pragma unroll
+      for (int u = 0; u < TG; ++u) load_b(jn, u, bn[u]);
+#pragma unroll
+      for (int u = 0; u < TG; ++u) mma_all(av, u, bv[u]);
+#pragma unroll
+      for (int g = 0; g < GG; ++g)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) av[g][ks] = an[g][ks];
+#pragma unroll
+      for (int u = 0; u < TG; ++u)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) bv[u][ks] = bn[u][ks];
+    }
+  } else {
+    for (int jj = jlo; jj <= jhi; ++jj) {
+      T av[GG][KS];
+      load_x(jj, av);
+#pragma unroll
+      for (int u = 0; u < TG; ++u) {
+        T bv[KS];
+        load_b(jj, u, bv);
+        mma_all(av, u, bv);
+      }
+    }
+  }
+  const int i2 = I + lr;
+  if (i2 >= n) return;
+#pragma unroll
+  for (int u = 0; u < TG; ++u) {
+    const int t = tc * TG + u;
+    if (t >= nterms) continue;
+#pragma unroll
+    for (int g = 0; g < GG; ++g) {
+      const int grp = gc * GG + g;
+      if (grp >= ngroups) continue;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int c = grp * 8 + (MfmaT<T>::kRowMajorOut ? lk * 2 + q : lk + 4 * q);
+        if (c >= nc) continue;
+        T* z = Z + ((((long long)t * nc + c) * n + i1) * n + i2) * 2;
+        z[0] = MfmaT<T>::kRowMajorOut ? acc[u][g][2 * q] : acc[u][g][q];
+        z[1] = MfmaT<T>::kRowMajorOut ? acc[u][g][2 * q + 1] : acc[u][g][q + 2];
+      }
+    }
+  }
+}
+
+// (terms, channel groups, prefetch) per wave of each radial variant; variant 0 is the
+// one-term, one-group kernel
+constexpr int kRadTG[kFhamRadialVariants] = {1, 2, 3, 3, 2, 3, 3};
+constexpr int kRadGG[kFhamRadialVariants] = {1, 2, 2, 3, 2, 2, 3};
+constexpr bool kRadPipe[kFhamRadialVariants] = {false, false, false, false, true, true, true};
+
+template <class T, int B>
+static void launch_radial(int variant, int nc, int ncp, int n, int nterms, int ngroups, int tiles, const T* Xt,
+                          const T* BT, T* Z, const int* differs, hipStream_t s) {
+  const int TG = kRadTG[variant], GG = kRadGG[variant];
+  const long long nblocks =
+      (long long)n * tiles * ((nterms + TG - 1) / TG) * ((ngroups + GG - 1) / GG);
+  const dim3 grid((unsigned)((nblocks + 7) / 8 * 8));
+  switch (variant) {
+    case 0:
+      hipLaunchKernelGGL((fham_radial_kernel<T, B>), grid, dim3(64), 0, s, nc, ncp, n, nterms, ngroups, tiles,
+                         nblocks, Xt, BT, Z, differs);
+      break;
+#define PP_RADV(V)                                                                                                   \
+  case V:                                                                                                            \
+    hipLaunchKernelGGL((fham_radial_reuse_kernel<T, B, kRadTG[V], kRadGG[V], kRadPipe[V]>), grid, dim3(64), 0, s,   \
+                       nc, ncp, n, nterms, ngroups, tiles, nblocks, Xt, BT, Z, differs);                             \
+    break;
+      PP_RADV(1) PP_RADV(2) PP_RADV(3) PP_RADV(4) PP_RADV(5) PP_RADV(6)
+#undef PP_RADV
+    default:
+      throw Error(PP_ERR_INVALID, "radial variant out of range");
+  }
+}
+
 // Stage 2.  Workgroup of kAngWaves waves per (16 rows c', 16 complex positions p); the waves
-// split K = T nc and reduce through LDS.  A: [Mp][K] (row c', column t*nc + c).
+// split K = T nc and reduce through LDS.  A: [Mp][K] (row c', column t*nc + c).  The grid is
+// one-dimensional over nrb row blocks x position tiles, renumbered so that the row blocks of
+// one position tile run on one XCD together and read their Z columns through its L2.
 constexpr int kAngWaves = 16;
 constexpr int kAngBatch = 12;
 template <class T>
-__global__ __launch_bounds__(64 * kAngWaves) void fham_angular_kernel(int nc, int K, int Kreal, long long P,
+__global__ __launch_bounds__(64 * kAngWaves) void fham_angular_kernel(int nc, int nrb, int K, int Kreal, long long P,
                                                                       const T* __restrict__ A,
                                                                       const typename MfmaT<T>::C2* __restrict__ Z,
                                                                       FScales<T> sc,
                                                                       const typename MfmaT<T>::C2* addend,
                                                                       typename MfmaT<T>::C2 alpha,
-                                                                      typename MfmaT<T>::C2* out) {
+                                                                      const typename MfmaT<T>::C2* __restrict__ zs,
+                                                                      typename MfmaT<T>::C2 beta,
+                                                                      typename MfmaT<T>::C2* out,
+                                                                      typename MfmaT<T>::C2* out2,
+                                                                      typename MfmaT<T>::C2 alpha2,
+                                                                      typename MfmaT<T>::C2 beta2,
+                                                                      int* __restrict__ reset) {
   typedef typename MfmaT<T>::V4 V4;
   typedef typename MfmaT<T>::C2 C2;
   __shared__ V4 red[kAngWaves - 1][2][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const int m0 = blockIdx.y * 16;
-  const long long p0 = (long long)blockIdx.x * 16;
+  const long long nblocks = (long long)nrb * ((P + 15) / 16);
+  const long long per_xcd = (nblocks + 7) / 8;
+  const long long L = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (L >= nblocks) return;
+  // the radial stage of this multiply has read the comparison flag: re-arm it
+  if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;
+  const int m0 = (int)(L % nrb) * 16;
+  const long long p0 = (L / nrb) * 16;
   const int nks = K >> 2;
   const int per = (nks + kAngWaves - 1) / kAngWaves;
   const int ks0 = wave * per, ks1 = min(nks, ks0 + per);
@@ -220,31 +398,64 @@ __global__ __launch_bounds__(64 * kAngWaves) void fham_angular_kernel(int nc, in
     const T vr = are[r], vi = aim[r];
     o.x += alpha.x * vr - alpha.y * vi;
     o.y += alpha.x * vi + alpha.y * vr;
+    C2 v;
+    v.x = 0;
+    v.y = 0;
+    if (zs) v = zs[idx];  // overlap term: its angular part is the identity
+    o.x += beta.x * v.x - beta.y * v.y;
+    o.y += beta.x * v.y + beta.y * v.x;
     out[idx] = o;
+    if (out2) {  // second output from the same products (no addend)
+      C2 o2;
+      o2.x = alpha2.x * vr - alpha2.y * vi + beta2.x * v.x - beta2.y * v.y;
+      o2.y = alpha2.x * vi + alpha2.y * vr + beta2.x * v.y + beta2.y * v.x;
+      out2[idx] = o2;
+    }
   }
 }
 
 template <class T>
-static void fham_run(const FHam& F, const double* scalings, const typename MfmaT<T>::C2* src,
-                     typename MfmaT<T>::C2* dst, const typename MfmaT<T>::C2* addend, double2 alpha,
-                     const DeviceBuffer<T>& Ab, const DeviceBuffer<T>& Db, DeviceBuffer<T>& Zb, DeviceBuffer<T>& Xb,
-                     const int* skip_radial, hipStream_t s) {
+struct FOut {
+  typename MfmaT<T>::C2* dst = nullptr;
+  const typename MfmaT<T>::C2* addend = nullptr;
+  double2 alpha = make_double2(1.0, 0.0), beta = make_double2(0.0, 0.0);
+};
+
+template <class T>
+static typename MfmaT<T>::C2 to_c2(double2 v) {
+  typename MfmaT<T>::C2 c;
+  c.x = (T)v.x;
+  c.y = (T)v.y;
+  return c;
+}
+
+// o1.dst = o1.addend + o1.alpha Y + o1.beta Zs, o2 likewise without addend (may be NULL);
+// Y = sum_t f_t A_t Z_t, Zs = the overlap product.  ref / differs: reuse of the radial
+// products when src equals ref (FP64 only).
+template <class T>
+static void fham_run(const FHam& F, const double* scalings, const typename MfmaT<T>::C2* src, const FOut<T>& o1,
+                     const FOut<T>* o2, const DeviceBuffer<T>& Ab, const DeviceBuffer<T>& Db, DeviceBuffer<T>& Zb,
+                     DeviceBuffer<T>& Xb, const T* ref, int* differs, hipStream_t s) {
   const int nc = F.nc, n = F.n, nterms = F.nterms;
   const long long P = (long long)n * n;
-  if (Zb.count < (size_t)nterms * nc * P * 2) Zb.resize((size_t)nterms * nc * P * 2);
+  auto nz = [](double2 v) { return v.x != 0.0 || v.y != 0.0; };
+  const bool with_s = nz(o1.beta) || (o2 && nz(o2->beta));
+  PP_REQUIRE(!with_s || F.has_overlap, "overlap coefficient given but the operator has no overlap term");
+  PP_REQUIRE(!(o2 && o2->addend), "the second output takes no addend");
+  const int nrad = nterms + (with_s ? 1 : 0);  // radial terms formed (the overlap term is last)
+  const size_t zneed = (size_t)(nterms + (F.has_overlap ? 1 : 0)) * nc * P * 2;
+  if (Zb.count < zneed) Zb.resize(zneed);
   const int ngroups = (nc + 7) / 8;
   const int ncp = ngroups * 8;
   if (Xb.count < (size_t)P * ncp * 2) Xb.resize((size_t)P * ncp * 2);
   hipLaunchKernelGGL(fham_transpose_kernel<T>, dim3((unsigned)std::min<long long>(4096, ceil_div(P * ncp * 2, 256))),
-                     dim3(256), 0, s, nc, ncp, n, reinterpret_cast<const T*>(src), Xb.ptr, skip_radial);
+                     dim3(256), 0, s, nc, ncp, n, reinterpret_cast<const T*>(src), Xb.ptr, ref, differs);
   const int tiles = (n + 15) / 16;
-  const long long nblocks = (long long)nterms * n * tiles * ngroups;
-  const long long hw = (nblocks + 7) / 8 * 8;
+  const int variant = sizeof(T) == 8 ? F.rad_variant64 : F.rad_variant32;
   switch (F.bw) {
 #define PP_RAD(B)                                                                                                    \
   case B:                                                                                                            \
-    hipLaunchKernelGGL((fham_radial_kernel<T, B>), dim3((unsigned)hw), dim3(64), 0, s, nc, ncp, n, nterms, ngroups,   \
-                       tiles, nblocks, Xb.ptr, Db.ptr, Zb.ptr, skip_radial);                                          \
+    launch_radial<T, B>(variant, nc, ncp, n, nrad, ngroups, tiles, Xb.ptr, Db.ptr, Zb.ptr, differs, s);             \
     break;
     PP_RAD(1) PP_RAD(2) PP_RAD(3) PP_RAD(4) PP_RAD(5) PP_RAD(6) PP_RAD(7) PP_RAD(8)
 #undef PP_RAD
@@ -253,13 +464,15 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   }
   FScales<T> sc;
   for (int t = 0; t < kMaxFTerms; ++t) sc.s[t] = t < nterms ? (T)scalings[F.term_pot[t]] : T(0);
-  typename MfmaT<T>::C2 a;
-  a.x = (T)alpha.x;
-  a.y = (T)alpha.y;
-  hipLaunchKernelGGL(fham_angular_kernel<T>, dim3((unsigned)ceil_div(P, 16), (unsigned)ceil_div(nc, 16)),
-                     dim3(64 * kAngWaves),
-                     0, s, nc, F.Kp, nterms * nc, P, Ab.ptr, reinterpret_cast<const typename MfmaT<T>::C2*>(Zb.ptr), sc, addend, a,
-                     dst);
+  const int nrb = (nc + 15) / 16;
+  const long long ablocks = (long long)nrb * ceil_div(P, 16);
+  typedef typename MfmaT<T>::C2 C2;
+  const C2* Zc = reinterpret_cast<const C2*>(Zb.ptr);
+  hipLaunchKernelGGL(fham_angular_kernel<T>, dim3((unsigned)((ablocks + 7) / 8 * 8)), dim3(64 * kAngWaves), 0, s, nc,
+                     nrb, F.Kp, nterms * nc, P, Ab.ptr, Zc, sc, o1.addend, to_c2<T>(o1.alpha),
+                     with_s ? Zc + (size_t)nterms * nc * P : nullptr, to_c2<T>(o1.beta), o1.dst,
+                     o2 ? o2->dst : nullptr, to_c2<T>(o2 ? o2->alpha : make_double2(0, 0)),
+                     to_c2<T>(o2 ? o2->beta : make_double2(0, 0)), differs);
   PP_HIP(hipGetLastError());
 }
 
@@ -269,37 +482,77 @@ void fham_apply64(const FHam& f, const double* scalings, const double2* src, dou
 }
 
 void FHam::apply64(const double* scalings, const double2* src, double2* dst, const double2* addend, double2 alpha,
-                   hipStream_t s, const int* skip_radial) const {
+                   hipStream_t s, double2 beta) const {
+  FHamOut o;
+  o.dst = dst;
+  o.addend = addend;
+  o.alpha = alpha;
+  o.beta = beta;
+  apply64_ex(scalings, src, o, nullptr, nullptr, s);
+}
+
+void FHam::apply64_ex(const double* scalings, const double2* src, const FHamOut& o1, const FHamOut* o2,
+                      const double2* reuse_ref, hipStream_t s) const {
+  auto nz = [](double2 v) { return v.x != 0.0 || v.y != 0.0; };
+  const int nrad = nterms + ((nz(o1.beta) || (o2 && nz(o2->beta))) ? 1 : 0);
+  PP_REQUIRE(!reuse_ref || w64_terms >= nrad, "held radial products do not cover the requested terms");
+  if (reuse_ref && reuse_flag.count == 0) {
+    reuse_flag.resize(1);
+    PP_HIP(hipMemsetAsync(reuse_flag.ptr, 0, sizeof(int), s));  // re-armed by every multiply that reads it
+  }
   ++gen64;
-  fham_run<double>(*this, scalings, src, dst, addend, alpha, A64, bt64, W64, X64, skip_radial, s);
-}
-
-// flag = 0 when a and b hold the same values, 1 otherwise (flag preset to 0 by the first
-// block's thread 0 is not ordered with the others, so the caller presets it)
-__global__ void equal_check_kernel(long long n, const double2* __restrict__ a, const double2* __restrict__ b,
-                                   int* __restrict__ differs) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    if (a[i].x != b[i].x || a[i].y != b[i].y) *differs = 1;
-}
-__global__ void set_flag_kernel(int* __restrict__ f, int v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *f = v;
-}
-
-void fham_flag_differs(long long n, const double2* a, const double2* b, int* flag, hipStream_t s) {
-  hipLaunchKernelGGL(set_flag_kernel, dim3(1), dim3(64), 0, s, flag, 0);
-  hipLaunchKernelGGL(equal_check_kernel, dim3((unsigned)std::min<long long>(2048, ceil_div(n, 256))), dim3(256), 0, s, n,
-                     a, b, flag);
-  PP_HIP(hipGetLastError());
+  FOut<double> a1, a2;
+  a1.dst = o1.dst;
+  a1.addend = o1.addend;
+  a1.alpha = o1.alpha;
+  a1.beta = o1.beta;
+  if (o2) {
+    a2.dst = o2->dst;
+    a2.addend = o2->addend;
+    a2.alpha = o2->alpha;
+    a2.beta = o2->beta;
+  }
+  fham_run<double>(*this, scalings, src, a1, o2 ? &a2 : nullptr, A64, bt64, W64, X64,
+                   reinterpret_cast<const double*>(reuse_ref), reuse_ref ? reuse_flag.ptr : nullptr, s);
+  // when the products were reused they cover at least nrad terms; when recomputed, exactly nrad
+  w64_terms = nrad;
 }
 
 void FHam::apply32(const double* scalings, const float2* src, float2* dst, const float2* addend, double2 alpha,
-                   hipStream_t s) const {
-  fham_run<float>(*this, scalings, src, dst, addend, alpha, A32, bt32, W32, X32, nullptr, s);
+                   hipStream_t s, double2 beta) const {
+  FOut<float> o;
+  o.dst = dst;
+  o.addend = addend;
+  o.alpha = alpha;
+  o.beta = beta;
+  fham_run<float>(*this, scalings, src, o, nullptr, A32, bt32, W32, X32, nullptr, nullptr, s);
 }
 
 }  // namespace pp
 
 using namespace pp;
+
+// Band tiles in MFMA lane order [t][i1][jj][tile][ks][lane] from dia[t][i1][jj][d][i2]
+// (element B[k][m] = dia[t][i1][jj][k - m][16 tile + m], zero outside the band).
+static std::vector<double> band_tiles(int nterms, int n, int bw, const double* dia) {
+  const int w = 2 * bw + 1;
+  const int tiles = (n + 15) / 16;
+  const int KS = (16 + 2 * bw + 3) / 4;
+  std::vector<double> bt((size_t)nterms * n * w * tiles * KS * 64, 0.0);
+  for (int t = 0; t < nterms; ++t)
+    for (int i1 = 0; i1 < n; ++i1)
+      for (int jj = 0; jj < w; ++jj)
+        for (int tile = 0; tile < tiles; ++tile)
+          for (int ks = 0; ks < KS; ++ks)
+            for (int lane = 0; lane < 64; ++lane) {
+              const int k = ks * 4 + lane / 16, m = lane % 16;
+              const int d = k - m, i2 = tile * 16 + m;
+              if (d < 0 || d >= w || i2 >= n) continue;
+              bt[((((((size_t)t * n + i1) * w + jj) * tiles + tile) * KS + ks) * 64) + lane] =
+                  dia[((((size_t)t * n + i1) * w + jj) * w + d) * n + i2];
+            }
+  return bt;
+}
 
 extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double* A, const int* term_potential,
                               const double* dia, int npotentials, pp_fham** out) {
@@ -325,24 +578,8 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
     for (int t = 0; t < nterms; ++t)
       for (int r = 0; r < nchan; ++r)
         for (int c = 0; c < nchan; ++c) a[(size_t)r * F.Kp + (size_t)t * nchan + c] = A[((size_t)t * nchan + r) * nchan + c];
-    const int w = 2 * bw + 1;
-    const int tiles = (n + 15) / 16;
-    const int KS = (16 + 2 * bw + 3) / 4;
-    // band tiles in MFMA lane order: [t][i1][jj][tile][ks][lane]
-    const size_t nbt = (size_t)nterms * n * w * tiles * KS * 64;
-    std::vector<double> bt(nbt, 0.0);
-    for (int t = 0; t < nterms; ++t)
-      for (int i1 = 0; i1 < n; ++i1)
-        for (int jj = 0; jj < w; ++jj)
-          for (int tile = 0; tile < tiles; ++tile)
-            for (int ks = 0; ks < KS; ++ks)
-              for (int lane = 0; lane < 64; ++lane) {
-                const int k = ks * 4 + lane / 16, m = lane % 16;
-                const int d = k - m, i2 = tile * 16 + m;
-                if (d < 0 || d >= w || i2 >= n) continue;
-                bt[((((((size_t)t * n + i1) * w + jj) * tiles + tile) * KS + ks) * 64) + lane] =
-                    dia[((((size_t)t * n + i1) * w + jj) * w + d) * n + i2];
-              }
+    std::vector<double> bt = band_tiles(nterms, n, bw, dia);
+    const size_t nbt = bt.size();
     try {
       hipStream_t s = nullptr;
       F.A64.upload(a.data(), a.size(), s);
@@ -379,5 +616,137 @@ extern "C" int pp_fham_multiply_f32(const pp_fham* f, const double* scalings, co
     PP_REQUIRE(f && scalings && src && dst, "null argument");
     f->f.apply32(scalings, reinterpret_cast<const float2*>(src), reinterpret_cast<float2*>(dst),
                  reinterpret_cast<const float2*>(addend), make_double2(alpha.re, alpha.im), (hipStream_t)stream);
+  });
+}
+
+extern "C" int pp_fham_set_radial_variant(pp_fham* f, int precision, int variant) {
+  return guarded([&] {
+    PP_REQUIRE(f && (precision == 64 || precision == 32), "precision must be 64 or 32");
+    PP_REQUIRE(variant >= 0 && variant < kFhamRadialVariants, "radial variant out of range");
+    (precision == 64 ? f->f.rad_variant64 : f->f.rad_variant32) = variant;
+  });
+}
+
+extern "C" int pp_fham_get_radial_variant(const pp_fham* f, int precision, int* variant) {
+  return guarded([&] {
+    PP_REQUIRE(f && variant && (precision == 64 || precision == 32), "invalid argument");
+    *variant = precision == 64 ? f->f.rad_variant64 : f->f.rad_variant32;
+  });
+}
+
+// Times every radial variant on a zero input (one warm-up and `trials` timed multiplies each,
+// HIP events on `stream`) and keeps the fastest for that precision.
+extern "C" int pp_fham_tune(pp_fham* f, int precision, int trials, int* best, double* timings_ms, pp_stream stream) {
+  return guarded([&] {
+    PP_REQUIRE(f && (precision == 64 || precision == 32) && trials >= 1, "invalid argument");
+    FHam& F = f->f;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t N = (size_t)F.nc * F.n * F.n;
+    const size_t esz = precision == 64 ? sizeof(double2) : sizeof(float2);
+    DeviceBuffer<char> src, dst;
+    src.resize(N * esz);
+    dst.resize(N * esz);
+    PP_HIP(hipMemsetAsync(src.ptr, 0, N * esz, s));
+    std::vector<double> sc(F.npot, 1.0);
+    int& var = precision == 64 ? F.rad_variant64 : F.rad_variant32;
+    auto run = [&] {
+      if (precision == 64)
+        F.apply64(sc.data(), reinterpret_cast<const double2*>(src.ptr), reinterpret_cast<double2*>(dst.ptr), nullptr,
+                  make_double2(1.0, 0.0), s);
+      else
+        F.apply32(sc.data(), reinterpret_cast<const float2*>(src.ptr), reinterpret_cast<float2*>(dst.ptr), nullptr,
+                  make_double2(1.0, 0.0), s);
+    };
+    hipEvent_t e0, e1;
+    PP_HIP(hipEventCreate(&e0));
+    PP_HIP(hipEventCreate(&e1));
+    double best_ms = 1e300;
+    int best_v = var;
+    for (int v = 0; v < kFhamRadialVariants; ++v) {
+      var = v;
+      run();  // warm-up
+      PP_HIP(hipEventRecord(e0, s));
+      for (int k = 0; k < trials; ++k) run();
+      PP_HIP(hipEventRecord(e1, s));
+      PP_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      PP_HIP(hipEventElapsedTime(&ms, e0, e1));
+      const double per = ms / trials;
+      if (timings_ms) timings_ms[v] = per;
+      if (per < best_ms) {
+        best_ms = per;
+        best_v = v;
+      }
+    }
+    PP_HIP(hipEventDestroy(e0));
+    PP_HIP(hipEventDestroy(e1));
+    var = best_v;
+    if (best) *best = best_v;
+  });
+}
+
+// Appends the overlap term S1 (x) S2 (dense n x n each, zero outside the operator's band) as
+// the last radial term.
+extern "C" int pp_fham_set_overlap(pp_fham* f, const double* S1, const double* S2) {
+  return guarded([&] {
+    PP_REQUIRE(f && S1 && S2, "null argument");
+    FHam& F = f->f;
+    PP_REQUIRE(!F.has_overlap, "the operator already has an overlap term");
+    const int n = F.n, bw = F.bw, w = 2 * bw + 1;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        PP_REQUIRE(std::abs(i - j) <= bw || (S1[(size_t)i * n + j] == 0.0 && S2[(size_t)i * n + j] == 0.0),
+                   "overlap matrix is wider than the operator's band");
+    // dia[i1][jj][d][i2] = S1[i1][i1 - bw + jj] S2[i2][i2 - bw + d]
+    std::vector<double> dia((size_t)n * w * w * n, 0.0);
+    for (int i1 = 0; i1 < n; ++i1)
+      for (int jj = 0; jj < w; ++jj) {
+        const int j1 = i1 - bw + jj;
+        if (j1 < 0 || j1 >= n) continue;
+        const double s1 = S1[(size_t)i1 * n + j1];
+        for (int d = 0; d < w; ++d)
+          for (int i2 = 0; i2 < n; ++i2) {
+            const int j2 = i2 - bw + d;
+            if (j2 < 0 || j2 >= n) continue;
+            dia[(((size_t)i1 * w + jj) * w + d) * n + i2] = s1 * S2[(size_t)i2 * n + j2];
+          }
+      }
+    const std::vector<double> bt = band_tiles(1, n, bw, dia.data());
+    const size_t old_count = F.bt64.count;
+    hipStream_t s = nullptr;
+    DeviceBuffer<double> b64(old_count + bt.size());
+    DeviceBuffer<float> b32(old_count + bt.size());
+    PP_HIP(hipMemcpyAsync(b64.ptr, F.bt64.ptr, old_count * sizeof(double), hipMemcpyDeviceToDevice, s));
+    PP_HIP(hipMemcpyAsync(b32.ptr, F.bt32.ptr, old_count * sizeof(float), hipMemcpyDeviceToDevice, s));
+    PP_HIP(hipMemcpyAsync(b64.ptr + old_count, bt.data(), bt.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    std::vector<float> bf(bt.begin(), bt.end());
+    PP_HIP(hipMemcpyAsync(b32.ptr + old_count, bf.data(), bf.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    PP_HIP(hipStreamSynchronize(s));
+    std::swap(F.bt64.ptr, b64.ptr);
+    std::swap(F.bt64.count, b64.count);
+    std::swap(F.bt32.ptr, b32.ptr);
+    std::swap(F.bt32.count, b32.count);
+    F.has_overlap = true;
+  });
+}
+
+extern "C" int pp_fham_multiply_overlap(const pp_fham* f, const double* scalings, const pp_cplx* src, pp_cplx* dst,
+                                        const pp_cplx* addend, pp_cplx alpha, pp_cplx beta, pp_stream stream) {
+  return guarded([&] {
+    PP_REQUIRE(f && scalings && src && dst, "null argument");
+    f->f.apply64(scalings, reinterpret_cast<const double2*>(src), reinterpret_cast<double2*>(dst),
+                 reinterpret_cast<const double2*>(addend), make_double2(alpha.re, alpha.im), (hipStream_t)stream,
+                 make_double2(beta.re, beta.im));
+  });
+}
+
+extern "C" int pp_fham_multiply_overlap_f32(const pp_fham* f, const double* scalings, const pp_cplxf* src,
+                                            pp_cplxf* dst, const pp_cplxf* addend, pp_cplx alpha, pp_cplx beta,
+                                            pp_stream stream) {
+  return guarded([&] {
+    PP_REQUIRE(f && scalings && src && dst, "null argument");
+    f->f.apply32(scalings, reinterpret_cast<const float2*>(src), reinterpret_cast<float2*>(dst),
+                 reinterpret_cast<const float2*>(addend), make_double2(alpha.re, alpha.im), (hipStream_t)stream,
+                 make_double2(beta.re, beta.im));
   });
 }

@@ -42,29 +42,159 @@ static int grid_n(long long n) {
 // ------------------------------------------------------------------------------------
 // FP32 vector kernels (FP64 accumulation in reductions)
 // ------------------------------------------------------------------------------------
-__global__ void f64_to_f32_kernel(long long n, const double2* __restrict__ x, float2* __restrict__ y) {
+// y = s x in FP32 (the scale applied in FP64)
+__global__ void f64_to_f32_scaled_kernel(long long n, double sc, const double2* __restrict__ x, float2* __restrict__ y) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    y[i] = make_float2((float)x[i].x, (float)x[i].y);
+    y[i] = make_float2((float)(sc * x[i].x), (float)(sc * x[i].y));
 }
 
-// y += x (FP32 into FP64)
-__global__ void add_f32_to_f64_kernel(long long n, const float2* __restrict__ x, double2* __restrict__ y) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    double2 v = y[i];
-    v.x += (double)x[i].x;
-    v.y += (double)x[i].y;
-    y[i] = v;
+
+
+// Sum over the blocks of partial[b][j] (j < count, partial rows of `width`) in a fixed order:
+// 8 threads per column stride over the blocks, then a shuffle tree.  Every caller uses
+// 256-thread blocks, so any block that evaluates it, and finish_partials_kernel, obtains the
+// same bits.  dst may be LDS or global memory; the caller synchronises.
+__device__ void sum_partials(const double2* __restrict__ partial, int blocks, int width, int count, double2* dst) {
+  const int q = threadIdx.x & 7;
+  for (int j0 = 0; j0 < count; j0 += 32) {
+    const int j = j0 + (int)(threadIdx.x >> 3);
+    double2 acc = make_double2(0, 0);
+    if (j < count) {
+      // four independent chains so that the loads of a pass are in flight together
+      double2 a4[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+      for (int b0 = q; b0 < blocks; b0 += 32) {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int b = b0 + 8 * u;
+          v[u] = b < blocks ? partial[(long long)b * width + j] : make_double2(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          a4[u].x += v[u].x;
+          a4[u].y += v[u].y;
+        }
+      }
+      acc.x = (a4[0].x + a4[1].x) + (a4[2].x + a4[3].x);
+      acc.y = (a4[0].y + a4[1].y) + (a4[2].y + a4[3].y);
+    }
+    for (int off = 4; off > 0; off >>= 1) {
+      acc.x += __shfl_down(acc.x, off, 8);
+      acc.y += __shfl_down(acc.y, off, 8);
+    }
+    if (q == 0 && j < count) dst[j] = acc;
   }
 }
 
-// out = (in == 0): skip flag from the "differs" flag
-__global__ void invert_flag_kernel(const int* __restrict__ in, int* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *out = (*in == 0) ? 1 : 0;
+// out = [sum of p1 (c1 columns), sum of p2 (c2 columns), sum of p3 (c3 columns)], one block
+__global__ __launch_bounds__(256) void finish_partials_kernel(int blocks, const double2* __restrict__ p1, int w1,
+                                                              int c1, const double2* __restrict__ p2, int w2, int c2,
+                                                              const double2* __restrict__ p3, int w3, int c3,
+                                                              double2* __restrict__ out) {
+  if (c1) sum_partials(p1, blocks, w1, c1, out);
+  if (c2) sum_partials(p2, blocks, w2, c2, out + c1);
+  if (c3) sum_partials(p3, blocks, w3, c3, out + c1 + c2);
 }
 
-// r = b - r (r holds A x on entry), with |r|^2 block partials
-__global__ __launch_bounds__(256) void residual_kernel(long long n, const double2* __restrict__ b,
-                                                       double2* __restrict__ r, double2* __restrict__ partial) {
+// One pass of classical Gram-Schmidt over FP32 vectors with FP64 sums; block partials go to
+// partial[block][*] (finished by the next kernel or finish_partials_kernel):
+//   UPDATE: h = the finished previous dots (every block sums hpart itself, in the same order),
+//           then w -= sum_{j<m} V_j h_j (applied in FP32)
+//   DOT:    partial[b][j] = V_j^H w (j < m), partial[b][m] = |w|^2, after the update;
+//           else partial[b][0] = |w|^2
+// The update and the dots of a block cover the same slice with the same thread-to-element
+// map, so each thread reads back only what it wrote.
+template <bool UPDATE, bool DOT>
+__global__ __launch_bounds__(256) void cgs32_kernel(long long n, int m, const float2* __restrict__ V, long long ld,
+                                                    const double2* __restrict__ hpart, int hwidth,
+                                                    float2* __restrict__ w, double2* __restrict__ partial) {
+  __shared__ double2 h64s[128];
+  __shared__ float2 hs[128];
+  __shared__ double2 sh[4];
+  const int width = DOT ? m + 1 : 1;
+  const long long chunk = (n + gridDim.x - 1) / gridDim.x;
+  const long long beg = chunk * blockIdx.x;
+  const long long end = min(n, beg + chunk);
+  double nrm = 0;
+  if (UPDATE) {
+    sum_partials(hpart, gridDim.x, hwidth, m, h64s);
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += blockDim.x) hs[j] = make_float2((float)h64s[j].x, (float)h64s[j].y);
+    __syncthreads();
+    for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
+      float2 t = make_float2(0.f, 0.f);
+      for (int j = 0; j < m; ++j) {
+        const float2 v = V[j * ld + i];
+        t.x = fmaf(v.x, hs[j].x, t.x);
+        t.x = fmaf(-v.y, hs[j].y, t.x);
+        t.y = fmaf(v.x, hs[j].y, t.y);
+        t.y = fmaf(v.y, hs[j].x, t.y);
+      }
+      float2 wv = w[i];
+      wv.x -= t.x;
+      wv.y -= t.y;
+      w[i] = wv;
+      nrm += (double)wv.x * wv.x + (double)wv.y * wv.y;
+    }
+  }
+  if (DOT) {
+    for (int j0 = 0; j0 <= m; j0 += 8) {
+      double2 acc[8];
+#pragma unroll
+      for (int g = 0; g < 8; ++g) acc[g] = make_double2(0, 0);
+      for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
+        const float2 wv = w[i];
+#pragma unroll
+        for (int g = 0; g < 8; ++g) {
+          const int j = j0 + g;
+          if (j < m) {
+            const float2 v = V[j * ld + i];
+            acc[g].x += (double)v.x * wv.x + (double)v.y * wv.y;
+            acc[g].y += (double)v.x * wv.y - (double)v.y * wv.x;
+          } else if (j == m) {
+            acc[g].x += (double)wv.x * wv.x + (double)wv.y * wv.y;
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < 8; ++g) {
+        const int j = j0 + g;
+        if (j > m) break;  // uniform over the block
+        double2 v = acc[g];
+        for (int off = 32; off > 0; off >>= 1) {
+          v.x += __shfl_down(v.x, off, 64);
+          v.y += __shfl_down(v.y, off, 64);
+        }
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          double2 t = sh[0];
+          for (int q = 1; q < 4; ++q) {
+            t.x += sh[q].x;
+            t.y += sh[q].y;
+          }
+          partial[(long long)blockIdx.x * width + j] = t;
+        }
+      }
+    }
+  } else {
+    if (!UPDATE)
+      for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
+        const float2 wv = w[i];
+        nrm += (double)wv.x * wv.x + (double)wv.y * wv.y;
+      }
+    for (int off = 32; off > 0; off >>= 1) nrm += __shfl_down(nrm, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = make_double2(nrm, 0.0);
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sh[0].x + sh[1].x + sh[2].x + sh[3].x, 0.0);
+  }
+}
+
+// r = b - r (r holds A x on entry); partial[block] = block's |r|^2
+__global__ __launch_bounds__(256) void residual_partial_kernel(long long n, const double2* __restrict__ b,
+                                                               double2* __restrict__ r,
+                                                               double2* __restrict__ partial) {
   __shared__ double sh[4];
   double acc = 0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -79,111 +209,12 @@ __global__ __launch_bounds__(256) void residual_kernel(long long n, const double
   if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sh[0] + sh[1] + sh[2] + sh[3], 0.0);
 }
 
-// out[j] = sum over blocks of partial[b][j]: one 256-thread workgroup per column j, strided
-// loads then a tree (deterministic order)
-__global__ __launch_bounds__(256) void finish64_kernel(const double2* __restrict__ partial, int blocks, int width,
-                                                       int count, double2* __restrict__ out) {
-  __shared__ double2 sh[256];
-  const int j = blockIdx.x;
-  if (j >= count) return;
-  double2 s = make_double2(0, 0);
-  for (int b = threadIdx.x; b < blocks; b += 256) {
-    s.x += partial[(long long)b * width + j].x;
-    s.y += partial[(long long)b * width + j].y;
-  }
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) {
-      sh[threadIdx.x].x += sh[threadIdx.x + off].x;
-      sh[threadIdx.x].y += sh[threadIdx.x + off].y;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[j] = sh[0];
-}
-
-// partial[b][j] = sum over the block's slice of conj(V_j) w (j < m); partial[b][m] = |w|^2
-__global__ __launch_bounds__(256) void block_dot32_kernel(long long n, int m, const float2* __restrict__ V, long long ld,
-                                                          const float2* __restrict__ w, double2* __restrict__ partial,
-                                                          int width) {
-  __shared__ double2 sh[4];
-  const long long chunk = (n + gridDim.x - 1) / gridDim.x;
-  const long long beg = chunk * blockIdx.x;
-  const long long end = min(n, beg + chunk);
-  for (int j0 = 0; j0 <= m; j0 += 8) {
-    double2 acc[8];
-#pragma unroll
-    for (int g = 0; g < 8; ++g) acc[g] = make_double2(0, 0);
-    for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
-      const float2 wv = w[i];
-#pragma unroll
-      for (int g = 0; g < 8; ++g) {
-        const int j = j0 + g;
-        if (j < m) {
-          const float2 v = V[j * ld + i];
-          acc[g].x += (double)v.x * wv.x + (double)v.y * wv.y;
-          acc[g].y += (double)v.x * wv.y - (double)v.y * wv.x;
-        } else if (j == m) {
-          acc[g].x += (double)wv.x * wv.x + (double)wv.y * wv.y;
-        }
-      }
-    }
-#pragma unroll
-    for (int g = 0; g < 8; ++g) {
-      const int j = j0 + g;
-      if (j > m) break;  // uniform over the block
-      double2 v = acc[g];
-      for (int off = 32; off > 0; off >>= 1) {
-        v.x += __shfl_down(v.x, off, 64);
-        v.y += __shfl_down(v.y, off, 64);
-      }
-      __syncthreads();
-      if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double2 s = sh[0];
-        for (int q = 1; q < 4; ++q) {
-          s.x += sh[q].x;
-          s.y += sh[q].y;
-        }
-        partial[(long long)blockIdx.x * width + j] = s;
-      }
-    }
-  }
-}
-
-// w -= sum_j V_j h_j (coefficients FP64 on the device, applied in FP32); |w|^2 block partials
-__global__ __launch_bounds__(256) void block_update32_kernel(long long n, int m, const float2* __restrict__ V,
-                                                             long long ld, const double2* __restrict__ h,
-                                                             float2* __restrict__ w, double2* __restrict__ partial) {
-  __shared__ float2 hs[128];
-  __shared__ double sh[4];
-  for (int j = threadIdx.x; j < m; j += blockDim.x) hs[j] = make_float2((float)h[j].x, (float)h[j].y);
-  __syncthreads();
-  const long long chunk = (n + gridDim.x - 1) / gridDim.x;
-  const long long beg = chunk * blockIdx.x;
-  const long long end = min(n, beg + chunk);
-  double acc = 0;
-  for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
-    float2 t = make_float2(0.f, 0.f);
-    for (int j = 0; j < m; ++j) {
-      const float2 v = V[j * ld + i];
-      t.x = fmaf(v.x, hs[j].x, t.x);
-      t.x = fmaf(-v.y, hs[j].y, t.x);
-      t.y = fmaf(v.x, hs[j].y, t.y);
-      t.y = fmaf(v.y, hs[j].x, t.y);
-    }
-    float2 wv = w[i];
-    wv.x -= t.x;
-    wv.y -= t.y;
-    w[i] = wv;
-    acc += (double)wv.x * wv.x + (double)wv.y * wv.y;
-  }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sh[0] + sh[1] + sh[2] + sh[3], 0.0);
+// blocks of the fused reductions: about 1024 elements each
+static int fused_blocks(long long n) {
+  long long b = (n + 1023) / 1024;
+  if (b < 1) b = 1;
+  if (b > 512) b = 512;
+  return (int)b;
 }
 
 __global__ void scale32_kernel(long long n, float s, const float2* __restrict__ x, float2* __restrict__ y) {
@@ -191,22 +222,50 @@ __global__ void scale32_kernel(long long n, float s, const float2* __restrict__ 
     y[i] = make_float2(s * x[i].x, s * x[i].y);
 }
 
-// out = sum_j c_j Z_j (FP32 vectors, coefficients FP64 on the device)
-__global__ void lincomb32_kernel(long long n, int m, const float2* __restrict__ Z, long long ld,
-                                 const double2* __restrict__ c, float2* __restrict__ out) {
-  __shared__ float2 cs[128];
-  for (int j = threadIdx.x; j < m; j += blockDim.x) cs[j] = make_float2((float)c[j].x, (float)c[j].y);
-  __syncthreads();
+// out = sum_j c_j Z_j (FP32 vectors; the coefficients travel in the kernel arguments)
+struct LincombCoef {
+  float2 c[120];
+};
+
+// x = base + sum_j c_j Z_j (the combination in FP32, added in FP64; base may alias x)
+__global__ void lincomb_add32_kernel(long long n, int m, const float2* __restrict__ Z, long long ld, LincombCoef cs,
+                                     const double2* base, double2* x) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     float2 t = make_float2(0.f, 0.f);
     for (int j = 0; j < m; ++j) {
       const float2 v = Z[j * ld + i];
-      t.x = fmaf(v.x, cs[j].x, t.x);
-      t.x = fmaf(-v.y, cs[j].y, t.x);
-      t.y = fmaf(v.x, cs[j].y, t.y);
-      t.y = fmaf(v.y, cs[j].x, t.y);
+      t.x = fmaf(v.x, cs.c[j].x, t.x);
+      t.x = fmaf(-v.y, cs.c[j].y, t.x);
+      t.y = fmaf(v.x, cs.c[j].y, t.y);
+      t.y = fmaf(v.y, cs.c[j].x, t.y);
     }
-    out[i] = t;
+    const double2 b = base[i];
+    x[i] = make_double2(b.x + (double)t.x, b.y + (double)t.y);
+  }
+}
+
+// partial[block][0] = |a|^2, partial[block][1] = |b|^2 over the block's share (FP64)
+__global__ __launch_bounds__(256) void norm2_pair_partial_kernel(long long n, const double2* __restrict__ a,
+                                                                 const double2* __restrict__ b,
+                                                                 double2* __restrict__ partial) {
+  __shared__ double sh[2][4];
+  double sa = 0, sb = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    sa = fma(a[i].x, a[i].x, fma(a[i].y, a[i].y, sa));
+    sb = fma(b[i].x, b[i].x, fma(b[i].y, b[i].y, sb));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sa += __shfl_down(sa, off, 64);
+    sb += __shfl_down(sb, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = sa;
+    sh[1][threadIdx.x >> 6] = sb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partial[2 * (long long)blockIdx.x] = make_double2(sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3], 0.0);
+    partial[2 * (long long)blockIdx.x + 1] = make_double2(sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3], 0.0);
   }
 }
 
@@ -353,7 +412,7 @@ __device__ inline void kron_stage(const R* __restrict__ M, const C* in, C* out, 
 template <class R, class C, bool E_IN_LDS, bool FWD>
 __global__ __launch_bounds__(1024) void kron_kernel(const R* __restrict__ E, const R* __restrict__ lam,
                                                     const int* __restrict__ l1, const int* __restrict__ l2, int n,
-                                                    C a, C* __restrict__ psi) {
+                                                    C a, const C* in, C* psi, R in_scale, C* in_wb) {
   extern __shared__ unsigned char smem[];
   const int c = blockIdx.x;
   const int n2 = n * n;
@@ -374,7 +433,14 @@ __global__ __launch_bounds__(1024) void kron_kernel(const R* __restrict__ E, con
   const R* lam1 = lam + (long long)l1[c] * n;
   const R* lam2 = lam + (long long)l2[c] * n;
   C* x = psi + (long long)c * n2;
-  for (int i = threadIdx.x; i < n2; i += blockDim.x) B[i] = x[i];
+  const C* xin = in + (long long)c * n2;  // may alias x: read completely before any write
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+    C v = xin[i];
+    v.x *= in_scale;
+    v.y *= in_scale;
+    B[i] = v;
+    if (in_wb) in_wb[(long long)c * n2 + i] = v;  // the scaled input, in place of in
+  }
   __syncthreads();
   kron_stage<R, C, true, true, 0>(E1, B, T, n, lam1, lam2, a);              // T = E1^T B
   __syncthreads();
@@ -445,7 +511,8 @@ __device__ inline void kron_mfma_stage(const float* M, const float* in_re, const
 
 __global__ __launch_bounds__(1024) void kron_mfma_kernel(const float* __restrict__ E, const float* __restrict__ lam,
                                                          const int* __restrict__ l1, const int* __restrict__ l2,
-                                                         int n, float2 a, float2* __restrict__ psi) {
+                                                         int n, float2 a, const float2* in, float2* psi,
+                                                         float in_scale, float2* in_wb) {
   extern __shared__ float kl[];
   float* Bre = kl;
   float* Bim = Bre + 64 * KLD;
@@ -460,10 +527,12 @@ __global__ __launch_bounds__(1024) void kron_mfma_kernel(const float* __restrict
   const float* lam1 = lam + (long long)l1[c] * n;
   const float* lam2 = lam + (long long)l2[c] * n;
   float2* x = psi + (long long)c * n2;
+  const float2* xin = in + (long long)c * n2;  // may alias x: read completely before any write
   for (int idx = threadIdx.x; idx < 64 * 64; idx += blockDim.x) {
     const int i = idx >> 6, j = idx & 63;
     const bool ok = i < n && j < n;
-    const float2 v = ok ? x[i * n + j] : make_float2(0.f, 0.f);
+    const float2 v = ok ? make_float2(in_scale * xin[i * n + j].x, in_scale * xin[i * n + j].y) : make_float2(0.f, 0.f);
+    if (ok && in_wb) in_wb[(long long)c * n2 + i * n + j] = v;  // the scaled input, in place of in
     Bre[i * KLD + j] = v.x;
     Bim[i * KLD + j] = v.y;
     e1[i * KLD + j] = ok ? E1[i * n + j] : 0.f;
@@ -505,25 +574,28 @@ void kron_solve(const Kron& k, double2 a, double2* psi, hipStream_t s) { k.apply
 
 void Kron::apply64(double2 a, double2* psi, hipStream_t s) const {
   hipLaunchKernelGGL((kron_kernel<double, double2, false, false>), dim3(nchan), dim3(1024), kron_lds64(n), s, E64.ptr,
-                     lam64.ptr, l1.ptr, l2.ptr, n, a, psi);
+                     lam64.ptr, l1.ptr, l2.ptr, n, a, psi, psi, 1.0, (double2*)nullptr);
   PP_HIP(hipGetLastError());
 }
 
 void Kron::multiply64(double2 a, double2* psi, hipStream_t s) const {
   hipLaunchKernelGGL((kron_kernel<double, double2, false, true>), dim3(nchan), dim3(1024), kron_lds64(n), s, F64.ptr,
-                     lam64.ptr, l1.ptr, l2.ptr, n, a, psi);
+                     lam64.ptr, l1.ptr, l2.ptr, n, a, psi, psi, 1.0, (double2*)nullptr);
   PP_HIP(hipGetLastError());
 }
 
-void Kron::apply32(double2 a, float2* psi, hipStream_t s) const {
+void Kron::apply32(double2 a, float2* psi, hipStream_t s) const { apply32(a, psi, psi, s); }
+
+void Kron::apply32(double2 a, float2* in, float2* out, hipStream_t s, float in_scale, bool write_back) const {
+  float2* wb = write_back ? in : nullptr;
   if (n <= 64) {
     hipLaunchKernelGGL(kron_mfma_kernel, dim3(nchan), dim3(1024), kron_lds_mfma(), s, E32.ptr, lam32.ptr, l1.ptr,
-                       l2.ptr, n, make_float2((float)a.x, (float)a.y), psi);
+                       l2.ptr, n, make_float2((float)a.x, (float)a.y), in, out, in_scale, wb);
     PP_HIP(hipGetLastError());
     return;
   }
   hipLaunchKernelGGL((kron_kernel<float, float2, true, false>), dim3(nchan), dim3(1024), kron_lds32(n), s, E32.ptr,
-                     lam32.ptr, l1.ptr, l2.ptr, n, make_float2((float)a.x, (float)a.y), psi);
+                     lam32.ptr, l1.ptr, l2.ptr, n, make_float2((float)a.x, (float)a.y), in, out, in_scale, wb);
   PP_HIP(hipGetLastError());
 }
 
@@ -574,10 +646,13 @@ void MixedWork::ensure(long long n_, int m_) {
   Z.resize((size_t)m * n);
   w.resize(n);
   tmp.resize(n);
-  rhs.resize(n);
-  d.resize(n);
-  small.resize(4 * (m + 4));
-  coef.resize(m + 1);
+  if (!small) {
+    // results the host reads: kernels write them into mapped, coherent pinned memory, so a
+    // read is one stream synchronisation and no copy
+    PP_HIP(hipHostMalloc((void**)&small, kSmall * sizeof(double2), hipHostMallocMapped | hipHostMallocCoherent));
+    PP_HIP(hipHostGetDevicePointer((void**)&small_dev, small, 0));
+  }
+  PP_REQUIRE(2 * (m + 2) + 1 <= kSmall, "inner basis too large for the host result block");
   b64.resize(n);
   r64.resize(n);
   x64.resize(n);
@@ -586,61 +661,97 @@ void MixedWork::ensure(long long n_, int m_) {
 }
 
 void MixedWork::read_small(int count, hipStream_t s) {
-  host_small.resize(count);
-  PP_HIP(hipMemcpyAsync(host_small.data(), small.ptr, count * sizeof(double2), hipMemcpyDeviceToHost, s));
-  PP_HIP(hipStreamSynchronize(s));
+  PP_REQUIRE(count <= kSmall, "host result block too small");
+  PP_HIP(hipStreamSynchronize(s));  // the finishing kernels wrote straight into small
+  host_small.assign(small, small + count);
 }
 
-void MixedWork::block_dot32(int k, const float2* w_, double2* out, hipStream_t s) {
-  const int blocks = reduction_blocks(n);
+MixedWork::~MixedWork() {
+  if (small) (void)hipHostFree(small);
+}
+
+void MixedWork::norm2_pair(const double2* a, const double2* b, double2* out, hipStream_t s) {
+  const int blocks = fused_blocks(n);
+  red.ensure(blocks, 2);
+  hipLaunchKernelGGL(norm2_pair_partial_kernel, dim3(blocks), dim3(256), 0, s, n, a, b, red.partial.ptr);
+  hipLaunchKernelGGL(finish_partials_kernel, dim3(1), dim3(256), 0, s, blocks, red.partial.ptr, 2, 2, nullptr, 0, 0,
+                     nullptr, 0, 0, out);
+  PP_HIP(hipGetLastError());
+}
+
+// Classical Gram-Schmidt, two passes, against V_0..V_{k-1} in four launches:
+// out[0..k] = first dots (+|w|^2), out[k+1..2k+1] = second dots (+|w|^2), out[2k+2] = |w|^2
+// after both updates.
+void MixedWork::cgs2_32(int k, float2* w_, double2* out, hipStream_t s) {
+  const int blocks = fused_blocks(n);
   const int width = k + 1;
-  red.ensure(blocks, width);
-  hipLaunchKernelGGL(block_dot32_kernel, dim3(blocks), dim3(256), 0, s, n, k, V.ptr, n, w_, red.partial.ptr, width);
-  hipLaunchKernelGGL(finish64_kernel, dim3(width), dim3(256), 0, s, red.partial.ptr, blocks, width, width, out);
+  red.ensure(blocks, 2 * width + 1);
+  double2* p1 = red.partial.ptr;
+  double2* p2 = p1 + (size_t)blocks * width;
+  double2* p3 = p2 + (size_t)blocks * width;
+  const dim3 g(blocks), b(256);
+  hipLaunchKernelGGL((cgs32_kernel<false, true>), g, b, 0, s, n, k, V.ptr, n, nullptr, 0, w_, p1);
+  hipLaunchKernelGGL((cgs32_kernel<true, true>), g, b, 0, s, n, k, V.ptr, n, p1, width, w_, p2);
+  hipLaunchKernelGGL((cgs32_kernel<true, false>), g, b, 0, s, n, k, V.ptr, n, p2, width, w_, p3);
+  hipLaunchKernelGGL(finish_partials_kernel, dim3(1), b, 0, s, blocks, p1, width, width, p2, width, width, p3, 1, 1,
+                     out);
+  PP_HIP(hipGetLastError());
 }
 
-void MixedWork::block_update32(int k, const double2* h, float2* w_, double2* out, hipStream_t s) {
-  const int blocks = reduction_blocks(n);
+// out[0] = |w|^2 (w is only read)
+void MixedWork::norm2_32(const float2* w_, double2* out, hipStream_t s) {
+  const int blocks = fused_blocks(n);
   red.ensure(blocks, 1);
-  hipLaunchKernelGGL(block_update32_kernel, dim3(blocks), dim3(256), 0, s, n, k, V.ptr, n, h, w_, red.partial.ptr);
-  hipLaunchKernelGGL(finish64_kernel, dim3(1), dim3(256), 0, s, red.partial.ptr, blocks, 1, 1, out);
+  hipLaunchKernelGGL((cgs32_kernel<false, false>), dim3(blocks), dim3(256), 0, s, n, 0, V.ptr, n, nullptr, 0,
+                     const_cast<float2*>(w_), red.partial.ptr);
+  hipLaunchKernelGGL(finish_partials_kernel, dim3(1), dim3(256), 0, s, blocks, red.partial.ptr, 1, 1, nullptr, 0, 0,
+                     nullptr, 0, 0, out);
+  PP_HIP(hipGetLastError());
+}
+
+void MixedWork::residual(const double2* b64_, double2* r, double2* out, hipStream_t s) {
+  const int blocks = fused_blocks(n);
+  red.ensure(blocks, 1);
+  hipLaunchKernelGGL(residual_partial_kernel, dim3(blocks), dim3(256), 0, s, n, b64_, r, red.partial.ptr);
+  hipLaunchKernelGGL(finish_partials_kernel, dim3(1), dim3(256), 0, s, blocks, red.partial.ptr, 1, 1, nullptr, 0, 0,
+                     nullptr, 0, 0, out);
+  PP_HIP(hipGetLastError());
 }
 
 int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
-                        const std::function<void(int, const float2*, float2*)>& apply_P, const float2* b, float2* x,
-                        double rtol, int max_it, double* rel_out, hipStream_t s) {
+                        const std::function<void(int, float2*, float, float2*)>& apply_P, const float2* b,
+                        double rtol, int max_it, double* rel_out, hipStream_t s, double beta_in, const double2* base,
+                        double2* xout) {
   using cd = std::complex<double>;
   const int mm = std::min(max_it, m);
   PP_REQUIRE(mm >= 1 && mm + 3 <= 120, "inner basis size out of range");
-  block_dot32(0, b, small.ptr, s);
-  read_small(1, s);
-  const double beta = std::sqrt(host_small[0].x);
+  double beta = beta_in;
+  if (b) {
+    norm2_32(b, small_dev, s);
+    read_small(1, s);
+    beta = std::sqrt(host_small[0].x);
+  }
   if (!(beta > 0)) {
-    PP_HIP(hipMemsetAsync(x, 0, n * sizeof(float2), s));
+    if (base != xout) PP_HIP(hipMemcpyAsync(xout, base, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
     if (rel_out) *rel_out = 0;
     return 0;
   }
-  hipLaunchKernelGGL(scale32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, (float)(1.0 / beta), b, V.ptr);
+  if (b) hipLaunchKernelGGL(scale32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, (float)(1.0 / beta), b, V.ptr);
   std::vector<cd> H((size_t)(mm + 1) * mm, 0.0);  // column-major (mm+1) x mm
   auto Hc = [&](int row, int col) -> cd& { return H[(size_t)col * (mm + 1) + row]; };
   std::vector<cd> y;
   int k = 0;
   double rel = 1.0;
+  float next_scale = 1.f;  // V_j is normalised by the preconditioner launch of iteration j
   for (int j = 0; j < mm; ++j) {
     float2* vj = V.ptr + (size_t)j * n;
     float2* zj = Z.ptr + (size_t)j * n;
     float2* vn = V.ptr + (size_t)(j + 1) * n;
-    apply_P(j, vj, zj);
+    apply_P(j, vj, next_scale, zj);
     apply_A(zj, vn);
     // classical Gram-Schmidt, two passes, one host read
     const int w1 = j + 2;  // h (j+1) + |w|^2
-    double2* h1 = small.ptr;
-    double2* h2 = small.ptr + w1;
-    double2* nr = small.ptr + 2 * w1;
-    block_dot32(j + 1, vn, h1, s);
-    block_update32(j + 1, h1, vn, nr, s);
-    block_dot32(j + 1, vn, h2, s);
-    block_update32(j + 1, h2, vn, nr, s);
+    cgs2_32(j + 1, vn, small_dev, s);  // [first dots, |w|^2 | second dots, |w|^2 | final |w|^2]
     read_small(2 * w1 + 1, s);
     for (int i = 0; i <= j; ++i)
       Hc(i, j) = cd(host_small[i].x + host_small[w1 + i].x, host_small[i].y + host_small[w1 + i].y);
@@ -660,12 +771,11 @@ int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply
     }
     rel = std::sqrt(e2) / beta;
     if (rel <= rtol || !(hn > 1e-30 * beta)) break;
-    hipLaunchKernelGGL(scale32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, (float)(1.0 / hn), vn, vn);
+    next_scale = (float)(1.0 / hn);
   }
-  std::vector<double2> yc(k);
-  for (int i = 0; i < k; ++i) yc[i] = make_double2(y[i].real(), y[i].imag());
-  coef.upload(yc.data(), yc.size(), s);
-  hipLaunchKernelGGL(lincomb32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, k, Z.ptr, n, coef.ptr, x);
+  LincombCoef yc;
+  for (int i = 0; i < k; ++i) yc.c[i] = make_float2((float)y[i].real(), (float)y[i].imag());
+  hipLaunchKernelGGL(lincomb_add32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, k, Z.ptr, n, yc, base, xout);
   PP_HIP(hipGetLastError());
   if (rel_out) *rel_out = rel;
   return k;
@@ -685,56 +795,65 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   const double a = 0.5 * dt;
   const Kron& K = *kron;
   const double2 ka = make_double2(0.0, a);
-  // h = H psi ; b = S psi - i a h ; r = -2 i a h ; x = psi
-  vec_copy(n, psi, work.t64.ptr, s);
-  R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
-  if (fham) {
-    // The last residual of the previous step evaluated the radial products at the accepted
-    // solution; when psi still holds it (checked on the device) only the angular stage with
-    // the new time scalings runs.
-    const int* skip = nullptr;
-    if (z_valid && z_psi == psi && fham->gen64 == z_gen) {
-      if (flags.count == 0) flags.resize(2);
-      fham_flag_differs(n, psi, work.x64.ptr, flags.ptr, s);
-      hipLaunchKernelGGL(invert_flag_kernel, dim3(1), dim3(64), 0, s, flags.ptr, flags.ptr + 1);
-      skip = flags.ptr + 1;
-    }
-    fham->apply64(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), s, skip);
+  // with an overlap term in the factored operator, S + c H is one multiply
+  const bool fused_s = fham && fham->has_overlap;
+  // b = S psi - i a H psi ; r = b - A psi = -2 i a H psi ; x = psi
+  if (fused_s) {
+    // Both from one multiply.  The last residual of the previous step formed the radial
+    // products (overlap term included) at the accepted solution; when psi still holds it --
+    // compared on the device -- only the angular stage with the new time scalings runs.
+    const bool reuse = z_valid && z_psi == psi && fham->gen64 == z_gen;
+    FHamOut ob, orr;
+    ob.dst = work.b64.ptr;
+    ob.alpha = make_double2(0.0, -a);
+    ob.beta = make_double2(1.0, 0.0);
+    orr.dst = work.r64.ptr;
+    orr.alpha = make_double2(0.0, -2.0 * a);
+    orr.beta = make_double2(0.0, 0.0);
+    fham->apply64_ex(scal.data(), psi, ob, &orr, reuse ? work.x64.ptr : nullptr, s);
   } else {
-    H.apply(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), s);
+    vec_copy(n, psi, work.t64.ptr, s);
+    R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
+    if (fham) fham->apply64(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), s);
+    else H.apply(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), s);
+    vec_combine(n, make_double2(1.0, 0.0), work.t64.ptr, make_double2(0.0, -a), work.h64.ptr, work.b64.ptr, s);
+    vec_combine(n, make_double2(0.0, 0.0), work.h64.ptr, make_double2(0.0, -2.0 * a), work.h64.ptr, work.r64.ptr,
+                s);
   }
   z_valid = false;
   ++stats.outer_ops;
-  vec_combine(n, make_double2(1.0, 0.0), work.t64.ptr, make_double2(0.0, -a), work.h64.ptr, work.b64.ptr, s);
-  vec_combine(n, make_double2(0.0, 0.0), work.h64.ptr, make_double2(0.0, -2.0 * a), work.h64.ptr, work.r64.ptr, s);
-  vec_copy(n, psi, work.x64.ptr, s);
-  vec_norm2(n, work.b64.ptr, work.small.ptr, work.red, s);
-  vec_norm2(n, work.r64.ptr, work.small.ptr + 1, work.red, s);
+  work.norm2_pair(work.b64.ptr, work.r64.ptr, work.small_dev, s);
   work.read_small(2, s);
   const double bnorm = std::sqrt(work.host_small[0].x);
   double rn = std::sqrt(work.host_small[1].x);
 
   auto apply_A32 = [&](const float2* in, float2* out) {
     // out = S in + i a H in
-    work.overlap_multiply32(R, in, work.tmp.ptr, work.w.ptr, s);
-    if (fham) fham->apply32(scal.data(), in, out, work.tmp.ptr, make_double2(0.0, a), s);
-    else H.apply_f32(scal.data(), in, out, work.tmp.ptr, make_float2(0.f, (float)a), s);
+    if (fused_s) {
+      fham->apply32(scal.data(), in, out, nullptr, make_double2(0.0, a), s, make_double2(1.0, 0.0));
+    } else {
+      work.overlap_multiply32(R, in, work.tmp.ptr, work.w.ptr, s);
+      if (fham) fham->apply32(scal.data(), in, out, work.tmp.ptr, make_double2(0.0, a), s);
+      else H.apply_f32(scal.data(), in, out, work.tmp.ptr, make_float2(0.f, (float)a), s);
+    }
     ++stats.inner_ops;
   };
   int correction = 0;
-  auto apply_P32 = [&](int j, const float2* in, float2* out) {
+  // z = P (scale in); in is replaced by scale in (the normalised basis vector)
+  auto apply_P32 = [&](int j, float2* in, float scale, float2* out) {
     // the learned correction acts on the first direction of the first correction, whose
     // right-hand side r = -2 i a H psi is the kind of vector it was trained on
     if (j == 0 && correction == 0 && learned) {
+      if (scale != 1.f) hipLaunchKernelGGL(scale32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, scale, in, in);
       learned_apply(*learned, K, ka, in, out, s);
       ++stats.learned_applies;
     } else {
-      PP_HIP(hipMemcpyAsync(out, in, n * sizeof(float2), hipMemcpyDeviceToDevice, s));
-      K.apply32(ka, out, s);
+      K.apply32(ka, in, out, s, scale, scale != 1.f);
     }
     ++stats.precond_applies;
   };
   bool converged = false;
+  bool x_is_psi = true;  // x has not been formed yet: it equals psi
   for (int outer = 0;; ++outer) {
     stats.residual = rn;
     stats.rel_residual = bnorm > 0 ? rn / bnorm : 0.0;
@@ -744,28 +863,32 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     }
     if (outer >= max_outer || stats.inner_iterations > fallback_iterations) break;
     ++stats.outer_iterations;
-    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, work.r64.ptr, work.rhs.ptr);
+    // V_0 = r / |r| in FP32, with |r| the FP64 norm already on the host (the FP32 rounding
+    // changes it by ~1e-8 relative, far below the inner tolerance)
+    hipLaunchKernelGGL(f64_to_f32_scaled_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, 1.0 / rn, work.r64.ptr,
+                       work.V.ptr);
     double rel = 0;
-    stats.inner_iterations += work.fgmres32(apply_A32, apply_P32, work.rhs.ptr, work.d.ptr, inner_rtol, m, &rel, s);
+    // x = (psi or x) + d, d from the inner solve
+    stats.inner_iterations += work.fgmres32(apply_A32, apply_P32, nullptr, inner_rtol, m, &rel, s, rn,
+                                            x_is_psi ? psi : work.x64.ptr, work.x64.ptr);
+    x_is_psi = false;
     ++correction;
-    hipLaunchKernelGGL(add_f32_to_f64_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, work.d.ptr, work.x64.ptr);
     // r = b - (S x + i a H x)
-    vec_copy(n, work.x64.ptr, work.t64.ptr, s);
-    R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
-    if (fham) fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
-    else H.apply(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
+    if (fused_s) {
+      fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, nullptr, make_double2(0.0, a), s, make_double2(1.0, 0.0));
+    } else {
+      vec_copy(n, work.x64.ptr, work.t64.ptr, s);
+      R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
+      if (fham) fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
+      else H.apply(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
+    }
     ++stats.outer_ops;
-    const int blocks = reduction_blocks(n);
-    work.red.ensure(blocks, 1);
-    hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(256), 0, s, n, work.b64.ptr, work.r64.ptr,
-                       work.red.partial.ptr);
-    hipLaunchKernelGGL(finish64_kernel, dim3(1), dim3(256), 0, s, work.red.partial.ptr, blocks, 1, 1, work.small.ptr);
-    PP_HIP(hipGetLastError());
+    work.residual(work.b64.ptr, work.r64.ptr, work.small_dev, s);
     work.read_small(1, s);
     rn = std::sqrt(work.host_small[0].x);
   }
-  if (converged && fham && stats.outer_iterations > 0) {
-    // W64 now holds the radial products of x64, which becomes psi below
+  if (converged && fused_s && !x_is_psi) {
+    // W64 now holds the radial products (overlap term included) of x64, which becomes psi below
     z_valid = true;
     z_gen = fham->gen64;
     z_psi = psi;
@@ -777,7 +900,7 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     stats.fallback = 1;
     return;
   }
-  vec_copy(n, work.x64.ptr, psi, s);
+  if (!x_is_psi) vec_copy(n, work.x64.ptr, psi, s);
 }
 
 }  // namespace pp

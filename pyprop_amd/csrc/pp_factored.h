@@ -10,6 +10,13 @@
 namespace pp {
 
 constexpr int kMaxFTerms = 32;
+constexpr int kFhamRadialVariants = 7;  // radial-stage kernel shapes (terms x channel groups per wave, prefetch)
+
+struct FHamOut {
+  double2* dst = nullptr;
+  const double2* addend = nullptr;
+  double2 alpha = make_double2(1.0, 0.0), beta = make_double2(0.0, 0.0);
+};
 
 struct FHam {
   int nc = 0, n = 0, nterms = 0, bw = 0, npot = 0, Kp = 0;  // Kp: angular GEMM K = nterms * nc rounded up to 4
@@ -20,18 +27,28 @@ struct FHam {
   mutable DeviceBuffer<float> W32;
   mutable DeviceBuffer<double> X64;   // channel-minor copy of the input
   mutable DeviceBuffer<float> X32;
-  // dst = addend + alpha * sum_t scalings[term_pot[t]] (A_t (x) R_t) src
-  // skip_radial (device int, may be NULL): when it reads non-zero, the radial products of
-  // the previous FP64 multiply are reused (src must then hold the same values as that call's)
+  // Optional overlap term: radial term index nterms holds S1 (x) S2 (identity angular part),
+  // added with its own coefficient beta outside the angular contraction.
+  bool has_overlap = false;
+  int rad_variant64 = 0, rad_variant32 = 0;  // radial kernel shape per precision
+  mutable unsigned long long gen64 = 0;      // FP64 multiplies issued
+  mutable int w64_terms = 0;                 // radial terms the FP64 products W64 are known to hold
+  mutable DeviceBuffer<int> reuse_flag;      // set by the input comparison, re-armed by the angular stage
+  // dst = addend + alpha * sum_t scalings[term_pot[t]] (A_t (x) R_t) src + beta (S1 (x) S2) src
+  // (beta must be 0 without an overlap term; the overlap radial product is only formed when
+  // a beta is non-zero)
   void apply64(const double* scalings, const double2* src, double2* dst, const double2* addend, double2 alpha,
-               hipStream_t s, const int* skip_radial = nullptr) const;
-  mutable unsigned long long gen64 = 0;  // FP64 multiplies issued (invalidates reuse)
+               hipStream_t s, double2 beta = make_double2(0.0, 0.0)) const;
   void apply32(const double* scalings, const float2* src, float2* dst, const float2* addend, double2 alpha,
-               hipStream_t s) const;
+               hipStream_t s, double2 beta = make_double2(0.0, 0.0)) const;
+  // FP64 multiply with up to two outputs from the same products (o2 may be NULL and takes no
+  // addend).  reuse_ref (may be NULL): the input of an earlier FP64 multiply whose radial
+  // products W64 still holds (covering the terms needed now); the radial stage is skipped when
+  // src equals it element for element, checked on the device.
+  void apply64_ex(const double* scalings, const double2* src, const FHamOut& o1, const FHamOut* o2,
+                  const double2* reuse_ref, hipStream_t s) const;
 };
 
-// *flag = 1 if a and b differ anywhere (n elements), else 0
-void fham_flag_differs(long long n, const double2* a, const double2* b, int* flag, hipStream_t s);
 
 }  // namespace pp
 

@@ -26,6 +26,8 @@ struct Kron {
   // psi <- K^-1 psi, a = complex factor of the one-electron part (i dt / 2)
   void apply64(double2 a, double2* psi, hipStream_t s) const;
   void apply32(double2 a, float2* psi, hipStream_t s) const;
+  // out <- K^-1 (in_scale in) (may alias in); write_back: in <- in_scale in as well
+  void apply32(double2 a, float2* in, float2* out, hipStream_t s, float in_scale = 1.f, bool write_back = false) const;
   // psi <- K psi (FP64)
   void multiply64(double2 a, double2* psi, hipStream_t s) const;
 };
@@ -45,25 +47,38 @@ struct MixedStats {
 };
 
 struct MixedWork {
+  static constexpr int kSmall = 256;
   long long n = -1;
   int m = 0;
-  DeviceBuffer<float2> V, Z, w, tmp, rhs, d;
-  DeviceBuffer<double2> small, coef, b64, r64, x64, t64, h64;
+  DeviceBuffer<float2> V, Z, w, tmp;
+  DeviceBuffer<double2> b64, r64, x64, t64, h64;
+  double2* small = nullptr;      // kSmall results for the host (pinned, mapped, coherent)
+  double2* small_dev = nullptr;  // its device address
   Reducer red;
   std::vector<double2> host_small;
+  MixedWork() = default;
+  MixedWork(const MixedWork&) = delete;
+  MixedWork& operator=(const MixedWork&) = delete;
+  ~MixedWork();
   void ensure(long long n, int m);
-  void read_small(int count, hipStream_t s);
-  // out[0..k-1] = V_i^H w (FP64 accumulation), out[k] = |w|^2
-  void block_dot32(int k, const float2* w, double2* out, hipStream_t s);
-  // w -= sum_i V_i h_i, out[0] = |w|^2
-  void block_update32(int k, const double2* h, float2* w, double2* out, hipStream_t s);
+  void read_small(int count, hipStream_t s);  // synchronise s, copy small[0..count) to host_small
+  void norm2_pair(const double2* a, const double2* b, double2* out, hipStream_t s);  // |a|^2, |b|^2
+  // Two-pass classical Gram-Schmidt of w against V_0..V_{k-1} (FP64 sums): out[0..k] = first
+  // dots and |w|^2, out[k+1..2k+1] = second dots and |w|^2, out[2k+2] = final |w|^2
+  void cgs2_32(int k, float2* w, double2* out, hipStream_t s);
+  void norm2_32(const float2* w, double2* out, hipStream_t s);  // out[0] = |w|^2
+  // r = b - r, out[0] = |r|^2 (FP64)
+  void residual(const double2* b, double2* r, double2* out, hipStream_t s);
   // out = S x S in (FP32, out-of-place; scratch has n elements)
   void overlap_multiply32(const Repr& r, const float2* in, float2* out, float2* scratch, hipStream_t s);
   std::vector<DeviceBuffer<float>> u32 = std::vector<DeviceBuffer<float>>(kMaxRank);
-  // flexible GMRES(m) on A d = rhs, x0 = 0, right preconditioner P_j (j = iteration)
+  // Flexible GMRES(m) on A d = rhs, d0 = 0, right preconditioner P_j (j = iteration), then
+  // xout = base + d (FP64; base may alias xout).  rhs = NULL: V_0 already holds rhs / |rhs|
+  // and beta = |rhs|.  apply_P(j, v, scale, z): v <- scale v (normalising the basis vector),
+  // z = P_j v.
   int fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
-               const std::function<void(int, const float2*, float2*)>& apply_P, const float2* rhs, float2* d,
-               double rtol, int max_it, double* rel_out, hipStream_t s);
+               const std::function<void(int, float2*, float, float2*)>& apply_P, const float2* rhs, double rtol,
+               int max_it, double* rel_out, hipStream_t s, double beta, const double2* base, double2* xout);
 };
 
 struct MixedCayley {
@@ -83,7 +98,6 @@ struct MixedCayley {
   bool z_valid = false;
   unsigned long long z_gen = 0;
   const double2* z_psi = nullptr;
-  DeviceBuffer<int> flags;
   std::vector<double> scal;
   void step(double2* psi, double dt, const double* scalings, hipStream_t s);
 };

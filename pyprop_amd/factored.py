@@ -69,7 +69,9 @@ def factor_potentials(potentials: Sequence, nchan: int, n: int):
 class FactoredHamiltonian:
     """Device factored Hamiltonian over a BasisPropagator's potentials (same scalings)."""
 
-    def __init__(self, base):
+    def __init__(self, base, overlap: bool = True):
+        """overlap: also hold S1 (x) S2 of the radial ranks as a term, so that MultiplyOverlap
+        (and the mixed-precision Cayley step) forms S + c H in one multiply."""
         shape = base.psi.repr.shape
         if len(shape) != 3 or shape[1] != shape[2]:
             raise ValueError("the factored multiply needs psi shaped (channels, n, n)")
@@ -85,6 +87,13 @@ class FactoredHamiltonian:
         N.check(N.load().pp_fham_create(nchan, n, self.nterms, bw, D.hptr(A), D.hptr(tpot), D.hptr(dia),
                                         len(base.PotentialList), C.byref(h)))
         self.h = h
+        self.has_overlap = False
+        bases = base.psi.repr.bases
+        if overlap and bases[1] is not None and bases[2] is not None:
+            S1 = np.ascontiguousarray(bases[1].overlap_band(), dtype=np.float64)
+            S2 = np.ascontiguousarray(bases[2].overlap_band(), dtype=np.float64)
+            N.check(N.load().pp_fham_set_overlap(self.h, D.hptr(S1), D.hptr(S2)))
+            self.has_overlap = True
 
     def Multiply(self, src, dst, t, addend=None, alpha=1.0):
         """dst = addend + alpha * H(t) src (complex128 or complex64 device tensors)."""
@@ -94,6 +103,38 @@ class FactoredHamiltonian:
         a = None if addend is None else getattr(addend, "data", addend)
         fn = N.load().pp_fham_multiply_f32 if s.dtype == D.torch.complex64 else N.load().pp_fham_multiply
         N.check(fn(self.h, D.hptr(sc), D.ptr(s), D.ptr(d), D.ptr(a), N.cplx(alpha), D.stream_handle()))
+
+    # (terms, channel groups of 8, next band step prefetched) per wave of each radial variant
+    def MultiplyOverlap(self, src, dst, t, alpha=1.0, beta=1.0, addend=None):
+        """dst = addend + alpha * H(t) src + beta * (S (x) S) src (needs the overlap term)."""
+        sc = self.base.scalings(t)
+        s = getattr(src, "data", src)
+        d = getattr(dst, "data", dst)
+        a = None if addend is None else getattr(addend, "data", addend)
+        fn = (N.load().pp_fham_multiply_overlap_f32 if s.dtype == D.torch.complex64
+              else N.load().pp_fham_multiply_overlap)
+        N.check(fn(self.h, D.hptr(sc), D.ptr(s), D.ptr(d), D.ptr(a), N.cplx(alpha), N.cplx(beta),
+                   D.stream_handle()))
+
+    RADIAL_SHAPES = ((1, 1, 0), (2, 2, 0), (3, 2, 0), (3, 3, 0), (2, 2, 1), (3, 2, 1), (3, 3, 1))
+
+    def set_radial_variant(self, precision: int, variant: int):
+        """Radial-stage kernel shape for the complex128 (precision 64) or complex64 (32)
+        multiply: index into RADIAL_SHAPES."""
+        N.check(N.load().pp_fham_set_radial_variant(self.h, precision, variant))
+
+    def radial_variant(self, precision: int) -> int:
+        v = C.c_int()
+        N.check(N.load().pp_fham_get_radial_variant(self.h, precision, C.byref(v)))
+        return v.value
+
+    def tune(self, precision: int, trials: int = 5):
+        """Time every radial variant on the device and keep the fastest; returns (best,
+        {shape: ms per multiply})."""
+        best = C.c_int()
+        tim = np.zeros(len(self.RADIAL_SHAPES))
+        N.check(N.load().pp_fham_tune(self.h, precision, trials, C.byref(best), D.hptr(tim), D.stream_handle()))
+        return best.value, dict(zip(self.RADIAL_SHAPES, tim.tolist()))
 
     def __del__(self):
         if getattr(self, "h", None) is not None and N._lib is not None:

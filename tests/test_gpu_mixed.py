@@ -220,14 +220,20 @@ def test_cayley_with_kron_preconditioner_vs_dense(torch, rng):
     _check_solution(psi.numpy(), sol, A, b, 1e-11, 1e-8)
 
 
-def test_mixed_cayley_vs_dense(torch, rng):
+@pytest.mark.parametrize("operator", ["tensor", "factored", "factored+overlap"])
+def test_mixed_cayley_vs_dense(torch, rng, operator):
+    """Mixed-precision Cayley steps against the dense FP64 solve, with the tensor-potential
+    multiply, the factored multiply plus separate overlap kernels, and the factored multiply
+    carrying the overlap term (S + i a H in one multiply)."""
+    from pyprop_amd.factored import FactoredHamiltonian
     from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
-    say("mixed-precision Cayley vs dense FP64 solve")
+    say("mixed-precision Cayley vs dense FP64 solve (%s)" % operator)
     sysm = _small_helium()
     repr_, psi, base = _device(sysm, torch)
     x0 = crandn(rng, sysm.shape)
     psi.data.copy_(torch.from_numpy(x0).to("cuda"))
-    prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-11)
+    fh = None if operator == "tensor" else FactoredHamiltonian(base, overlap=operator.endswith("overlap"))
+    prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-11, factored=fh)
     dt, t = 0.01, 40.0
     cur = x0
     for k in range(3):
@@ -426,6 +432,81 @@ def test_factored_multiply_vs_dense(torch, rng):
     assert rel(d32.cpu().numpy().ravel(), ref) < 2e-6
 
 
+def test_factored_overlap_term_vs_dense(torch, rng):
+    """dst = addend + alpha H src + beta (S x S) src in one multiply (FP64 and FP32), against
+    the dense operators; beta = 0 and alpha = 0 isolate each part."""
+    from pyprop_amd.factored import FactoredHamiltonian
+    from tests.test_oracle import dense_operator
+    say("factored multiply with the overlap term vs dense (small helium)")
+    sysm = _small_helium()
+    repr_, psi, base = _device(sysm, torch)
+    base.SetupStep()
+    fh = FactoredHamiltonian(base)
+    assert fh.has_overlap
+    t = 55.0
+    H, S = dense_operator(sysm, base.scalings(t))
+    x = crandn(rng, sysm.shape)
+    y0 = crandn(rng, sysm.shape)
+    src = torch.from_numpy(x).to("cuda")
+    add = torch.from_numpy(y0).to("cuda")
+    for alpha, beta, addend in ((0.37j, 1.0, None), (0.2 - 0.1j, 0.5 + 0.25j, add), (0.0, 1.0, None),
+                                (1.0, 0.0, add)):
+        ref = alpha * (H @ x.ravel()) + beta * (S @ x.ravel())
+        if addend is not None:
+            ref = ref + y0.ravel()
+        d = torch.empty_like(src)
+        fh.MultiplyOverlap(src, d, t, alpha, beta, addend)
+        e64 = rel(d.cpu().numpy().ravel(), ref)
+        d32 = torch.empty_like(src, dtype=torch.complex64)
+        fh.MultiplyOverlap(src.to(torch.complex64), d32, t, alpha, beta,
+                           None if addend is None else addend.to(torch.complex64))
+        e32 = rel(d32.cpu().numpy().ravel(), ref)
+        say("  alpha %s beta %s addend %s: FP64 %.2e FP32 %.2e" % (alpha, beta, addend is not None, e64, e32))
+        assert e64 < 1e-12 and e32 < 2e-6
+    # without the term, a non-zero beta is refused
+    fh2 = FactoredHamiltonian(base, overlap=False)
+    with pytest.raises(Exception):
+        fh2.MultiplyOverlap(src, torch.empty_like(src), t, 1.0, 1.0)
+
+
+def test_factored_radial_variants_vs_dense(torch, rng):
+    """Every radial kernel shape (terms x channel groups per wave) against the dense operator.
+    The small system has 13 channels (2 groups of 8, the second partly empty; 3 groups per
+    wave pads a whole group) and 14 terms (3 or 6 terms per wave pad the last wave)."""
+    from pyprop_amd.factored import FactoredHamiltonian
+    from pyprop_amd.systems import helium_system
+    from tests.test_oracle import dense_operator
+    say("factored radial variants vs dense (small helium)")
+    sysm = helium_system(lmax=3, L=(0, 1), order=5, n_breakpoints=12, rmax=15.0, n_inner=6, gamma=2.5)
+    repr_, psi, base = _device(sysm, torch)
+    base.SetupStep()
+    fh = FactoredHamiltonian(base)
+    nchan = sysm.shape[0]
+    say("  %d channels, %d terms" % (nchan, fh.nterms))
+    assert nchan == 13 and fh.nterms == 14
+    t = 77.0
+    H, _ = dense_operator(sysm, base.scalings(t))
+    x = crandn(rng, sysm.shape)
+    ref = H @ x.ravel()
+    src = torch.from_numpy(x).to("cuda")
+    s32 = src.to(torch.complex64)
+    for v, shape in enumerate(fh.RADIAL_SHAPES):
+        fh.set_radial_variant(64, v)
+        fh.set_radial_variant(32, v)
+        assert fh.radial_variant(64) == v and fh.radial_variant(32) == v
+        d = torch.empty_like(src)
+        fh.Multiply(src, d, t)
+        d32 = torch.empty_like(s32)
+        fh.Multiply(s32, d32, t)
+        e64 = rel(d.cpu().numpy().ravel(), ref)
+        e32 = rel(d32.cpu().numpy().ravel(), ref)
+        say("  shape %s: FP64 %.2e  FP32 %.2e" % (shape, e64, e32))
+        assert e64 < 1e-12 and e32 < 2e-6
+    best, tim = fh.tune(64, trials=2)
+    assert 0 <= best < len(fh.RADIAL_SHAPES) and fh.radial_variant(64) == best
+    assert all(ms > 0 for ms in tim.values())
+
+
 def test_factored_multiply_full_size_vs_tensor_potentials(torch, rng):
     """Full helium: factored multiply against the fused tensor-potential multiply (itself
     pinned to the oracle and to scipy.sparse at 1e-12)."""
@@ -451,6 +532,18 @@ def test_factored_multiply_full_size_vs_tensor_potentials(torch, rng):
     e32 = (torch.linalg.norm(g32.to(torch.complex128) - ref) / torch.linalg.norm(ref)).item()
     say("  FP64 %.2e  FP32 %.2e" % (e64, e32))
     assert e64 < 1e-12 and e32 < 2e-6
+    for prec in (64, 32):
+        best, tim = fh.tune(prec, trials=3)
+        say("  tune %d: best %s, %s" % (prec, fh.RADIAL_SHAPES[best],
+                                        ", ".join("%s %.1f us" % (k, 1e3 * v) for k, v in tim.items())))
+    for v in range(len(fh.RADIAL_SHAPES)):
+        fh.set_radial_variant(64, v)
+        fh.set_radial_variant(32, v)
+        fh.Multiply(x, got, t)
+        fh.Multiply(x.to(torch.complex64), g32, t)
+        e64 = (torch.linalg.norm(got - ref) / torch.linalg.norm(ref)).item()
+        e32 = (torch.linalg.norm(g32.to(torch.complex128) - ref) / torch.linalg.norm(ref)).item()
+        assert e64 < 1e-12 and e32 < 2e-6, (v, e64, e32)
 
 
 def test_mixed_cayley_factored_full_size_vs_plain(torch):
