@@ -795,7 +795,7 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   const double a = 0.5 * dt;
   const Kron& K = *kron;
   const double2 ka = make_double2(0.0, a);
-  // with an overlap term in the factored operator, S + c H is one multiply
+  // with an overlap term in the factored operator, the FP64 S + c H is one multiply
   const bool fused_s = fham && fham->has_overlap;
   // b = S psi - i a H psi ; r = b - A psi = -2 i a H psi ; x = psi
   if (fused_s) {
@@ -828,14 +828,13 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   double rn = std::sqrt(work.host_small[1].x);
 
   auto apply_A32 = [&](const float2* in, float2* out) {
-    // out = S in + i a H in
-    if (fused_s) {
-      fham->apply32(scal.data(), in, out, nullptr, make_double2(0.0, a), s, make_double2(1.0, 0.0));
-    } else {
-      work.overlap_multiply32(R, in, work.tmp.ptr, work.w.ptr, s);
-      if (fham) fham->apply32(scal.data(), in, out, work.tmp.ptr, make_double2(0.0, a), s);
-      else H.apply_f32(scal.data(), in, out, work.tmp.ptr, make_float2(0.f, (float)a), s);
-    }
+    // out = S in + i a H in.  S stays in its factored form U^T U here even when the factored
+    // operator holds S (x) S: in FP32, S (x) S applied entry by entry loses its small
+    // eigenvalues (relative error ~ eps32 cond(S (x) S)), and the refinement then cannot
+    // reduce the error in those directions, where the residual barely sees it.
+    work.overlap_multiply32(R, in, work.tmp.ptr, work.w.ptr, s);
+    if (fham) fham->apply32(scal.data(), in, out, work.tmp.ptr, make_double2(0.0, a), s);
+    else H.apply_f32(scal.data(), in, out, work.tmp.ptr, make_float2(0.f, (float)a), s);
     ++stats.inner_ops;
   };
   int correction = 0;

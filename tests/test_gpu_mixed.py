@@ -547,13 +547,17 @@ def test_factored_multiply_full_size_vs_tensor_potentials(torch, rng):
 
 
 def test_mixed_cayley_factored_full_size_vs_plain(torch):
-    """Full helium, 10 steps: mixed FP32 + factored multiply vs the plain FP64 path."""
+    """Full helium, 20 steps: mixed FP32 + factored multiply (with its FP64 overlap term)
+    against the plain FP64 path converged to 1e-14.  The bar, 2e-10 after 20 steps, is about
+    ten times the measured 2e-11: a solver whose refinement stalls in the small-eigenvalue
+    directions of S (x) S (an FP32 operator that applies S (x) S entry by entry does) passes a
+    residual test but drifts by ~1e-10 per step, and fails this."""
     from pyprop_amd.factored import FactoredHamiltonian
     from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
     from pyprop_amd.propagator import (BasisPropagator, CayleyPropagator, CombinedRepresentation,
                                        RadialTwoElectronPreconditioner, Wavefunction)
     from pyprop_amd.systems import helium_initial_state, helium_system
-    say("full-size helium: mixed+factored vs plain FP64, 10 steps")
+    say("full-size helium: mixed+factored vs plain FP64 (tol 1e-14), 20 steps")
     sysm = helium_system()
     repr_ = CombinedRepresentation(sysm.shape, sysm.bases)
     psi = Wavefunction(repr_, helium_initial_state(sysm))
@@ -561,20 +565,21 @@ def test_mixed_cayley_factored_full_size_vs_plain(torch):
     psi2 = psi.Copy()
     base = BasisPropagator(psi, sysm.potentials)
     dt = 0.01
-    plain = CayleyPropagator(psi, base, RadialTwoElectronPreconditioner(psi, sysm), krylov_basis_size=30,
-                             krylov_tolerance=1e-10)
+    plain = CayleyPropagator(psi, base, RadialTwoElectronPreconditioner(psi, sysm), krylov_basis_size=60,
+                             krylov_tolerance=1e-14)
     plain.SetupStep(dt)
     fh = FactoredHamiltonian(base)
+    assert fh.has_overlap
     mixed = MixedCayleyPropagator(psi2, base, KroneckerPreconditioner(psi2, sysm), tolerance=1e-10, factored=fh)
     t = 100.0
-    for k in range(10):
+    for k in range(20):
         plain.AdvanceStep(t, dt)
         mixed.AdvanceStep(t, dt)
         t += dt
     say("  last step stats %s" % mixed.GetStats())
     err = rel(psi2.numpy(), psi.numpy())
     say("  wavefunction difference %.2e" % err)
-    assert err < 1e-8
+    assert err < 2e-10
 
 
 def test_mixed_factored_reuse_detects_modified_psi(torch, rng):
