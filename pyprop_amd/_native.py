@@ -106,6 +106,8 @@ SIGNATURES = [
     ("pp_fham_multiply_f32", I, [P, P, P, P, P, PPCplx, P]),
     ("pp_fham_set_radial_variant", I, [P, I, I]),
     ("pp_fham_get_radial_variant", I, [P, I, P]),
+    ("pp_fham_set_angular_waves", I, [P, I, I]),
+    ("pp_fham_get_angular_waves", I, [P, I, P]),
     ("pp_fham_tune", I, [P, I, I, P, P, P]),
     ("pp_fham_set_overlap", I, [P, P, P]),
     ("pp_fham_multiply_overlap", I, [P, P, P, P, P, PPCplx, PPCplx, P]),

@@ -237,8 +237,12 @@ __global__ __launch_bounds__(64) void fham_radial_reuse_kernel(int nc, int ncp, 
       load_x(jn, an);
 #pragma unroll
       for (int u = 0; u < TG; ++u) load_b(jn, u, bn[u]);
+      // keep the next step's loads issued here, ahead of this step's MFMAs (the scheduler
+      // would otherwise sink them towards their first use)
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int u = 0; u < TG; ++u) mma_all(av, u, bv[u]);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int g = 0; g < GG; ++g)
 #pragma unroll
@@ -287,6 +291,8 @@ __global__ __launch_bounds__(64) void fham_radial_reuse_kernel(int nc, int ncp, 
 constexpr int kRadTG[kFhamRadialVariants] = {1, 2, 3, 3, 2, 3, 3};
 constexpr int kRadGG[kFhamRadialVariants] = {1, 2, 2, 3, 2, 2, 3};
 constexpr bool kRadPipe[kFhamRadialVariants] = {false, false, false, false, true, true, true};
+// waves splitting K in the angular stage, per angular variant
+constexpr int kAngWaveChoices[kFhamAngularVariants] = {16, 8, 4};
 
 template <class T, int B>
 static void launch_radial(int variant, int nc, int ncp, int n, int nterms, int ngroups, int tiles, const T* Xt,
@@ -312,14 +318,13 @@ static void launch_radial(int variant, int nc, int ncp, int n, int nterms, int n
   }
 }
 
-// Stage 2.  Workgroup of kAngWaves waves per (16 rows c', 16 complex positions p); the waves
+// Stage 2.  Workgroup of W waves per (16 rows c', 16 complex positions p); the waves
 // split K = T nc and reduce through LDS.  A: [Mp][K] (row c', column t*nc + c).  The grid is
 // one-dimensional over nrb row blocks x position tiles, renumbered so that the row blocks of
 // one position tile run on one XCD together and read their Z columns through its L2.
-constexpr int kAngWaves = 16;
 constexpr int kAngBatch = 12;
-template <class T>
-__global__ __launch_bounds__(64 * kAngWaves) void fham_angular_kernel(int nc, int nrb, int K, int Kreal, long long P,
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) void fham_angular_kernel(int nc, int nrb, int K, int Kreal, long long P,
                                                                       const T* __restrict__ A,
                                                                       const typename MfmaT<T>::C2* __restrict__ Z,
                                                                       FScales<T> sc,
@@ -334,7 +339,7 @@ __global__ __launch_bounds__(64 * kAngWaves) void fham_angular_kernel(int nc, in
                                                                       int* __restrict__ reset) {
   typedef typename MfmaT<T>::V4 V4;
   typedef typename MfmaT<T>::C2 C2;
-  __shared__ V4 red[kAngWaves - 1][2][64];
+  __shared__ V4 red[W - 1][2][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const long long nblocks = (long long)nrb * ((P + 15) / 16);
@@ -346,7 +351,7 @@ __global__ __launch_bounds__(64 * kAngWaves) void fham_angular_kernel(int nc, in
   const int m0 = (int)(L % nrb) * 16;
   const long long p0 = (L / nrb) * 16;
   const int nks = K >> 2;
-  const int per = (nks + kAngWaves - 1) / kAngWaves;
+  const int per = (nks + W - 1) / W;
   const int ks0 = wave * per, ks1 = min(nks, ks0 + per);
   const long long pc = p0 + lr;
   const bool col_ok = pc < P;
@@ -379,7 +384,7 @@ __global__ __launch_bounds__(64 * kAngWaves) void fham_angular_kernel(int nc, in
   __syncthreads();
   if (wave != 0) return;
 #pragma unroll
-  for (int q = 0; q < kAngWaves - 1; ++q) {
+  for (int q = 0; q < W - 1; ++q) {
     are += red[q][0][lane];
     aim += red[q][1][lane];
   }
@@ -468,11 +473,18 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   const long long ablocks = (long long)nrb * ceil_div(P, 16);
   typedef typename MfmaT<T>::C2 C2;
   const C2* Zc = reinterpret_cast<const C2*>(Zb.ptr);
-  hipLaunchKernelGGL(fham_angular_kernel<T>, dim3((unsigned)((ablocks + 7) / 8 * 8)), dim3(64 * kAngWaves), 0, s, nc,
-                     nrb, F.Kp, nterms * nc, P, Ab.ptr, Zc, sc, o1.addend, to_c2<T>(o1.alpha),
-                     with_s ? Zc + (size_t)nterms * nc * P : nullptr, to_c2<T>(o1.beta), o1.dst,
-                     o2 ? o2->dst : nullptr, to_c2<T>(o2 ? o2->alpha : make_double2(0, 0)),
-                     to_c2<T>(o2 ? o2->beta : make_double2(0, 0)), differs);
+  const dim3 agrid((unsigned)((ablocks + 7) / 8 * 8));
+  const int waves = kAngWaveChoices[sizeof(T) == 8 ? F.ang_variant64 : F.ang_variant32];
+#define PP_ANG(W)                                                                                                    \
+  hipLaunchKernelGGL((fham_angular_kernel<T, W>), agrid, dim3(64 * W), 0, s, nc, nrb, F.Kp, nterms * nc, P, Ab.ptr, \
+                     Zc, sc, o1.addend, to_c2<T>(o1.alpha), with_s ? Zc + (size_t)nterms * nc * P : nullptr,       \
+                     to_c2<T>(o1.beta), o1.dst, o2 ? o2->dst : nullptr,                                              \
+                     to_c2<T>(o2 ? o2->alpha : make_double2(0, 0)), to_c2<T>(o2 ? o2->beta : make_double2(0, 0)),   \
+                     differs)
+  if (waves == 4) PP_ANG(4);
+  else if (waves == 8) PP_ANG(8);
+  else PP_ANG(16);
+#undef PP_ANG
   PP_HIP(hipGetLastError());
 }
 
@@ -634,6 +646,24 @@ extern "C" int pp_fham_get_radial_variant(const pp_fham* f, int precision, int* 
   });
 }
 
+extern "C" int pp_fham_set_angular_waves(pp_fham* f, int precision, int waves) {
+  return guarded([&] {
+    PP_REQUIRE(f && (precision == 64 || precision == 32), "precision must be 64 or 32");
+    int v = -1;
+    for (int i = 0; i < kFhamAngularVariants; ++i)
+      if (kAngWaveChoices[i] == waves) v = i;
+    PP_REQUIRE(v >= 0, "angular wave count must be 16, 8 or 4");
+    (precision == 64 ? f->f.ang_variant64 : f->f.ang_variant32) = v;
+  });
+}
+
+extern "C" int pp_fham_get_angular_waves(const pp_fham* f, int precision, int* waves) {
+  return guarded([&] {
+    PP_REQUIRE(f && waves && (precision == 64 || precision == 32), "invalid argument");
+    *waves = kAngWaveChoices[precision == 64 ? f->f.ang_variant64 : f->f.ang_variant32];
+  });
+}
+
 // Times every radial variant on a zero input (one warm-up and `trials` timed multiplies each,
 // HIP events on `stream`) and keeps the fastest for that precision.
 extern "C" int pp_fham_tune(pp_fham* f, int precision, int trials, int* best, double* timings_ms, pp_stream stream) {
@@ -678,9 +708,30 @@ extern "C" int pp_fham_tune(pp_fham* f, int precision, int trials, int* best, do
         best_v = v;
       }
     }
+    var = best_v;
+    // then the angular stage's wave count, with the chosen radial shape
+    int& avar = precision == 64 ? F.ang_variant64 : F.ang_variant32;
+    double best_a = 1e300;
+    int best_av = avar;
+    for (int v = 0; v < kFhamAngularVariants; ++v) {
+      avar = v;
+      run();
+      PP_HIP(hipEventRecord(e0, s));
+      for (int k = 0; k < trials; ++k) run();
+      PP_HIP(hipEventRecord(e1, s));
+      PP_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      PP_HIP(hipEventElapsedTime(&ms, e0, e1));
+      const double per = ms / trials;
+      if (timings_ms) timings_ms[kFhamRadialVariants + v] = per;
+      if (per < best_a) {
+        best_a = per;
+        best_av = v;
+      }
+    }
+    avar = best_av;
     PP_HIP(hipEventDestroy(e0));
     PP_HIP(hipEventDestroy(e1));
-    var = best_v;
     if (best) *best = best_v;
   });
 }

@@ -11,6 +11,7 @@ namespace pp {
 
 constexpr int kMaxFTerms = 32;
 constexpr int kFhamRadialVariants = 7;  // radial-stage kernel shapes (terms x channel groups per wave, prefetch)
+constexpr int kFhamAngularVariants = 3;  // angular-stage wave counts (16, 8, 4 waves split K)
 
 struct FHamOut {
   double2* dst = nullptr;
@@ -31,6 +32,7 @@ struct FHam {
   // added with its own coefficient beta outside the angular contraction.
   bool has_overlap = false;
   int rad_variant64 = 0, rad_variant32 = 0;  // radial kernel shape per precision
+  int ang_variant64 = 0, ang_variant32 = 0;  // angular stage wave count per precision
   mutable unsigned long long gen64 = 0;      // FP64 multiplies issued
   mutable int w64_terms = 0;                 // radial terms the FP64 products W64 are known to hold
   mutable DeviceBuffer<int> reuse_flag;      // set by the input comparison, re-armed by the angular stage

@@ -129,12 +129,25 @@ class FactoredHamiltonian:
         return v.value
 
     def tune(self, precision: int, trials: int = 5):
-        """Time every radial variant on the device and keep the fastest; returns (best,
-        {shape: ms per multiply})."""
+        """Time every radial variant on the device and keep the fastest, then the angular
+        stage's wave count; returns (best radial variant, {shape: ms per multiply}) and leaves
+        the angular timings in self.angular_timings."""
         best = C.c_int()
-        tim = np.zeros(len(self.RADIAL_SHAPES))
+        tim = np.zeros(len(self.RADIAL_SHAPES) + len(self.ANGULAR_WAVES))
         N.check(N.load().pp_fham_tune(self.h, precision, trials, C.byref(best), D.hptr(tim), D.stream_handle()))
-        return best.value, dict(zip(self.RADIAL_SHAPES, tim.tolist()))
+        nr = len(self.RADIAL_SHAPES)
+        self.angular_timings = dict(zip(self.ANGULAR_WAVES, tim[nr:].tolist()))
+        return best.value, dict(zip(self.RADIAL_SHAPES, tim[:nr].tolist()))
+
+    ANGULAR_WAVES = (16, 8, 4)  # waves splitting K in the angular stage, per angular variant
+
+    def set_angular_waves(self, precision: int, waves: int):
+        N.check(N.load().pp_fham_set_angular_waves(self.h, precision, waves))
+
+    def angular_waves(self, precision: int) -> int:
+        w = C.c_int()
+        N.check(N.load().pp_fham_get_angular_waves(self.h, precision, C.byref(w)))
+        return w.value
 
     def __del__(self):
         if getattr(self, "h", None) is not None and N._lib is not None:

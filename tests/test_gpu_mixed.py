@@ -502,9 +502,24 @@ def test_factored_radial_variants_vs_dense(torch, rng):
         e32 = rel(d32.cpu().numpy().ravel(), ref)
         say("  shape %s: FP64 %.2e  FP32 %.2e" % (shape, e64, e32))
         assert e64 < 1e-12 and e32 < 2e-6
+    fh.set_radial_variant(64, 3)
+    fh.set_radial_variant(32, 2)
+    for waves in fh.ANGULAR_WAVES:
+        fh.set_angular_waves(64, waves)
+        fh.set_angular_waves(32, waves)
+        assert fh.angular_waves(64) == waves and fh.angular_waves(32) == waves
+        d = torch.empty_like(src)
+        fh.Multiply(src, d, t)
+        d32 = torch.empty_like(s32)
+        fh.Multiply(s32, d32, t)
+        e64 = rel(d.cpu().numpy().ravel(), ref)
+        e32 = rel(d32.cpu().numpy().ravel(), ref)
+        say("  angular %d waves: FP64 %.2e  FP32 %.2e" % (waves, e64, e32))
+        assert e64 < 1e-12 and e32 < 2e-6
     best, tim = fh.tune(64, trials=2)
     assert 0 <= best < len(fh.RADIAL_SHAPES) and fh.radial_variant(64) == best
-    assert all(ms > 0 for ms in tim.values())
+    assert all(ms > 0 for ms in tim.values()) and all(ms > 0 for ms in fh.angular_timings.values())
+    assert fh.angular_waves(64) in fh.ANGULAR_WAVES
 
 
 def test_factored_multiply_full_size_vs_tensor_potentials(torch, rng):
