@@ -183,48 +183,78 @@ def bench_batch(torch, steps=1000, per_graph=10, natoms=64, seed=2024, dt=0.01):
     return out
 
 
-def cpu_baseline(sysm, n_ops_per_step, sample_frac=0.1):
+def host_threads():
+    """Host threads for the all-cores baseline: the CPU share given to this process (the
+    OMP_NUM_THREADS convention when set, else the affinity mask)."""
+    import os
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(sysm, n_ops_per_step, sample_frac=0.1, threads=1):
     """The oracle (sequential C++ restatement of the reference loops) timed on the host:
-    the tensor-potential multiply on the first `sample_frac` of every potential's angular
-    pairs and the block-preconditioner solve of two channels, scaled to one Cayley step
-    (n_ops operator applications, each one multiply + one preconditioner solve)."""
+    the tensor-potential multiply on `sample_frac` of every potential's angular pairs and
+    the block-preconditioner solve of two channels, scaled to one Cayley step (n_ops
+    operator applications, each one multiply + one preconditioner solve).  With threads > 1
+    every thread runs the same kind of sample on its own slice of the angular pairs and its
+    own two channels at the same time (the calls release the GIL), the reference's
+    one-process-per-core model: the step time is the wall time per sample unit divided by
+    the number of threads."""
+    import threading
     import numpy as np
     import oracle
+    from pyprop_amd.systems import _expand
     rng = np.random.default_rng(3)
     nc, n1, n2 = sysm.shape
     psi = rng.standard_normal(sysm.shape) + 1j * rng.standard_normal(sysm.shape)
-    from pyprop_amd.systems import _expand
+
+    def run_parallel(fns):
+        ths = [threading.Thread(target=f) for f in fns]
+        t0 = time.perf_counter()
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        return time.perf_counter() - t0
+
+    # multiply: thread i takes pairs [i k, (i + 1) k) of every potential (wrapping around)
     t_mul = 0.0
     for p in sysm.potentials:
         P0 = p.geometries[0].pair_count
         k = max(1, int(P0 * sample_frac))
-        V = _expand(p.angular_coeffs[:, :k], p.radial_factors)
-        pairs0 = p.geometries[0].pairs[:k]
-        dst = np.zeros(sysm.shape, complex)
-        t0 = time.perf_counter()
-        oracle.tpmul(p.storage_ids, V, 1.0, psi, dst, [pairs0, p.geometries[1].pairs, None])
-        t_mul += (time.perf_counter() - t0) * P0 / k
-    # preconditioner: two channels, factor (not timed) then solve (timed)
-    from pyprop_amd.propagator import band_upper
+        jobs = []
+        for i in range(threads):
+            sel = (np.arange(k) + i * k) % P0
+            V = _expand(p.angular_coeffs[:, sel], p.radial_factors)
+            pairs0 = np.ascontiguousarray(p.geometries[0].pairs[sel])
+            dst = np.zeros(sysm.shape, complex)
+            jobs.append(lambda V=V, pairs0=pairs0, dst=dst, p=p: oracle.tpmul(
+                p.storage_ids, V, 1.0, psi, dst, [pairs0, p.geometries[1].pairs, None]))
+        wall = run_parallel(jobs)
+        t_mul += wall * P0 / (k * threads)
+    # preconditioner: each thread factors (not timed) and solves (timed) two channels
     b1 = sysm.bases[1]
     S1 = b1.overlap_band()
     g = sysm.precond_potentials[0].geometries
-    from pyprop_amd.systems import _expand
-    # the angular-diagonal pairs are (c, c) in channel order: the first two are channels 0, 1
     P = sum(_expand(x.angular_coeffs[:, :2], x.radial_factors) for x in sysm.precond_potentials)
-    s = oracle.System((2, n1, n2))
-    s.set_preconditioner(S1, S1, b1.order - 1, P, g[1].pairs, g[2].pairs, 0.005j)
-    x = psi[:2].copy()
-    t0 = time.perf_counter()
+    systems, xs = [], []
+    for i in range(threads):
+        s = oracle.System((2, n1, n2))
+        s.set_preconditioner(S1, S1, b1.order - 1, P, g[1].pairs, g[2].pairs, 0.005j)
+        systems.append(s)
+        xs.append(psi[:2].copy())
     reps = 3
-    for _ in range(reps):
-        s.precond_solve(x)
-    t_pc = (time.perf_counter() - t0) / reps * nc / 2
+    wall = run_parallel([lambda s=s, x=x: [s.precond_solve(x) for _ in range(reps)] for s, x in zip(systems, xs)])
+    t_pc = wall / reps * nc / (2 * threads)
     step = (n_ops_per_step + 1) * (t_mul + t_pc)
-    return dict(value=1.0 / step, unit="steps/s", cores=1, kind="port",
-                sample="oracle multiply on %d%% of each potential's angular pairs (t_full_matvec=%.3fs) and "
-                       "2-channel preconditioner solve (t_full=%.3fs), scaled to one step of %.1f operator "
-                       "applications + right-hand side" % (int(sample_frac * 100), t_mul, t_pc, n_ops_per_step))
+    return dict(value=1.0 / step, unit="steps/s", cores=threads, kind="port",
+                sample="oracle multiply on %d%% of each potential's angular pairs per thread (full multiply %.3fs) "
+                       "and 2-channel preconditioner solves per thread (full solve %.3fs), %d thread(s) at once, "
+                       "scaled to one step of %.1f operator applications + right-hand side"
+                       % (int(sample_frac * 100), t_mul, t_pc, threads, n_ops_per_step))
 
 
 def run_steps(torch, prop, psi, t0, dt, warmup, steps, stats=None):
@@ -444,10 +474,15 @@ def main():
     batch = None if a.no_batch else bench_batch(torch)
 
     cpu = None
+    cpu1 = None
     if not a.no_cpu_baseline:
         t0 = time.time()
-        cpu = cpu_baseline(sysm, ops_plain)
-        log("cpu baseline %.4f steps/s (%.1fs)" % (cpu["value"], time.time() - t0))
+        cpu1 = cpu_baseline(sysm, ops_plain)
+        log("cpu baseline, 1 thread: %.4f steps/s (%.1fs)" % (cpu1["value"], time.time() - t0))
+        nt = host_threads()
+        t0 = time.time()
+        cpu = cpu_baseline(sysm, ops_plain, threads=nt) if nt > 1 else cpu1
+        log("cpu baseline, %d threads: %.4f steps/s (%.1fs)" % (nt, cpu["value"], time.time() - t0))
 
     out = {
         "metric": "CN steps/s, helium rank-3: learned+FP32 vs plain FP64 GPU at 1e-8 err; GB/s",
@@ -482,7 +517,8 @@ def main():
                                 "epochs": a.train_epochs, "gen_s": t_gen, "fit_s": t_fit,
                                 "loss_first": learned.loss_history[0], "loss_last": learned.loss_history[-1]},
                    "energy": e_plain, "norm_drift_plain": abs(n_plain - norm0),
-                   "batched_h_atoms": batch},
+                   "batched_h_atoms": batch,
+                   "cpu_baseline_single_thread": cpu1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
                      "traffic_source": "profiles/tp_dia_kernel_helium_fp64_matvec.md (rocprofv3 FETCH_SIZE + "
