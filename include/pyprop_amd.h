@@ -381,6 +381,56 @@ int pp_fham_multiply_overlap_f32(const pp_fham* f, const double* scalings, const
 int pp_cayley_set_factored(pp_cayley* c, const pp_fham* f);
 
 /* ------------------------------------------------------------------------------
+ * Initial-state eigensolvers.
+ *
+ * pIRAM: implicitly restarted Arnoldi (core/krylov/piram/piram/piram.h:28-690, wrapped by
+ * pyprop/solver/PiramSolver.py:3-141).  basis_size = krylov_basis_size (>= 2 x
+ * eigenvalue_count, piram.h:269-272), eigenvalue_count = krylov_eigenvalue_count,
+ * max_restarts = krylov_max_iteration_count.  The wanted Ritz values are those with the
+ * smallest real part (CompareComplexLessReal, functors.h:95-102), shifts are the others.
+ * Convergence: |y_m| |f| <= tolerance max(eps^(2/3), |theta|) (ARPACK's test; see
+ * eigen.hip for the reference's variant).  start = NULL gives the reference's uniform
+ * [0,1) real start (fixed seed).  op is any pp_operator_fn: pp_ham_op_apply (the
+ * Hamiltonian), pp_shift_invert_apply (inverse iterations) or a caller's own.
+ * ------------------------------------------------------------------------------ */
+typedef struct pp_piram pp_piram;
+int pp_piram_create(int64_t n, int basis_size, int eigenvalue_count, double tolerance, int max_restarts,
+                    int max_orthogonalization, pp_piram** out);
+int pp_piram_destroy(pp_piram* p);
+int pp_piram_solve(pp_piram* p, pp_operator_fn op, void* ctx, const pp_cplx* start, pp_stream stream);
+/* GetEigenvalues / GetErrorEstimates / GetConvergenceEstimates (PiramSolver.py:116-121,
+ * piram.h:182-199): eigenvalue_count values, ascending real part */
+int pp_piram_eigenvalues(const pp_piram* p, pp_cplx* values, double* error_estimates,
+                         double* convergence_estimates, int* converged_count);
+/* GetEigenvector (PiramSolver.py:123-134): unit 2-norm Ritz vector into out (device) */
+int pp_piram_eigenvector(pp_piram* p, int index, pp_cplx* out, pp_stream stream);
+int pp_piram_stats(const pp_piram* p, int* operator_count, int* restart_count, int* orthogonalization_count);
+
+/* GMRESShiftInvertSolver (pyprop/solver/GMRESShiftInvertSolver.py:1-132): dst = (H - shift S)^-1 S src
+ * by GMRES(basis_size, tolerance) with left preconditioner pc (may be NULL), which the caller
+ * sets up with Hamiltonian scaling 1 and overlap scaling -shift (as __init__ does, 44-46).
+ * pp_shift_invert_apply has the pp_operator_fn signature (ctx = the pp_shift_invert*), the
+ * InverseIterations matrix_vector_func that PiramSolver takes (PiramSolver.py:62-70). */
+typedef struct pp_shift_invert pp_shift_invert;
+int pp_shift_invert_create(pp_hamiltonian* ham, const pp_precond* pc, int basis_size, double tolerance,
+                           int double_orthogonalization, double shift, pp_shift_invert** out);
+int pp_shift_invert_destroy(pp_shift_invert* si);
+int pp_shift_invert_set_scalings(pp_shift_invert* si, const double* scalings);
+int pp_shift_invert_set_factored(pp_shift_invert* si, const pp_fham* f);
+int pp_shift_invert_solve(pp_shift_invert* si, const pp_cplx* src, pp_cplx* dst, pp_stream stream);
+int pp_shift_invert_apply(void* ctx, const pp_cplx* in, pp_cplx* out, pp_stream stream);
+int pp_shift_invert_stats(const pp_shift_invert* si, int* solves, int* operator_count, double* last_error);
+
+/* The Hamiltonian as a pp_operator_fn (ctx = the pp_ham_op*): mode 0 = H
+ * (MultiplyHamiltonianNoOverlap), 1 = S^-1 H (MultiplyHamiltonian), 2 = U^-T H U^-1
+ * (MultiplyHamiltonianBalancedOverlap), BasisPropagator.py:116-159; scalings fixed at
+ * creation (NULL: all 1). */
+typedef struct pp_ham_op pp_ham_op;
+int pp_ham_op_create(pp_hamiltonian* ham, const double* scalings, int mode, pp_ham_op** out);
+int pp_ham_op_destroy(pp_ham_op* h);
+int pp_ham_op_apply(void* ctx, const pp_cplx* in, pp_cplx* out, pp_stream stream);
+
+/* ------------------------------------------------------------------------------
  * Mixed-precision Cayley step.  No reference counterpart; solves the system of
  * pp_cayley_step (CayleyPropagator.py:71-150) by FP64 iterative refinement around a
  * complex64 flexible GMRES with the Kronecker preconditioner (plus the learned

@@ -110,6 +110,22 @@ SIGNATURES = [
     ("pp_fham_get_angular_waves", I, [P, I, P]),
     ("pp_fham_tune", I, [P, I, I, P, P, P]),
     ("pp_fham_set_overlap", I, [P, P, P]),
+    ("pp_piram_create", I, [I64, I, I, D, I, I, P]),
+    ("pp_piram_destroy", I, [P]),
+    ("pp_piram_solve", I, [P, P, P, P, P]),
+    ("pp_piram_eigenvalues", I, [P, P, P, P, P]),
+    ("pp_piram_eigenvector", I, [P, I, P, P]),
+    ("pp_piram_stats", I, [P, P, P, P]),
+    ("pp_shift_invert_create", I, [P, P, I, D, I, D, P]),
+    ("pp_shift_invert_destroy", I, [P]),
+    ("pp_shift_invert_set_scalings", I, [P, P]),
+    ("pp_shift_invert_set_factored", I, [P, P]),
+    ("pp_shift_invert_solve", I, [P, P, P, P]),
+    ("pp_shift_invert_apply", I, [P, P, P, P]),
+    ("pp_shift_invert_stats", I, [P, P, P, P]),
+    ("pp_ham_op_create", I, [P, P, I, P]),
+    ("pp_ham_op_destroy", I, [P]),
+    ("pp_ham_op_apply", I, [P, P, P, P]),
     ("pp_fham_multiply_overlap", I, [P, P, P, P, P, PPCplx, PPCplx, P]),
     ("pp_fham_multiply_overlap_f32", I, [P, P, P, P, P, PPCplx, PPCplx, P]),
     ("pp_batch_create", I, [I, P, P, I, P, P, P, P, P, I, I, I, P]),
