@@ -186,3 +186,20 @@ def ShiftInvertEnergies(theta, shift: float):
     """Eigenvalues E of (H, S) from the Ritz values theta of (H - shift S)^-1 S."""
     theta = np.asarray(theta)
     return shift + 1.0 / theta
+
+
+def ImaginaryTimePropagation(propagator, psi, timestep: float, steps: int, renormalize: bool = True, t0: float = 0.0):
+    """Ground state by propagation in imaginary time, as the reference's FindGroundstate
+    (examples/tensor/helium_stabilization/example.py:379-389) runs Problem.Advance with
+    imtime: each step propagates with dt = -i timestep (exp(-timestep H) for the
+    propagators here), then S-normalises psi (Problem.py:161-190).  The potentials are
+    evaluated at the real time t0.  Returns the energy <psi|H|psi> after each step."""
+    energies = []
+    dt = -1j * float(timestep)
+    base = propagator.BasePropagator
+    for _ in range(steps):
+        propagator.AdvanceStep(t0, dt)
+        if renormalize:
+            psi.Normalize()
+        energies.append(base.GetEnergyExpectationValue(psi, t0))
+    return np.array(energies)
