@@ -42,8 +42,12 @@ static int grid_n(long long n) {
 // ------------------------------------------------------------------------------------
 // FP32 vector kernels (FP64 accumulation in reductions)
 // ------------------------------------------------------------------------------------
-// y = s x in FP32 (the scale applied in FP64)
-__global__ void f64_to_f32_scaled_kernel(long long n, double sc, const double2* __restrict__ x, float2* __restrict__ y) {
+// y = x / sqrt(*n2) in FP32 (0 where *n2 is 0): the squared norm is read on the device,
+// from the finishing kernel of the reduction that formed it
+__global__ void f64_to_f32_unit_kernel(long long n, const double2* __restrict__ n2, const double2* __restrict__ x,
+                                       float2* __restrict__ y) {
+  const double q = n2->x;
+  const double sc = q > 0 ? 1.0 / sqrt(q) : 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     y[i] = make_float2((float)(sc * x[i].x), (float)(sc * x[i].y));
 }
@@ -86,14 +90,21 @@ __device__ void sum_partials(const double2* __restrict__ partial, int blocks, in
   }
 }
 
-// out = [sum of p1 (c1 columns), sum of p2 (c2 columns), sum of p3 (c3 columns)], one block
+// out = [sum of p1 (c1 columns), sum of p2 (c2 columns), sum of p3 (c3 columns)], one block;
+// out2 (device memory, may be NULL) receives the same values, for kernels that follow
 __global__ __launch_bounds__(256) void finish_partials_kernel(int blocks, const double2* __restrict__ p1, int w1,
                                                               int c1, const double2* __restrict__ p2, int w2, int c2,
                                                               const double2* __restrict__ p3, int w3, int c3,
-                                                              double2* __restrict__ out) {
-  if (c1) sum_partials(p1, blocks, w1, c1, out);
-  if (c2) sum_partials(p2, blocks, w2, c2, out + c1);
-  if (c3) sum_partials(p3, blocks, w3, c3, out + c1 + c2);
+                                                              double2* __restrict__ out, double2* __restrict__ out2) {
+  __shared__ double2 res[128];
+  if (c1) sum_partials(p1, blocks, w1, c1, res);
+  if (c2) sum_partials(p2, blocks, w2, c2, res + c1);
+  if (c3) sum_partials(p3, blocks, w3, c3, res + c1 + c2);
+  __syncthreads();
+  for (int j = threadIdx.x; j < c1 + c2 + c3; j += blockDim.x) {
+    out[j] = res[j];
+    if (out2) out2[j] = res[j];
+  }
 }
 
 // One pass of classical Gram-Schmidt over FP32 vectors with FP64 sums; block partials go to
@@ -658,6 +669,7 @@ void MixedWork::ensure(long long n_, int m_) {
   x64.resize(n);
   t64.resize(n);
   h64.resize(n);
+  norms.resize(4);
 }
 
 void MixedWork::read_small(int count, hipStream_t s) {
@@ -670,12 +682,12 @@ MixedWork::~MixedWork() {
   if (small) (void)hipHostFree(small);
 }
 
-void MixedWork::norm2_pair(const double2* a, const double2* b, double2* out, hipStream_t s) {
+void MixedWork::norm2_pair(const double2* a, const double2* b, double2* out, double2* out_dev, hipStream_t s) {
   const int blocks = fused_blocks(n);
   red.ensure(blocks, 2);
   hipLaunchKernelGGL(norm2_pair_partial_kernel, dim3(blocks), dim3(256), 0, s, n, a, b, red.partial.ptr);
   hipLaunchKernelGGL(finish_partials_kernel, dim3(1), dim3(256), 0, s, blocks, red.partial.ptr, 2, 2, nullptr, 0, 0,
-                     nullptr, 0, 0, out);
+                     nullptr, 0, 0, out, out_dev);
   PP_HIP(hipGetLastError());
 }
 
@@ -694,7 +706,7 @@ void MixedWork::cgs2_32(int k, float2* w_, double2* out, hipStream_t s) {
   hipLaunchKernelGGL((cgs32_kernel<true, true>), g, b, 0, s, n, k, V.ptr, n, p1, width, w_, p2);
   hipLaunchKernelGGL((cgs32_kernel<true, false>), g, b, 0, s, n, k, V.ptr, n, p2, width, w_, p3);
   hipLaunchKernelGGL(finish_partials_kernel, dim3(1), b, 0, s, blocks, p1, width, width, p2, width, width, p3, 1, 1,
-                     out);
+                     out, (double2*)nullptr);
   PP_HIP(hipGetLastError());
 }
 
@@ -705,23 +717,23 @@ void MixedWork::norm2_32(const float2* w_, double2* out, hipStream_t s) {
   hipLaunchKernelGGL((cgs32_kernel<false, false>), dim3(blocks), dim3(256), 0, s, n, 0, V.ptr, n, nullptr, 0,
                      const_cast<float2*>(w_), red.partial.ptr);
   hipLaunchKernelGGL(finish_partials_kernel, dim3(1), dim3(256), 0, s, blocks, red.partial.ptr, 1, 1, nullptr, 0, 0,
-                     nullptr, 0, 0, out);
+                     nullptr, 0, 0, out, (double2*)nullptr);
   PP_HIP(hipGetLastError());
 }
 
-void MixedWork::residual(const double2* b64_, double2* r, double2* out, hipStream_t s) {
+void MixedWork::residual(const double2* b64_, double2* r, double2* out, double2* out_dev, hipStream_t s) {
   const int blocks = fused_blocks(n);
   red.ensure(blocks, 1);
   hipLaunchKernelGGL(residual_partial_kernel, dim3(blocks), dim3(256), 0, s, n, b64_, r, red.partial.ptr);
   hipLaunchKernelGGL(finish_partials_kernel, dim3(1), dim3(256), 0, s, blocks, red.partial.ptr, 1, 1, nullptr, 0, 0,
-                     nullptr, 0, 0, out);
+                     nullptr, 0, 0, out, out_dev);
   PP_HIP(hipGetLastError());
 }
 
 int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
                         const std::function<void(int, float2*, float, float2*)>& apply_P, const float2* b,
                         double rtol, int max_it, double* rel_out, hipStream_t s, double beta_in, const double2* base,
-                        double2* xout) {
+                        double2* xout, const std::function<bool(double&)>& first_read) {
   using cd = std::complex<double>;
   const int mm = std::min(max_it, m);
   PP_REQUIRE(mm >= 1 && mm + 3 <= 120, "inner basis size out of range");
@@ -731,11 +743,12 @@ int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply
     read_small(1, s);
     beta = std::sqrt(host_small[0].x);
   }
-  if (!(beta > 0)) {
+  auto no_solve = [&]() {
     if (base != xout) PP_HIP(hipMemcpyAsync(xout, base, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
     if (rel_out) *rel_out = 0;
     return 0;
-  }
+  };
+  if (!first_read && !(beta > 0)) return no_solve();
   if (b) hipLaunchKernelGGL(scale32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, (float)(1.0 / beta), b, V.ptr);
   std::vector<cd> H((size_t)(mm + 1) * mm, 0.0);  // column-major (mm+1) x mm
   auto Hc = [&](int row, int col) -> cd& { return H[(size_t)col * (mm + 1) + row]; };
@@ -753,6 +766,14 @@ int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply
     const int w1 = j + 2;  // h (j+1) + |w|^2
     cgs2_32(j + 1, vn, small_dev, s);  // [first dots, |w|^2 | second dots, |w|^2 | final |w|^2]
     read_small(2 * w1 + 1, s);
+    if (j == 0 && first_read) {
+      // the caller learns |rhs| (and may decide the solve is not needed) only now
+      if (!first_read(beta)) {
+        if (rel_out) *rel_out = 0;
+        return -1;
+      }
+      if (!(beta > 0)) return no_solve();
+    }
     for (int i = 0; i <= j; ++i)
       Hc(i, j) = cd(host_small[i].x + host_small[w1 + i].x, host_small[i].y + host_small[w1 + i].y);
     const double hn = std::sqrt(std::max(0.0, host_small[2 * w1].x));
@@ -822,10 +843,10 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   }
   z_valid = false;
   ++stats.outer_ops;
-  work.norm2_pair(work.b64.ptr, work.r64.ptr, work.small_dev, s);
-  work.read_small(2, s);
-  const double bnorm = std::sqrt(work.host_small[0].x);
-  double rn = std::sqrt(work.host_small[1].x);
+  // |b|^2 and |r|^2: written to the pinned block for the host and to the device for the
+  // scaling of V_0; the host reads them at the first synchronisation of the first correction
+  constexpr int kB = MixedWork::kSmall - 4, kR0 = MixedWork::kSmall - 3, kR = MixedWork::kSmall - 2;
+  work.norm2_pair(work.b64.ptr, work.r64.ptr, work.small_dev + kB, work.norms.ptr, s);
 
   auto apply_A32 = [&](const float2* in, float2* out) {
     // out = S in + i a H in.  S stays in its factored form U^T U here even when the factored
@@ -853,23 +874,52 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   };
   bool converged = false;
   bool x_is_psi = true;  // x has not been formed yet: it equals psi
-  for (int outer = 0;; ++outer) {
+  double bnorm = -1.0, rn = -1.0;
+  // a residual norm written by a kernel and not yet read: its index in the pinned block and
+  // its device copy
+  int pending = kR0;
+  const double2* pending_dev = work.norms.ptr + 1;
+  double predicted = HUGE_VAL;  // relative residual the pending value is expected to show
+  auto read_pending = [&]() {   // the stream has been synchronised
+    if (bnorm < 0) bnorm = std::sqrt(std::max(0.0, work.small[kB].x));
+    rn = std::sqrt(std::max(0.0, work.small[pending].x));
+    pending = -1;
     stats.residual = rn;
     stats.rel_residual = bnorm > 0 ? rn / bnorm : 0.0;
-    if (rn <= tol_rel * bnorm) {
-      converged = true;
-      break;
+    return rn <= tol_rel * bnorm;
+  };
+  for (int outer = 0;; ++outer) {
+    const bool limit = outer >= max_outer || stats.inner_iterations > fallback_iterations;
+    // Read a pending residual now when it is likely converged or a limit is reached; else
+    // the next correction starts at once and the residual is read at that correction's
+    // first synchronisation (a needless correction is then abandoned after one iteration).
+    if (pending >= 0 && (predicted <= 10.0 * tol_rel || limit)) {
+      PP_HIP(hipStreamSynchronize(s));
+      if (read_pending()) {
+        converged = true;
+        break;
+      }
     }
-    if (outer >= max_outer || stats.inner_iterations > fallback_iterations) break;
+    if (limit) break;
     ++stats.outer_iterations;
-    // V_0 = r / |r| in FP32, with |r| the FP64 norm already on the host (the FP32 rounding
-    // changes it by ~1e-8 relative, far below the inner tolerance)
-    hipLaunchKernelGGL(f64_to_f32_scaled_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, 1.0 / rn, work.r64.ptr,
+    // V_0 = r / |r| in FP32, the norm read on the device
+    hipLaunchKernelGGL(f64_to_f32_unit_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, pending_dev, work.r64.ptr,
                        work.V.ptr);
     double rel = 0;
     // x = (psi or x) + d, d from the inner solve
-    stats.inner_iterations += work.fgmres32(apply_A32, apply_P32, nullptr, inner_rtol, m, &rel, s, rn,
-                                            x_is_psi ? psi : work.x64.ptr, work.x64.ptr);
+    const int its = work.fgmres32(
+        apply_A32, apply_P32, nullptr, inner_rtol, m, &rel, s, 0.0, x_is_psi ? psi : work.x64.ptr, work.x64.ptr,
+        [&](double& beta) {
+          if (pending >= 0 && read_pending()) return false;  // already converged: no correction
+          beta = rn;
+          return true;
+        });
+    if (its < 0) {
+      --stats.outer_iterations;
+      converged = true;
+      break;
+    }
+    stats.inner_iterations += its;
     x_is_psi = false;
     ++correction;
     // r = b - (S x + i a H x)
@@ -882,9 +932,10 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
       else H.apply(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
     }
     ++stats.outer_ops;
-    work.residual(work.b64.ptr, work.r64.ptr, work.small_dev, s);
-    work.read_small(1, s);
-    rn = std::sqrt(work.host_small[0].x);
+    work.residual(work.b64.ptr, work.r64.ptr, work.small_dev + kR, work.norms.ptr + 2, s);
+    pending = kR;
+    pending_dev = work.norms.ptr + 2;
+    predicted = (bnorm > 0 ? rn / bnorm : 0.0) * rel;
   }
   if (converged && fused_s && !x_is_psi) {
     // W64 now holds the radial products (overlap term included) of x64, which becomes psi below

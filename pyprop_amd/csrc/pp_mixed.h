@@ -62,23 +62,28 @@ struct MixedWork {
   ~MixedWork();
   void ensure(long long n, int m);
   void read_small(int count, hipStream_t s);  // synchronise s, copy small[0..count) to host_small
-  void norm2_pair(const double2* a, const double2* b, double2* out, hipStream_t s);  // |a|^2, |b|^2
+  DeviceBuffer<double2> norms;   // device copies of step-level squared norms (|b|^2, |r0|^2, |r|^2)
+  // out[0] = |a|^2, out[1] = |b|^2 (pinned), the same into out_dev (device)
+  void norm2_pair(const double2* a, const double2* b, double2* out, double2* out_dev, hipStream_t s);
   // Two-pass classical Gram-Schmidt of w against V_0..V_{k-1} (FP64 sums): out[0..k] = first
   // dots and |w|^2, out[k+1..2k+1] = second dots and |w|^2, out[2k+2] = final |w|^2
   void cgs2_32(int k, float2* w, double2* out, hipStream_t s);
   void norm2_32(const float2* w, double2* out, hipStream_t s);  // out[0] = |w|^2
-  // r = b - r, out[0] = |r|^2 (FP64)
-  void residual(const double2* b, double2* r, double2* out, hipStream_t s);
+  // r = b - r, out[0] = |r|^2 (FP64; pinned out and device out_dev)
+  void residual(const double2* b, double2* r, double2* out, double2* out_dev, hipStream_t s);
   // out = S x S in (FP32, out-of-place; scratch has n elements)
   void overlap_multiply32(const Repr& r, const float2* in, float2* out, float2* scratch, hipStream_t s);
   std::vector<DeviceBuffer<float>> u32 = std::vector<DeviceBuffer<float>>(kMaxRank);
   // Flexible GMRES(m) on A d = rhs, d0 = 0, right preconditioner P_j (j = iteration), then
   // xout = base + d (FP64; base may alias xout).  rhs = NULL: V_0 already holds rhs / |rhs|
-  // and beta = |rhs|.  apply_P(j, v, scale, z): v <- scale v (normalising the basis vector),
-  // z = P_j v.
+  // and beta = |rhs|, or, with first_read, beta is set by first_read at the first host
+  // synchronisation (after iteration 0), which may also return false: the solve is then
+  // abandoned without touching xout and -1 is returned.  apply_P(j, v, scale, z):
+  // v <- scale v (normalising the basis vector), z = P_j v.
   int fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
                const std::function<void(int, float2*, float, float2*)>& apply_P, const float2* rhs, double rtol,
-               int max_it, double* rel_out, hipStream_t s, double beta, const double2* base, double2* xout);
+               int max_it, double* rel_out, hipStream_t s, double beta, const double2* base, double2* xout,
+               const std::function<bool(double&)>& first_read = nullptr);
 };
 
 struct MixedCayley {
