@@ -271,10 +271,14 @@ __global__ void global_norm_kernel(int nchan, double* __restrict__ nrm) {
   nrm[nchan] = std::sqrt(s);
 }
 
-// x[c] = bf16(v_c / |v|) with the global norm |v| = nrm[nchan] (zero vector -> 0)
-__global__ void block_pack_kernel(int nchan, int nb, const float2* __restrict__ v, const double* __restrict__ nrm,
+// x[c] = bf16(v_c / |v|) with the global norm |v| = sqrt(sum_c nrm[c]^2), summed by every
+// block in the same order; block 0 also stores it in nrm[nchan] (zero vector -> 0)
+__global__ void block_pack_kernel(int nchan, int nb, const float2* __restrict__ v, double* __restrict__ nrm,
                                   uint16_t* __restrict__ x) {
-  const double g = nrm[nchan];
+  double g2 = 0;
+  for (int c = 0; c < nchan; ++c) g2 += nrm[c] * nrm[c];
+  const double g = std::sqrt(g2);
+  if (blockIdx.x == 0 && threadIdx.x == 0) nrm[nchan] = g;
   const float inv = g > 0 ? (float)(1.0 / g) : 0.f;
   const long long total = (long long)nchan * nb;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
@@ -284,19 +288,6 @@ __global__ void block_pack_kernel(int nchan, int nb, const float2* __restrict__ 
   }
 }
 
-// out_c += sigma |v| y_c
-__global__ void learned_combine_kernel(int nchan, int nb, const float* __restrict__ y, const double* __restrict__ nrm,
-                                       float sigma, float2* __restrict__ out) {
-  const float sc = sigma * (float)nrm[nchan];
-  const long long total = (long long)nchan * nb;
-  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
-       idx += (long long)gridDim.x * blockDim.x) {
-    float2 o = out[idx];
-    o.x = fmaf(sc, y[2 * idx], o.x);
-    o.y = fmaf(sc, y[2 * idx + 1], o.y);
-    out[idx] = o;
-  }
-}
 
 // training rows: x_c = r_c / |r|, t_c = (kd_c - r_c) / |r| (FP32) with the global |r| = nrm[nchan];
 // nrm_out[c] = |r_c| / |r|
@@ -418,13 +409,10 @@ void learned_apply(Learned& L, const Kron& K, double2 a, const float2* in, float
   L.ensure(L.nchan);
   const long long n = (long long)L.nchan * nb;
   hipLaunchKernelGGL((block_norm_kernel<float2>), dim3(L.nchan), dim3(256), 0, s, nb, in, L.nrm.ptr);
-  hipLaunchKernelGGL(global_norm_kernel, dim3(1), dim3(64), 0, s, L.nchan, L.nrm.ptr);
   hipLaunchKernelGGL(block_pack_kernel, dim3(grid_for_n(n)), dim3(256), 0, s, L.nchan, nb, in, L.nrm.ptr, L.xh.ptr);
   L.forward(L.nchan, L.iota.ptr, s);
-  if (out != in) PP_HIP(hipMemcpyAsync(out, in, n * sizeof(float2), hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(learned_combine_kernel, dim3(grid_for_n(n)), dim3(256), 0, s, L.nchan, nb, L.y.ptr, L.nrm.ptr,
-                     L.sigma, out);
-  K.apply32(a, out, s);
+  // out = K^-1 (in + sigma |v| y): the combination is formed as the Kronecker solve loads its input
+  K.apply32_add(a, in, reinterpret_cast<const float2*>(L.y.ptr), L.nrm.ptr + L.nchan, L.sigma, out, s);
   PP_HIP(hipGetLastError());
 }
 

@@ -423,7 +423,8 @@ __device__ inline void kron_stage(const R* __restrict__ M, const C* in, C* out, 
 template <class R, class C, bool E_IN_LDS, bool FWD>
 __global__ __launch_bounds__(1024) void kron_kernel(const R* __restrict__ E, const R* __restrict__ lam,
                                                     const int* __restrict__ l1, const int* __restrict__ l2, int n,
-                                                    C a, const C* in, C* psi, R in_scale, C* in_wb) {
+                                                    C a, const C* in, C* psi, R in_scale, C* in_wb, const C* add,
+                                                    const double* add_norm, R add_sigma) {
   extern __shared__ unsigned char smem[];
   const int c = blockIdx.x;
   const int n2 = n * n;
@@ -445,10 +446,15 @@ __global__ __launch_bounds__(1024) void kron_kernel(const R* __restrict__ E, con
   const R* lam2 = lam + (long long)l2[c] * n;
   C* x = psi + (long long)c * n2;
   const C* xin = in + (long long)c * n2;  // may alias x: read completely before any write
+  const R as = add ? add_sigma * (R)*add_norm : R(0);
   for (int i = threadIdx.x; i < n2; i += blockDim.x) {
     C v = xin[i];
     v.x *= in_scale;
     v.y *= in_scale;
+    if (add) {  // in + sigma |v| add (the learned correction), |v| read on the device
+      v.x += as * add[(long long)c * n2 + i].x;
+      v.y += as * add[(long long)c * n2 + i].y;
+    }
     B[i] = v;
     if (in_wb) in_wb[(long long)c * n2 + i] = v;  // the scaled input, in place of in
   }
@@ -523,7 +529,8 @@ __device__ inline void kron_mfma_stage(const float* M, const float* in_re, const
 __global__ __launch_bounds__(1024) void kron_mfma_kernel(const float* __restrict__ E, const float* __restrict__ lam,
                                                          const int* __restrict__ l1, const int* __restrict__ l2,
                                                          int n, float2 a, const float2* in, float2* psi,
-                                                         float in_scale, float2* in_wb) {
+                                                         float in_scale, float2* in_wb, const float2* __restrict__ add,
+                                                         const double* __restrict__ add_norm, float add_sigma) {
   extern __shared__ float kl[];
   float* Bre = kl;
   float* Bim = Bre + 64 * KLD;
@@ -542,7 +549,13 @@ __global__ __launch_bounds__(1024) void kron_mfma_kernel(const float* __restrict
   for (int idx = threadIdx.x; idx < 64 * 64; idx += blockDim.x) {
     const int i = idx >> 6, j = idx & 63;
     const bool ok = i < n && j < n;
-    const float2 v = ok ? make_float2(in_scale * xin[i * n + j].x, in_scale * xin[i * n + j].y) : make_float2(0.f, 0.f);
+    float2 v = ok ? make_float2(in_scale * xin[i * n + j].x, in_scale * xin[i * n + j].y) : make_float2(0.f, 0.f);
+    if (ok && add) {  // in + sigma |v| add (the learned correction), |v| read on the device
+      const float as = add_sigma * (float)*add_norm;
+      const float2 y = add[(long long)c * n2 + i * n + j];
+      v.x = fmaf(as, y.x, v.x);
+      v.y = fmaf(as, y.y, v.y);
+    }
     if (ok && in_wb) in_wb[(long long)c * n2 + i * n + j] = v;  // the scaled input, in place of in
     Bre[i * KLD + j] = v.x;
     Bim[i * KLD + j] = v.y;
@@ -585,28 +598,41 @@ void kron_solve(const Kron& k, double2 a, double2* psi, hipStream_t s) { k.apply
 
 void Kron::apply64(double2 a, double2* psi, hipStream_t s) const {
   hipLaunchKernelGGL((kron_kernel<double, double2, false, false>), dim3(nchan), dim3(1024), kron_lds64(n), s, E64.ptr,
-                     lam64.ptr, l1.ptr, l2.ptr, n, a, psi, psi, 1.0, (double2*)nullptr);
+                     lam64.ptr, l1.ptr, l2.ptr, n, a, psi, psi, 1.0, (double2*)nullptr, (const double2*)nullptr,
+                     (const double*)nullptr, 0.0);
   PP_HIP(hipGetLastError());
 }
 
 void Kron::multiply64(double2 a, double2* psi, hipStream_t s) const {
   hipLaunchKernelGGL((kron_kernel<double, double2, false, true>), dim3(nchan), dim3(1024), kron_lds64(n), s, F64.ptr,
-                     lam64.ptr, l1.ptr, l2.ptr, n, a, psi, psi, 1.0, (double2*)nullptr);
+                     lam64.ptr, l1.ptr, l2.ptr, n, a, psi, psi, 1.0, (double2*)nullptr, (const double2*)nullptr,
+                     (const double*)nullptr, 0.0);
   PP_HIP(hipGetLastError());
 }
 
 void Kron::apply32(double2 a, float2* psi, hipStream_t s) const { apply32(a, psi, psi, s); }
 
 void Kron::apply32(double2 a, float2* in, float2* out, hipStream_t s, float in_scale, bool write_back) const {
-  float2* wb = write_back ? in : nullptr;
+  launch32(a, in, out, s, in_scale, write_back ? in : nullptr, nullptr, nullptr, 0.f);
+}
+
+void Kron::apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
+                       float2* out, hipStream_t s) const {
+  launch32(a, in, out, s, 1.f, nullptr, add, add_norm, add_sigma);
+}
+
+void Kron::launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
+                    const float2* add, const double* add_norm, float add_sigma) const {
   if (n <= 64) {
     hipLaunchKernelGGL(kron_mfma_kernel, dim3(nchan), dim3(1024), kron_lds_mfma(), s, E32.ptr, lam32.ptr, l1.ptr,
-                       l2.ptr, n, make_float2((float)a.x, (float)a.y), in, out, in_scale, wb);
+                       l2.ptr, n, make_float2((float)a.x, (float)a.y), in, out, in_scale, wb, add, add_norm,
+                       add_sigma);
     PP_HIP(hipGetLastError());
     return;
   }
   hipLaunchKernelGGL((kron_kernel<float, float2, true, false>), dim3(nchan), dim3(1024), kron_lds32(n), s, E32.ptr,
-                     lam32.ptr, l1.ptr, l2.ptr, n, make_float2((float)a.x, (float)a.y), in, out, in_scale, wb);
+                     lam32.ptr, l1.ptr, l2.ptr, n, make_float2((float)a.x, (float)a.y), in, out, in_scale, wb, add,
+                     add_norm, add_sigma);
   PP_HIP(hipGetLastError());
 }
 

@@ -28,6 +28,11 @@ struct Kron {
   void apply32(double2 a, float2* psi, hipStream_t s) const;
   // out <- K^-1 (in_scale in) (may alias in); write_back: in <- in_scale in as well
   void apply32(double2 a, float2* in, float2* out, hipStream_t s, float in_scale = 1.f, bool write_back = false) const;
+  // out <- K^-1 (in + add_sigma * (*add_norm) * add), *add_norm read on the device
+  void apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
+                   float2* out, hipStream_t s) const;
+  void launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
+                const float2* add, const double* add_norm, float add_sigma) const;
   // psi <- K psi (FP64)
   void multiply64(double2 a, double2* psi, hipStream_t s) const;
 };
