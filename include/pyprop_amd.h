@@ -435,8 +435,10 @@ int pp_ham_op_apply(void* ctx, const pp_cplx* in, pp_cplx* out, pp_stream stream
  * pp_cayley_step (CayleyPropagator.py:71-150) by FP64 iterative refinement around a
  * complex64 flexible GMRES with the Kronecker preconditioner (plus the learned
  * correction on the first direction when attached):
- *   stop when |b - A x| <= tol_rel |b| (FP64 residual); inner solves stop at
- *   inner_rtol relative to their right-hand side.
+ *   stop when |b - A x| <= tol_rel |b| (FP64 residual); inner solves (basis_size
+ *   1..60) stop at inner_rtol relative to their right-hand side.  Each correction runs
+ *   as many inner iterations as it needed in the previous step before the host reads
+ *   any result (more columns than needed only lower the residual further).
  * With a fallback solver attached, a step whose inner iterations exceed
  * fallback_iterations (or that does not converge in max_outer corrections) is redone
  * by the plain solver from the unchanged psi.

@@ -424,7 +424,11 @@ template <class R, class C, bool E_IN_LDS, bool FWD>
 __global__ __launch_bounds__(1024) void kron_kernel(const R* __restrict__ E, const R* __restrict__ lam,
                                                     const int* __restrict__ l1, const int* __restrict__ l2, int n,
                                                     C a, const C* in, C* psi, R in_scale, C* in_wb, const C* add,
-                                                    const double* add_norm, R add_sigma) {
+                                                    const double* add_norm, R add_sigma, const double2* in_n2) {
+  if (in_n2) {  // scale = 1 / |in| from a squared norm on the device (0 for a zero vector)
+    const double q = in_n2->x;
+    in_scale = q > 0 ? (R)(1.0 / sqrt(q)) : R(0);
+  }
   extern __shared__ unsigned char smem[];
   const int c = blockIdx.x;
   const int n2 = n * n;
@@ -530,7 +534,12 @@ __global__ __launch_bounds__(1024) void kron_mfma_kernel(const float* __restrict
                                                          const int* __restrict__ l1, const int* __restrict__ l2,
                                                          int n, float2 a, const float2* in, float2* psi,
                                                          float in_scale, float2* in_wb, const float2* __restrict__ add,
-                                                         const double* __restrict__ add_norm, float add_sigma) {
+                                                         const double* __restrict__ add_norm, float add_sigma,
+                                                         const double2* __restrict__ in_n2) {
+  if (in_n2) {  // scale = 1 / |in| from a squared norm on the device (0 for a zero vector)
+    const double q = in_n2->x;
+    in_scale = q > 0 ? (float)(1.0 / sqrt(q)) : 0.f;
+  }
   extern __shared__ float kl[];
   float* Bre = kl;
   float* Bim = Bre + 64 * KLD;
@@ -599,40 +608,41 @@ void kron_solve(const Kron& k, double2 a, double2* psi, hipStream_t s) { k.apply
 void Kron::apply64(double2 a, double2* psi, hipStream_t s) const {
   hipLaunchKernelGGL((kron_kernel<double, double2, false, false>), dim3(nchan), dim3(1024), kron_lds64(n), s, E64.ptr,
                      lam64.ptr, l1.ptr, l2.ptr, n, a, psi, psi, 1.0, (double2*)nullptr, (const double2*)nullptr,
-                     (const double*)nullptr, 0.0);
+                     (const double*)nullptr, 0.0, (const double2*)nullptr);
   PP_HIP(hipGetLastError());
 }
 
 void Kron::multiply64(double2 a, double2* psi, hipStream_t s) const {
   hipLaunchKernelGGL((kron_kernel<double, double2, false, true>), dim3(nchan), dim3(1024), kron_lds64(n), s, F64.ptr,
                      lam64.ptr, l1.ptr, l2.ptr, n, a, psi, psi, 1.0, (double2*)nullptr, (const double2*)nullptr,
-                     (const double*)nullptr, 0.0);
+                     (const double*)nullptr, 0.0, (const double2*)nullptr);
   PP_HIP(hipGetLastError());
 }
 
 void Kron::apply32(double2 a, float2* psi, hipStream_t s) const { apply32(a, psi, psi, s); }
 
-void Kron::apply32(double2 a, float2* in, float2* out, hipStream_t s, float in_scale, bool write_back) const {
-  launch32(a, in, out, s, in_scale, write_back ? in : nullptr, nullptr, nullptr, 0.f);
+void Kron::apply32(double2 a, float2* in, float2* out, hipStream_t s, float in_scale, bool write_back,
+                   const double2* in_n2) const {
+  launch32(a, in, out, s, in_scale, write_back ? in : nullptr, nullptr, nullptr, 0.f, in_n2);
 }
 
 void Kron::apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
                        float2* out, hipStream_t s) const {
-  launch32(a, in, out, s, 1.f, nullptr, add, add_norm, add_sigma);
+  launch32(a, in, out, s, 1.f, nullptr, add, add_norm, add_sigma, nullptr);
 }
 
 void Kron::launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
-                    const float2* add, const double* add_norm, float add_sigma) const {
+                    const float2* add, const double* add_norm, float add_sigma, const double2* in_n2) const {
   if (n <= 64) {
     hipLaunchKernelGGL(kron_mfma_kernel, dim3(nchan), dim3(1024), kron_lds_mfma(), s, E32.ptr, lam32.ptr, l1.ptr,
                        l2.ptr, n, make_float2((float)a.x, (float)a.y), in, out, in_scale, wb, add, add_norm,
-                       add_sigma);
+                       add_sigma, in_n2);
     PP_HIP(hipGetLastError());
     return;
   }
   hipLaunchKernelGGL((kron_kernel<float, float2, true, false>), dim3(nchan), dim3(1024), kron_lds32(n), s, E32.ptr,
                      lam32.ptr, l1.ptr, l2.ptr, n, make_float2((float)a.x, (float)a.y), in, out, in_scale, wb, add,
-                     add_norm, add_sigma);
+                     add_norm, add_sigma, in_n2);
   PP_HIP(hipGetLastError());
 }
 
@@ -689,13 +699,15 @@ void MixedWork::ensure(long long n_, int m_) {
     PP_HIP(hipHostMalloc((void**)&small, kSmall * sizeof(double2), hipHostMallocMapped | hipHostMallocCoherent));
     PP_HIP(hipHostGetDevicePointer((void**)&small_dev, small, 0));
   }
-  PP_REQUIRE(2 * (m + 2) + 1 <= kSmall, "inner basis too large for the host result block");
+  // per inner iteration j: 2 (j + 2) + 1 results at stride 2 (m + 1) + 1, then 8 step-level slots
+  PP_REQUIRE((long long)m * (2 * (m + 1) + 1) <= kSmall - 8, "inner basis too large for the host result block");
   b64.resize(n);
   r64.resize(n);
   x64.resize(n);
   t64.resize(n);
   h64.resize(n);
   norms.resize(4);
+  hdev.resize(kSmall);
 }
 
 void MixedWork::read_small(int count, hipStream_t s) {
@@ -720,9 +732,11 @@ void MixedWork::norm2_pair(const double2* a, const double2* b, double2* out, dou
 // Classical Gram-Schmidt, two passes, against V_0..V_{k-1} in four launches:
 // out[0..k] = first dots (+|w|^2), out[k+1..2k+1] = second dots (+|w|^2), out[2k+2] = |w|^2
 // after both updates.
-void MixedWork::cgs2_32(int k, float2* w_, double2* out, hipStream_t s) {
+void MixedWork::cgs2_32(int k, float2* w_, double2* out, double2* out_dev, hipStream_t s) {
   const int blocks = fused_blocks(n);
   const int width = k + 1;
+  // the finishing kernel holds all 2 width + 1 results in a 128-entry LDS array
+  PP_REQUIRE(2 * width + 1 <= 128, "Gram-Schmidt basis too large");
   red.ensure(blocks, 2 * width + 1);
   double2* p1 = red.partial.ptr;
   double2* p2 = p1 + (size_t)blocks * width;
@@ -732,7 +746,7 @@ void MixedWork::cgs2_32(int k, float2* w_, double2* out, hipStream_t s) {
   hipLaunchKernelGGL((cgs32_kernel<true, true>), g, b, 0, s, n, k, V.ptr, n, p1, width, w_, p2);
   hipLaunchKernelGGL((cgs32_kernel<true, false>), g, b, 0, s, n, k, V.ptr, n, p2, width, w_, p3);
   hipLaunchKernelGGL(finish_partials_kernel, dim3(1), b, 0, s, blocks, p1, width, width, p2, width, width, p3, 1, 1,
-                     out, (double2*)nullptr);
+                     out, out_dev);
   PP_HIP(hipGetLastError());
 }
 
@@ -757,9 +771,10 @@ void MixedWork::residual(const double2* b64_, double2* r, double2* out, double2*
 }
 
 int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
-                        const std::function<void(int, float2*, float, float2*)>& apply_P, const float2* b,
+                        const std::function<void(int, float2*, const double2*, float2*)>& apply_P, const float2* b,
                         double rtol, int max_it, double* rel_out, hipStream_t s, double beta_in, const double2* base,
-                        double2* xout, const std::function<bool(double&)>& first_read) {
+                        double2* xout, const std::function<bool(double&)>& first_read, int pre_iters,
+                        int* needed) {
   using cd = std::complex<double>;
   const int mm = std::min(max_it, m);
   PP_REQUIRE(mm >= 1 && mm + 3 <= 120, "inner basis size out of range");
@@ -772,6 +787,7 @@ int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply
   auto no_solve = [&]() {
     if (base != xout) PP_HIP(hipMemcpyAsync(xout, base, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
     if (rel_out) *rel_out = 0;
+    if (needed) *needed = 0;
     return 0;
   };
   if (!first_read && !(beta > 0)) return no_solve();
@@ -781,45 +797,64 @@ int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply
   std::vector<cd> y;
   int k = 0;
   double rel = 1.0;
-  float next_scale = 1.f;  // V_j is normalised by the preconditioner launch of iteration j
-  for (int j = 0; j < mm; ++j) {
+  // Results of iteration j go to their own slot of the pinned block (stride) and of its
+  // device copy.  The Arnoldi process needs no host value: iteration j normalises V_j by
+  // the device copy of |w|^2 from iteration j - 1.  So the first `pre` iterations run
+  // before the host reads anything; after that the host reads every iteration.
+  const int stride = 2 * (mm + 1) + 1;
+  PP_REQUIRE(mm * stride <= kSmall - 8, "inner basis too large for the host result block");
+  int pre = std::max(1, std::min(pre_iters, mm));
+  int processed = 0, first_conv = 0;
+  bool done = false;
+  for (int j = 0; j < mm && !done; ++j) {
     float2* vj = V.ptr + (size_t)j * n;
     float2* zj = Z.ptr + (size_t)j * n;
     float2* vn = V.ptr + (size_t)(j + 1) * n;
-    apply_P(j, vj, next_scale, zj);
+    const double2* n2 = j == 0 ? nullptr : hdev.ptr + (size_t)(j - 1) * stride + 2 * (j + 1);
+    apply_P(j, vj, n2, zj);
     apply_A(zj, vn);
-    // classical Gram-Schmidt, two passes, one host read
-    const int w1 = j + 2;  // h (j+1) + |w|^2
-    cgs2_32(j + 1, vn, small_dev, s);  // [first dots, |w|^2 | second dots, |w|^2 | final |w|^2]
-    read_small(2 * w1 + 1, s);
-    if (j == 0 && first_read) {
-      // the caller learns |rhs| (and may decide the solve is not needed) only now
-      if (!first_read(beta)) {
-        if (rel_out) *rel_out = 0;
-        return -1;
+    // classical Gram-Schmidt, two passes: [first dots, |w|^2 | second dots, |w|^2 | final |w|^2]
+    cgs2_32(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s);
+    if (j + 1 < pre) continue;
+    PP_HIP(hipStreamSynchronize(s));  // the finishing kernels wrote straight into small
+    for (int jj = processed; jj <= j && !done; ++jj) {
+      const double2* hs = small + (size_t)jj * stride;
+      const int w1 = jj + 2;  // h (jj+1) + |w|^2
+      if (jj == 0 && first_read) {
+        // the caller learns |rhs| (and may decide the solve is not needed) only now
+        if (!first_read(beta)) {
+          if (rel_out) *rel_out = 0;
+          if (needed) *needed = 0;
+          return -1;
+        }
+        if (!(beta > 0)) return no_solve();
       }
-      if (!(beta > 0)) return no_solve();
+      for (int i = 0; i <= jj; ++i) Hc(i, jj) = cd(hs[i].x + hs[w1 + i].x, hs[i].y + hs[w1 + i].y);
+      const double hn = std::sqrt(std::max(0.0, hs[2 * w1].x));
+      Hc(jj + 1, jj) = hn;
+      k = jj + 1;
+      std::vector<cd> A((size_t)(k + 1) * k), rhs_ls(k + 1, 0.0);
+      for (int c = 0; c < k; ++c)
+        for (int r = 0; r <= k; ++r) A[(size_t)c * (k + 1) + r] = Hc(r, c);
+      rhs_ls[0] = beta;
+      least_squares_small(k + 1, k, A, rhs_ls, y);
+      double e2 = 0;
+      for (int r = 0; r <= k; ++r) {
+        cd acc = r == 0 ? -beta : 0.0;
+        for (int c = 0; c < k; ++c) acc += Hc(r, c) * y[c];
+        e2 += std::norm(acc);
+      }
+      rel = std::sqrt(e2) / beta;
+      processed = jj + 1;
+      if (rel <= rtol && !first_conv) first_conv = k;
+      // a breakdown ends the basis here (later columns, if any were run, are not used);
+      // otherwise every column already computed is used, since each lowers the residual
+      if (!(hn > 1e-30 * beta)) done = true;
     }
-    for (int i = 0; i <= j; ++i)
-      Hc(i, j) = cd(host_small[i].x + host_small[w1 + i].x, host_small[i].y + host_small[w1 + i].y);
-    const double hn = std::sqrt(std::max(0.0, host_small[2 * w1].x));
-    Hc(j + 1, j) = hn;
-    k = j + 1;
-    std::vector<cd> A((size_t)(k + 1) * k), rhs_ls(k + 1, 0.0);
-    for (int c = 0; c < k; ++c)
-      for (int r = 0; r <= k; ++r) A[(size_t)c * (k + 1) + r] = Hc(r, c);
-    rhs_ls[0] = beta;
-    least_squares_small(k + 1, k, A, rhs_ls, y);
-    double e2 = 0;
-    for (int r = 0; r <= k; ++r) {
-      cd acc = r == 0 ? -beta : 0.0;
-      for (int c = 0; c < k; ++c) acc += Hc(r, c) * y[c];
-      e2 += std::norm(acc);
-    }
-    rel = std::sqrt(e2) / beta;
-    if (rel <= rtol || !(hn > 1e-30 * beta)) break;
-    next_scale = (float)(1.0 / hn);
+    if (first_conv) done = true;
+    pre = 0;
   }
+  if (needed) *needed = first_conv ? first_conv : k;
   LincombCoef yc;
   for (int i = 0; i < k; ++i) yc.c[i] = make_float2((float)y[i].real(), (float)y[i].imag());
   hipLaunchKernelGGL(lincomb_add32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, k, Z.ptr, n, yc, base, xout);
@@ -885,16 +920,16 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     ++stats.inner_ops;
   };
   int correction = 0;
-  // z = P (scale in); in is replaced by scale in (the normalised basis vector)
-  auto apply_P32 = [&](int j, float2* in, float scale, float2* out) {
+  // z = P (in / |in|) with |in|^2 on the device (n2 = NULL: in is already normalised); in
+  // is replaced by in / |in| (the normalised basis vector)
+  auto apply_P32 = [&](int j, float2* in, const double2* n2, float2* out) {
     // the learned correction acts on the first direction of the first correction, whose
     // right-hand side r = -2 i a H psi is the kind of vector it was trained on
-    if (j == 0 && correction == 0 && learned) {
-      if (scale != 1.f) hipLaunchKernelGGL(scale32_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, scale, in, in);
+    if (j == 0 && correction == 0 && learned && !n2) {
       learned_apply(*learned, K, ka, in, out, s);
       ++stats.learned_applies;
     } else {
-      K.apply32(ka, in, out, s, scale, scale != 1.f);
+      K.apply32(ka, in, out, s, 1.f, n2 != nullptr, n2);
     }
     ++stats.precond_applies;
   };
@@ -933,13 +968,18 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
                        work.V.ptr);
     double rel = 0;
     // x = (psi or x) + d, d from the inner solve
+    if ((int)pre_its.size() <= correction) pre_its.resize(correction + 1, 1);
+    int needed = 0;
     const int its = work.fgmres32(
         apply_A32, apply_P32, nullptr, inner_rtol, m, &rel, s, 0.0, x_is_psi ? psi : work.x64.ptr, work.x64.ptr,
         [&](double& beta) {
           if (pending >= 0 && read_pending()) return false;  // already converged: no correction
           beta = rn;
           return true;
-        });
+        },
+        pre_its[correction], &needed);
+    // the next step runs as many iterations of this correction before its first read
+    if (its >= 0) pre_its[correction] = std::max(1, needed);
     if (its < 0) {
       --stats.outer_iterations;
       converged = true;
@@ -1063,7 +1103,7 @@ extern "C" int pp_mixed_create(pp_hamiltonian* ham, const pp_kron* k, int basis_
                                double inner_rtol, int max_outer, pp_mixed** out) {
   return guarded([&] {
     PP_REQUIRE(ham && k && out, "null argument");
-    PP_REQUIRE(basis_size >= 1 && basis_size <= 100, "inner basis size must be in 1..100");
+    PP_REQUIRE(basis_size >= 1 && basis_size <= 60, "inner basis size must be in 1..60");
     PP_REQUIRE(tol_rel > 0 && inner_rtol > 0 && max_outer >= 1, "invalid tolerances");
     PP_REQUIRE((long long)k->k.nchan * k->k.n * k->k.n == ham->h.n, "Kronecker shape does not match psi");
     PP_REQUIRE(ham->h.fused, "the mixed solver needs the fused banded Hamiltonian");

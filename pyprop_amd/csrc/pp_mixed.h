@@ -27,12 +27,14 @@ struct Kron {
   void apply64(double2 a, double2* psi, hipStream_t s) const;
   void apply32(double2 a, float2* psi, hipStream_t s) const;
   // out <- K^-1 (in_scale in) (may alias in); write_back: in <- in_scale in as well
-  void apply32(double2 a, float2* in, float2* out, hipStream_t s, float in_scale = 1.f, bool write_back = false) const;
+  // in_n2 (device, may be NULL): the scale is 1 / sqrt(in_n2->x) instead of in_scale
+  void apply32(double2 a, float2* in, float2* out, hipStream_t s, float in_scale = 1.f, bool write_back = false,
+               const double2* in_n2 = nullptr) const;
   // out <- K^-1 (in + add_sigma * (*add_norm) * add), *add_norm read on the device
   void apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
                    float2* out, hipStream_t s) const;
   void launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
-                const float2* add, const double* add_norm, float add_sigma) const;
+                const float2* add, const double* add_norm, float add_sigma, const double2* in_n2) const;
   // psi <- K psi (FP64)
   void multiply64(double2 a, double2* psi, hipStream_t s) const;
 };
@@ -52,7 +54,7 @@ struct MixedStats {
 };
 
 struct MixedWork {
-  static constexpr int kSmall = 256;
+  static constexpr int kSmall = 8192;  // inner basis sizes up to 60
   long long n = -1;
   int m = 0;
   DeviceBuffer<float2> V, Z, w, tmp;
@@ -68,11 +70,12 @@ struct MixedWork {
   void ensure(long long n, int m);
   void read_small(int count, hipStream_t s);  // synchronise s, copy small[0..count) to host_small
   DeviceBuffer<double2> norms;   // device copies of step-level squared norms (|b|^2, |r0|^2, |r|^2)
+  DeviceBuffer<double2> hdev;    // device copy of the inner solve's results (the Arnoldi scaling)
   // out[0] = |a|^2, out[1] = |b|^2 (pinned), the same into out_dev (device)
   void norm2_pair(const double2* a, const double2* b, double2* out, double2* out_dev, hipStream_t s);
   // Two-pass classical Gram-Schmidt of w against V_0..V_{k-1} (FP64 sums): out[0..k] = first
   // dots and |w|^2, out[k+1..2k+1] = second dots and |w|^2, out[2k+2] = final |w|^2
-  void cgs2_32(int k, float2* w, double2* out, hipStream_t s);
+  void cgs2_32(int k, float2* w, double2* out, double2* out_dev, hipStream_t s);
   void norm2_32(const float2* w, double2* out, hipStream_t s);  // out[0] = |w|^2
   // r = b - r, out[0] = |r|^2 (FP64; pinned out and device out_dev)
   void residual(const double2* b, double2* r, double2* out, double2* out_dev, hipStream_t s);
@@ -82,13 +85,16 @@ struct MixedWork {
   // Flexible GMRES(m) on A d = rhs, d0 = 0, right preconditioner P_j (j = iteration), then
   // xout = base + d (FP64; base may alias xout).  rhs = NULL: V_0 already holds rhs / |rhs|
   // and beta = |rhs|, or, with first_read, beta is set by first_read at the first host
-  // synchronisation (after iteration 0), which may also return false: the solve is then
-  // abandoned without touching xout and -1 is returned.  apply_P(j, v, scale, z):
-  // v <- scale v (normalising the basis vector), z = P_j v.
+  // synchronisation, which may also return false: the solve is then abandoned without
+  // touching xout and -1 is returned.  apply_P(j, v, n2, z): v <- v / sqrt(n2->x) when n2 (a
+  // device squared norm) is given, normalising the basis vector, then z = P_j v.  The first
+  // pre_iters iterations run before the host reads anything; *needed (may be NULL) receives
+  // the iteration count at which the tolerance was first met.
   int fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
-               const std::function<void(int, float2*, float, float2*)>& apply_P, const float2* rhs, double rtol,
-               int max_it, double* rel_out, hipStream_t s, double beta, const double2* base, double2* xout,
-               const std::function<bool(double&)>& first_read = nullptr);
+               const std::function<void(int, float2*, const double2*, float2*)>& apply_P, const float2* rhs,
+               double rtol, int max_it, double* rel_out, hipStream_t s, double beta, const double2* base,
+               double2* xout, const std::function<bool(double&)>& first_read = nullptr, int pre_iters = 1,
+               int* needed = nullptr);
 };
 
 struct MixedCayley {
@@ -109,6 +115,7 @@ struct MixedCayley {
   unsigned long long z_gen = 0;
   const double2* z_psi = nullptr;
   std::vector<double> scal;
+  std::vector<int> pre_its;  // per correction: inner iterations run before the first host read
   void step(double2* psi, double dt, const double* scalings, hipStream_t s);
 };
 
