@@ -117,6 +117,48 @@ int pp_expand_potential(int terms, int64_t p0, int64_t r_count, const double* co
                         pp_cplx* out, pp_stream stream);
 
 /* ------------------------------------------------------------------------------
+ * Potential generation on the device (TensorPotentialGenerator.GeneratePotential,
+ * TensorPotentialGenerator.py:80-274: evaluate on the quadrature grid, then represent
+ * each B-spline rank in the basis, last rank first, :165-230)
+ * ------------------------------------------------------------------------------ */
+typedef struct pp_bspline_projection pp_bspline_projection;
+
+/* The projection of one B-spline rank onto a pair list: the state of
+ * RepresentPotentialInBasisBSpline (core/tensorpotential/basis_bspline.h:10-47) for one
+ * (basis, pairs, storage, differentiation).  Host inputs: quadrature weights (n_points),
+ * spline values and the differentiation-th derivative tables (n_splines, n_points), zero
+ * outside each spline's support [support_first, support_first + support_count), and
+ * pairs (pair_count, 2) as (row, col).  Entry (row, col) is
+ * BSplineGlobalOverlapIntegral(f, differentiation, row, col)
+ * (core/transform/bspline/bspline.cpp:372-466): the sum over the common support of
+ * w f B_i B_j^(d) with (i, j) = (min, max)(row, col), times (-1)^d when row > col, zero
+ * when |row - col| >= order, halved on the diagonal when hermitian ("Herm" storage).
+ * n_points <= 1200. */
+int pp_bspline_projection_create(int n_splines, int n_points, int order, const double* weights,
+                                 const double* values, const double* deriv_values, const int* support_first,
+                                 const int* support_count, int differentiation, int pair_count,
+                                 const int* pairs, int hermitian, pp_bspline_projection** out);
+int pp_bspline_projection_destroy(pp_bspline_projection* p);
+int pp_bspline_projection_info(const pp_bspline_projection* p, int* n_points, int* pair_count,
+                               int* max_support);
+/* One rank of RepresentPotentialInBasisBSpline on the device: src (real float64) viewed
+ * as (pre, n_points, post), dst as (pre, pair_count, post), float64 (dst_complex = 0) or
+ * complex128 with zero imaginary part (dst_complex = 1); accumulate: dst += instead of
+ * dst =.  src and dst are device buffers and must not overlap. */
+int pp_bspline_represent(const pp_bspline_projection* proj, int64_t pre, int64_t post, const double* src,
+                         void* dst, int dst_complex, int accumulate, pp_stream stream);
+/* Grid functions (device buffers), grid[a][q1][q2] for a < pair_count:
+ *   multipole: sum_l coeffs[l][a] r<^lams[l] / r>^(lams[l]+1), r< = min(r1[q1], r2[q2]),
+ *     lams ascending: the 1/r12 expansion CustomPotentialEvaluationR12 evaluates
+ *     (examples/tensor/helium_stabilization/coupledspherical.cpp:60-111);
+ *   separable: sum_t coeffs[t][a] f1[t][q1] f2[t][q2] (one-electron potentials,
+ *     examples/tensor/helium_stabilization/potential.cpp:9-76, coupledlength.cpp:9-123). */
+int pp_potential_grid_multipole(int64_t pair_count, int nlam, const int* lams, const double* coeffs, int n1,
+                                const double* r1, int n2, const double* r2, double* grid, pp_stream stream);
+int pp_potential_grid_separable(int64_t pair_count, int terms, const double* coeffs, int n1, const double* f1,
+                                int n2, const double* f2, double* grid, pp_stream stream);
+
+/* ------------------------------------------------------------------------------
  * Overlap (banded Cholesky of the B-spline overlap)
  * ------------------------------------------------------------------------------ */
 

@@ -56,6 +56,9 @@ def lib():
         L.oracle_pamp_propagate.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int, OP, P, P, C.c_double, C.c_double,
                                             P, P, P]
         L.oracle_pade_exp.argtypes = [C.c_int, P, C.c_int, P]
+        L.oracle_bspline_tables.argtypes = [C.c_int, P, C.c_int, C.c_int, P, C.c_int, P, P, P, P, P, P, P]
+        L.oracle_bspline_represent.argtypes = [C.c_int, P, C.c_int, C.c_int, P, C.c_int, P, P, C.c_int, C.c_int,
+                                               P, C.c_int, C.c_int64, C.c_int64, P, P]
         L.oracle_sys_create.restype = C.c_void_p
         L.oracle_sys_create.argtypes = [C.c_int, P]
         L.oracle_sys_destroy.argtypes = [P]
@@ -171,6 +174,42 @@ def pade_exp(A, order=6):
     out = np.zeros((m, m), dtype=np.complex128, order="F")
     _check(lib().oracle_pade_exp(m, _ptr(Af), order, _ptr(out)))
     return np.asarray(out)
+
+
+def _bspline_args(basis):
+    knots = np.ascontiguousarray(basis.knots, dtype=np.float64)
+    bp = np.ascontiguousarray(basis.breakpoints, dtype=np.float64)
+    nodes = np.ascontiguousarray(basis.nodes, dtype=np.float64)
+    weights = np.ascontiguousarray(basis.weights, dtype=np.float64)
+    keep = (knots, bp, nodes, weights)
+    return keep, (knots.size, _ptr(knots), int(basis.order), bp.size, _ptr(bp), nodes.size, _ptr(nodes),
+                  _ptr(weights))
+
+
+def bspline_tables(basis):
+    """(grid, weights, B, B', B'') of the oracle's own B-spline evaluation, from the
+    basis's knots, breakpoints and reference-interval Gauss rule."""
+    keep, args = _bspline_args(basis)
+    Q = (basis.breakpoints.size - 1) * basis.nodes.size
+    N = basis.knots.size - basis.order
+    grid, gw = np.zeros(Q), np.zeros(Q)
+    b0, b1, b2 = (np.zeros((N, Q)) for _ in range(3))
+    _check(lib().oracle_bspline_tables(*args, _ptr(grid), _ptr(gw), _ptr(b0), _ptr(b1), _ptr(b2)))
+    return grid, gw, b0, b1, b2
+
+
+def bspline_represent(basis, src, differentiation, pairs, hermitian, axis):
+    """Project the grid axis `axis` of the real array src onto the pair list (one rank of
+    RepresentPotentialInBasisBSpline)."""
+    keep, args = _bspline_args(basis)
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    pairs = np.ascontiguousarray(pairs, dtype=np.int32)
+    pre = int(np.prod(src.shape[:axis]))
+    post = int(np.prod(src.shape[axis + 1:]))
+    out = np.zeros(src.shape[:axis] + (pairs.shape[0],) + src.shape[axis + 1:])
+    _check(lib().oracle_bspline_represent(*args, int(differentiation), pairs.shape[0], _ptr(pairs), int(hermitian),
+                                          pre, post, _ptr(src), _ptr(out)))
+    return out
 
 
 class System:

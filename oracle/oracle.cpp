@@ -12,6 +12,11 @@
 //   oracle_overlap_apply     core/representation/overlapmatrix.cpp:111-229 (ztbmv/ztbsv loops)
 //   oracle_gmres_solve       core/krylov/gmres/gmres.h:260-505 (+ zgels by Householder QR, zlarfg convention)
 //   oracle_pamp_propagate    core/krylov/pamp/pamp/pamp.h:270-670 (Pade via zgetrf/zgetrs loops)
+//   oracle_bspline_*         core/transform/bspline/bspline.cpp:26-161 (value/derivative
+//                            recursions), :372-466 (BSplineGlobalOverlapIntegral),
+//                            core/tensorpotential/basis_bspline.h:10-47 (one rank of
+//                            RepresentPotentialInBasisBSpline), pyprop/bspline/BSpline.py:190-242
+//                            (global quadrature grid)
 //   oracle_sys_*             pyprop/tensorpotential/BasisPropagator.py:116-159,
 //                            pyprop/propagator/CayleyPropagator.py:71-150,
 //                            pyprop/propagator/PamPropagator.py:50-75,
@@ -852,5 +857,151 @@ extern "C" int oracle_sys_inner_product(void* p, const double* a, const double* 
     cd r = zdotc(s->N, reinterpret_cast<const cd*>(a), w.data());
     out[0] = r.real();
     out[1] = r.imag();
+  });
+}
+
+// ------------------------------------------------------------------------------------
+// B-spline projection of grid functions (potential generation)
+// ------------------------------------------------------------------------------------
+struct OracleBSpline {
+  int k = 0, N = 0, Q = 0;
+  std::vector<double> t, x, w;            // knots; global grid points and weights
+  std::vector<int> ivl;                   // interval of each grid point
+  std::vector<double> B[3];               // value, first and second derivative, (N, Q)
+  std::vector<int> first, count;          // support of each spline on the grid
+};
+
+// B(m, i, x) for every i, m = 1..k by the Cox-de Boor recursion; a term whose knot
+// difference vanishes is dropped (bspline.cpp:26-76 drops it through its eps guard on B).
+static std::vector<std::vector<double>> bspline_levels(const std::vector<double>& t, int k, double x) {
+  const int nk = (int)t.size();
+  std::vector<std::vector<double>> L(k + 1);
+  L[1].assign(nk - 1, 0.0);
+  for (int i = 0; i + 1 < nk; ++i) L[1][i] = (t[i] <= x && x < t[i + 1]) ? 1.0 : 0.0;
+  for (int m = 2; m <= k; ++m) {
+    L[m].assign(nk - m, 0.0);
+    for (int i = 0; i + m < nk; ++i) {
+      double v = 0;
+      const double d0 = t[i + m - 1] - t[i], d1 = t[i + m] - t[i + 1];
+      if (d0 > 0) v += (x - t[i]) / d0 * L[m - 1][i];
+      if (d1 > 0) v += (t[i + m] - x) / d1 * L[m - 1][i + 1];
+      L[m][i] = v;
+    }
+  }
+  return L;
+}
+
+static OracleBSpline bspline_setup(int nknots, const double* knots, int k, int nbp, const double* bp, int g,
+                                   const double* nodes, const double* weights) {
+  OracleBSpline b;
+  b.k = k;
+  b.t.assign(knots, knots + nknots);
+  b.N = nknots - k;
+  // global grid: g Gauss points per breakpoint interval, x -> (x - 1)(b - a)/2 + b
+  for (int m = 0; m + 1 < nbp; ++m) {
+    const double lo = bp[m], hi = bp[m + 1];
+    for (int q = 0; q < g; ++q) {
+      b.x.push_back((nodes[q] - 1.0) * (hi - lo) / 2.0 + hi);
+      b.w.push_back((hi - lo) / 2.0 * weights[q]);
+      b.ivl.push_back(m);
+    }
+  }
+  b.Q = (int)b.x.size();
+  for (auto& v : b.B) v.assign((size_t)b.N * b.Q, 0.0);
+  const auto& t = b.t;
+  for (int q = 0; q < b.Q; ++q) {
+    const auto L = bspline_levels(t, k, b.x[q]);
+    for (int i = 0; i < b.N; ++i) {
+      b.B[0][(size_t)i * b.Q + q] = L[k][i];
+      // B' = (k-1) [B(k-1,i) / (t[i+k-1]-t[i]) - B(k-1,i+1) / (t[i+k]-t[i+1])]
+      double d1 = 0;
+      const double a0 = t[i + k - 1] - t[i], a1 = t[i + k] - t[i + 1];
+      if (a0 > 0) d1 += (k - 1.0) / a0 * L[k - 1][i];
+      if (a1 > 0) d1 -= (k - 1.0) / a1 * L[k - 1][i + 1];
+      b.B[1][(size_t)i * b.Q + q] = d1;
+      // B'' from differentiating the B' formula once more
+      double d2 = 0;
+      if (k >= 3) {
+        auto dlow = [&](int j) {  // derivative of B(k-1, j)
+          double r = 0;
+          const double c0 = t[j + k - 2] - t[j], c1 = t[j + k - 1] - t[j + 1];
+          if (c0 > 0) r += (k - 2.0) / c0 * L[k - 2][j];
+          if (c1 > 0) r -= (k - 2.0) / c1 * L[k - 2][j + 1];
+          return r;
+        };
+        if (a0 > 0) d2 += (k - 1.0) / a0 * dlow(i);
+        if (a1 > 0) d2 -= (k - 1.0) / a1 * dlow(i + 1);
+      }
+      b.B[2][(size_t)i * b.Q + q] = d2;
+    }
+  }
+  // support: the grid points inside [t_i, t_{i+k}] (interior Gauss points only)
+  b.first.assign(b.N, 0);
+  b.count.assign(b.N, 0);
+  for (int i = 0; i < b.N; ++i) {
+    int f = -1, c = 0;
+    for (int q = 0; q < b.Q; ++q)
+      if (b.x[q] > t[i] && b.x[q] < t[i + k]) {
+        if (f < 0) f = q;
+        ++c;
+      }
+    b.first[i] = std::max(f, 0);
+    b.count[i] = c;
+  }
+  return b;
+}
+
+// BSplineGlobalOverlapIntegral(f, d, row, col): (i, j) = (min, max), factor (-1)^d when
+// row > col, zero when j - i >= k, else the sum over the common support of
+// w f B_i B_j^(d).
+static double overlap_integral(const OracleBSpline& b, const double* f, size_t fstride, int d, int row, int col) {
+  int i = row, j = col;
+  double factor = 1.0;
+  if (i > j) {
+    std::swap(i, j);
+    factor = (d % 2) ? -1.0 : 1.0;
+  }
+  if (j - i >= b.k) return 0.0;
+  const int lo = std::max(b.first[i], b.first[j]);
+  const int hi = std::min(b.first[i] + b.count[i], b.first[j] + b.count[j]);
+  double I = 0;
+  for (int q = lo; q < hi; ++q)
+    I += b.w[q] * f[(size_t)q * fstride] * b.B[0][(size_t)i * b.Q + q] * b.B[d][(size_t)j * b.Q + q];
+  return factor * I;
+}
+
+// Grid points, weights and value/derivative tables (for checks against other restatements).
+extern "C" int oracle_bspline_tables(int nknots, const double* knots, int k, int nbp, const double* bp, int g,
+                                     const double* nodes, const double* weights, double* grid, double* gw,
+                                     double* b0, double* b1, double* b2) {
+  return guard([&] {
+    OracleBSpline b = bspline_setup(nknots, knots, k, nbp, bp, g, nodes, weights);
+    std::copy(b.x.begin(), b.x.end(), grid);
+    std::copy(b.w.begin(), b.w.end(), gw);
+    std::copy(b.B[0].begin(), b.B[0].end(), b0);
+    std::copy(b.B[1].begin(), b.B[1].end(), b1);
+    std::copy(b.B[2].begin(), b.B[2].end(), b2);
+  });
+}
+
+// One rank of RepresentPotentialInBasisBSpline: src viewed as (pre, Q, post) (MapToRank3),
+// dst (pre, pairs, post) = scaling * BSplineGlobalOverlapIntegral(src(pre, :, post), d,
+// row, col), scaling 0.5 on the diagonal pairs of "Herm" storage.
+extern "C" int oracle_bspline_represent(int nknots, const double* knots, int k, int nbp, const double* bp, int g,
+                                        const double* nodes, const double* weights, int d, int npairs,
+                                        const int* pairs, int herm, int64_t pre, int64_t post, const double* src,
+                                        double* dst) {
+  return guard([&] {
+    if (d < 0 || d > 2) throw std::runtime_error("invalid derivative");
+    OracleBSpline b = bspline_setup(nknots, knots, k, nbp, bp, g, nodes, weights);
+    const int Q = b.Q;
+    for (int64_t a = 0; a < pre; ++a)
+      for (int p = 0; p < npairs; ++p) {
+        const int row = pairs[2 * p], col = pairs[2 * p + 1];
+        const double sc = (herm && row == col) ? 0.5 : 1.0;
+        for (int64_t c = 0; c < post; ++c)
+          dst[((size_t)a * npairs + p) * post + c] =
+              sc * overlap_integral(b, src + (size_t)a * Q * post + c, (size_t)post, d, row, col);
+      }
   });
 }

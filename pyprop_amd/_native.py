@@ -41,6 +41,12 @@ SIGNATURES = [
     ("pp_tp_info", I, [P, P, P, P, P]),
     ("pp_tp_destroy", I, [P]),
     ("pp_expand_potential", I, [I, I64, I64, P, P, P, P]),
+    ("pp_bspline_projection_create", I, [I, I, I, P, P, P, P, P, I, I, P, I, P]),
+    ("pp_bspline_projection_destroy", I, [P]),
+    ("pp_bspline_projection_info", I, [P, P, P, P]),
+    ("pp_bspline_represent", I, [P, I64, I64, P, P, I, I, P]),
+    ("pp_potential_grid_multipole", I, [I64, I, P, P, I, P, I, P, P, P]),
+    ("pp_potential_grid_separable", I, [I64, I, P, I, P, I, P, P, P]),
     ("pp_overlap_create", I, [I, I, P, P]),
     ("pp_overlap_get_factor", I, [P, P]),
     ("pp_overlap_destroy", I, [P]),

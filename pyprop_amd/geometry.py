@@ -9,6 +9,7 @@ pyprop/tensorpotential/basis/ReducedSphericalHarmonic.py:5-45,121-140 (reduced s
 from __future__ import annotations
 
 from dataclasses import dataclass
+from typing import Optional
 
 import numpy as np
 
@@ -27,6 +28,7 @@ class Geometry:
     pairs: np.ndarray          # (P, 2) int32, row = dest index, col = source index
     use_grid: bool             # potential is evaluated on a grid and projected
     extent: int                # basis size of this rank
+    basis: Optional[BSplineBasis] = None  # the B-spline basis of a B-spline rank
 
     @property
     def pair_count(self) -> int:
@@ -54,15 +56,15 @@ def bspline_geometry(basis: BSplineBasis, name: str) -> Geometry:
     N = basis.number_of_bsplines
     k = basis.order
     if g == "identity":
-        return Geometry("Ident", np.zeros((1, 2), np.int32), True, N)
+        return Geometry("Ident", np.zeros((1, 2), np.int32), True, N, basis)
     if g == "banded-nonhermitian":
-        return Geometry("BandNH", banded_nonhermitian_pairs(N, k - 1), True, N)
+        return Geometry("BandNH", banded_nonhermitian_pairs(N, k - 1), True, N, basis)
     if g == "banded-packed":
-        return Geometry("Simp", banded_packed_pairs(N, k), True, N)
+        return Geometry("Simp", banded_packed_pairs(N, k), True, N, basis)
     if g == "banded-hermitian":
-        return Geometry("Band", banded_hermitian_pairs(N, k), True, N)
+        return Geometry("Band", banded_hermitian_pairs(N, k), True, N, basis)
     if g in ("dense", "hermitian"):
-        return Geometry("Simp", dense_pairs(N), True, N)
+        return Geometry("Simp", dense_pairs(N), True, N, basis)
     raise ValueError("geometry %r not supported by the B-spline basis" % name)
 
 

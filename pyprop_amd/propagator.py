@@ -153,9 +153,14 @@ class Wavefunction:
 class BasisPropagator:
     """Sum of device tensor potentials (BasisPropagator.py:1-168)."""
 
-    def __init__(self, psi: Wavefunction, potentials: Sequence[TensorPotentialData], consolidate: bool = True):
+    def __init__(self, psi: Wavefunction, potentials: Sequence[TensorPotentialData], consolidate: bool = True,
+                 generation: str = "factors"):
+        """generation: how the device potentials are made, "factors" (device expansion of
+        the separable factors) or "grid" (grid evaluation and B-spline projection on the
+        device, as GeneratePotential does)."""
         self.psi = psi
-        self.PotentialList = [TensorPotential(p, psi.repr.shape) for p in potentials]
+        self.generation = generation
+        self.PotentialList = [TensorPotential(p, psi.repr.shape, generation) for p in potentials]
         if consolidate:
             self.ConsolidatePotentials()
         self.ham = None
@@ -176,10 +181,11 @@ class BasisPropagator:
                                                  np.concatenate([d.angular_coeffs, o.angular_coeffs]) if
                                                  d.angular_coeffs is not None and o.angular_coeffs is not None else None,
                                                  np.concatenate([d.radial_factors, o.radial_factors]) if
-                                                 d.radial_factors is not None and o.radial_factors is not None else None)
+                                                 d.radial_factors is not None and o.radial_factors is not None else None,
+                                                 d.grid_terms + o.grid_terms if d.grid_terms and o.grid_terms else None)
                     if merged.angular_coeffs is None:
                         merged._data = d.data + o.data
-                    cur = TensorPotential(merged, cur.psi_shape)
+                    cur = TensorPotential(merged, cur.psi_shape, self.generation)
                 else:
                     rest.append(other)
             out.append(cur)

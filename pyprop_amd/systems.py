@@ -55,6 +55,23 @@ class LaserSimpleLength:
 # ---------------------------------------------------------------------------------
 
 @dataclass
+class GridTerm:
+    """One grid function of a potential, as GeneratePotential evaluates it on the
+    quadrature grid before projecting (TensorPotentialGenerator.py:80-274), with the
+    differentiation of each radial rank (the potential's `differentiation<r>` keys).
+
+    kind "separable": F[a][q1][q2] = sum_t coeffs[t, a] factors[0][t, q1] factors[1][t, q2]
+    (one radial rank: F[a][q] = sum_t coeffs[t, a] factors[0][t, q]);
+    kind "multipole": F[a][q1][q2] = sum_t coeffs[t, a] r<^lams[t] / r>^(lams[t] + 1)."""
+
+    kind: str
+    coeffs: np.ndarray                 # (T, P0) real
+    differentiation: tuple             # per radial rank
+    factors: Optional[list] = None     # separable: per radial rank, (T, Q_r)
+    lams: Optional[list] = None        # multipole: the lambda of each term
+
+
+@dataclass
 class TensorPotentialData:
     """Potential data in the reference storage layout plus its separable factors.
 
@@ -68,6 +85,7 @@ class TensorPotentialData:
     time_function: Optional[Callable] = None
     angular_coeffs: Optional[np.ndarray] = None   # (T, P0) real
     radial_factors: Optional[np.ndarray] = None   # (T, P1[, P2]) real
+    grid_terms: Optional[list] = None             # GridTerm list: the same potential on the grid
 
     @property
     def data(self) -> np.ndarray:
@@ -195,13 +213,16 @@ def hydrogen_system(lmax: int = 7, order: int = 7, n_breakpoints: int = 117,
     coeffs = np.stack([np.ones(nl), ls * (ls + 1.0), np.ones(nl)])
     radial = np.stack([R_kin, R_inv2, R_coul])
     H0 = TensorPotentialData("RadialKineticEnergy+AngularKineticEnergy+CoulombPotential",
-                             [gdiag, gr], None, None, coeffs, radial)
+                             [gdiag, gr], None, None, coeffs, radial,
+                             [GridTerm("separable", np.ones((1, nl)), (2,), [np.full((1, r.size), -0.5)]),
+                              GridTerm("separable", coeffs[1:], (0,), [np.stack([1.0 / (2.0 * r * r), -Z / r])])])
 
     gdip = reduced_spherical_geometry(lmax, "dipoleselectionrule")
     cos_el = np.array([angular.reduced_cos_theta(int(a), int(b)) for a, b in gdip.pairs])
     R_r = radial_1d(basis, gr, r.copy(), 0)
     dip = TensorPotentialData("LaserPotentialLength", [gdip, gr], None, pulse,
-                              cos_el[None, :], R_r[None, :])
+                              cos_el[None, :], R_r[None, :],
+                              [GridTerm("separable", cos_el[None, :], (0,), [r[None, :].copy()])])
     sysm = System("hydrogen", (nl, N), [None, basis], [H0, dip], pulse)
     sysm.info = dict(lmax=lmax, order=order, n_breakpoints=n_breakpoints, rmax=float(bp[-1]),
                      xpartition=rp, gamma=gamma, n_inner=n_inner, Z=Z)
@@ -257,9 +278,16 @@ def helium_system(lmax: int = 5, L=(0, 1, 2), M: int = 0, order: int = 7,
         coeffs = np.stack([np.ones(nd), np.ones(nd), l1s * (l1s + 1), l2s * (l2s + 1), np.ones(nd)])
         return coeffs, radial
 
+    def h0_grid(nd_, l1v, l2v):
+        c1 = np.ones((1, nd_))
+        return [GridTerm("separable", c1, (2, 0), [-0.5 * one[None], one[None]]),
+                GridTerm("separable", c1, (0, 2), [one[None], -0.5 * one[None]]),
+                GridTerm("separable", np.stack([l1v * (l1v + 1), l2v * (l2v + 1), np.ones(nd_), np.ones(nd_)]),
+                         (0, 0), [np.stack([inv2, one, -Z / r, one]), np.stack([one, inv2, one, -Z / r])])]
+
     c0, r0 = h0_terms(g1, g2)
     H0 = TensorPotentialData("RadialKineticEnergy1+RadialKineticEnergy2+AngularKineticEnergy+CoulombPotential",
-                             [gdiag, g1, g2], None, None, c0, r0)
+                             [gdiag, g1, g2], None, None, c0, r0, h0_grid(nd, l1s, l2s))
 
     # 1/r12 multipoles: grid functions f_l(q1,q2) = r<^l / r>^(l+1)
     rmin_ = np.minimum.outer(r, r)
@@ -281,14 +309,15 @@ def helium_system(lmax: int = 5, L=(0, 1, 2), M: int = 0, order: int = 7,
     cR = np.stack([A[l] for l in lams])
     rR = multipole_radial(g1, g2, lams)
     R12 = TensorPotentialData("ElectronicCouplingPotential", [gr12, g1, g2],
-                              None, None, cR, rR)
+                              None, None, cR, rR, [GridTerm("multipole", cR, (0, 0), lams=list(lams))])
 
     gdip = coupled_geometry(channels, "selectionrule_linearpolarizedfield")
     I1, I2 = angular.dipole_length_coefficients(channels, gdip.pairs)
     cD = np.stack([I1, I2])
     rD = np.stack([sep(r, 0, one, 0), sep(one, 0, r, 0)])
     dip = TensorPotentialData("DipolePotentialLength", [gdip, g1, g2],
-                              None, pulse, cD, rD)
+                              None, pulse, cD, rD,
+                              [GridTerm("separable", cD, (0, 0), [np.stack([r, one]), np.stack([one, r])])])
 
     # preconditioner potentials: angular-diagonal H0 + monopole (diagonal) part of 1/r12,
     # both as banded-packed x banded-packed (examples/tensor/helium_stabilization/config.ini:57-60,101-108)
@@ -298,8 +327,9 @@ def helium_system(lmax: int = 5, L=(0, 1, 2), M: int = 0, order: int = 7,
     lm = sorted(Amono.keys())
     cPm = np.stack([Amono[l] for l in lm])
     rPm = multipole_radial(gp, gp, lm)
-    prec = [TensorPotentialData("PreconditionerH0", [gdiag, gp, gp], None, None, cP0, rP0),
-            TensorPotentialData("ElectronicCouplingPotentialMonopoleTerm", [gdiag, gp, gp], None, None, cPm, rPm)]
+    prec = [TensorPotentialData("PreconditionerH0", [gdiag, gp, gp], None, None, cP0, rP0, h0_grid(nd, l1s, l2s)),
+            TensorPotentialData("ElectronicCouplingPotentialMonopoleTerm", [gdiag, gp, gp], None, None, cPm, rPm,
+                                [GridTerm("multipole", cPm, (0, 0), lams=list(lm))])]
 
     sysm = System("helium", (nc, N, N), [None, basis, basis], [H0, R12, dip], pulse, channels, prec)
     sysm.info = dict(lmax=lmax, L=list(L), M=M, order=order, n_breakpoints=n_breakpoints,

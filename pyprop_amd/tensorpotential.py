@@ -85,7 +85,13 @@ core = _CoreNamespace()
 class TensorPotential:
     """A tensor potential resident on the device (mirror of TensorPotential.py:3-163)."""
 
-    def __init__(self, data: TensorPotentialData, psi_shape):
+    def __init__(self, data: TensorPotentialData, psi_shape, generation: str = "factors"):
+        """generation: "factors" expands the separable factors on the device; "grid"
+        evaluates the potential on the quadrature grid and projects it on the device
+        (GeneratePotential, TensorPotentialGenerator.py:80-274)."""
+        if generation not in ("factors", "grid"):
+            raise ValueError("generation must be 'factors' or 'grid'")
+        self.generation = generation
         self.Name = data.name
         self.GeometryList = data.geometries
         self.IsTimeDependent = data.is_time_dependent
@@ -103,6 +109,9 @@ class TensorPotential:
     def device_potential(self):
         """complex128 potential data in the reference layout on the device."""
         d = self.source
+        if self.generation == "grid":
+            from .potgen import generate_device_potential
+            return generate_device_potential(d)
         if d._data is None and d.angular_coeffs is not None:
             import torch
             coeffs = D.to_device(np.ascontiguousarray(d.angular_coeffs, dtype=np.float64))
