@@ -48,12 +48,44 @@ def build_workload(torch):
     repr_ = CombinedRepresentation(sysm.shape, sysm.bases)
     psi = Wavefunction(repr_, helium_initial_state(sysm))
     psi.Normalize()
-    base = BasisPropagator(psi, sysm.potentials)
+    # the potentials are generated on the device from their grid functions (GeneratePotential)
+    base = BasisPropagator(psi, sysm.potentials, generation="grid")
     t0 = time.time()
     base.SetupStep(0.01)
     torch.cuda.synchronize()
-    log("device potentials + plans %.1fs" % (time.time() - t0))
+    log("device potentials (grid + projection) + plans %.1fs" % (time.time() - t0))
     return sysm, repr_, psi, base
+
+
+def time_potgen(torch, sysm, reps=3):
+    """Device generation (grid evaluation + two B-spline projections) of every helium
+    potential: wall time per potential with HIP events on torch's current stream (the
+    library's stream), after one generation that builds the projection tables."""
+    from pyprop_amd.potgen import generate_device_potential
+    out = {}
+    for p in sysm.potentials:
+        V = generate_device_potential(p)
+        del V
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(reps):
+            V = generate_device_potential(p)
+            del V
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        P0, P1, P2 = p.shape
+        Q1, Q2 = (g.basis.grid.size for g in p.geometries[1:])
+        terms = len(p.grid_terms)
+        # per grid term: grid write + read, intermediate (P0, Q1, P2) write + read, complex result
+        # write (read + write when accumulating)
+        nbytes = terms * (2 * 8 * P0 * Q1 * Q2 + 2 * 8 * P0 * Q1 * P2) + (2 * terms - 1) * 16 * P0 * P1 * P2
+        out[p.name.split("+")[0]] = {"ms": ms, "shape": [P0, P1, P2], "grid_terms": terms,
+                                     "bytes": nbytes, "GB_per_s": nbytes / ms / 1e6}
+        log("device generation %-40s %s: %.2f ms, %.1f GB/s" % (p.name[:40], p.shape, ms, nbytes / ms / 1e6))
+        torch.cuda.empty_cache()
+    return out
 
 
 def profiled_traffic():
@@ -290,6 +322,8 @@ def main():
                     help="only copy 1 GiB device to device 20 times (HBM counter calibration for rocprof)")
     ap.add_argument("--profile-cayley", type=int, default=0, help="only run this many plain Cayley steps (for rocprof)")
     ap.add_argument("--profile-factored", action="store_true", help="only time the factored multiply (for rocprof)")
+    ap.add_argument("--profile-potgen", action="store_true",
+                    help="only time the device generation of the helium potentials (for rocprof)")
     ap.add_argument("--profile-mixed", type=int, default=0,
                     help="only run this many mixed-precision steps with the Kronecker preconditioner (for rocprof)")
     a = ap.parse_args()
@@ -315,6 +349,10 @@ def main():
         return
     if a.profile_batch:
         bench_batch(torch, steps=a.profile_batch, per_graph=min(10, a.profile_batch))
+        return
+    if a.profile_potgen:
+        from pyprop_amd.systems import helium_system
+        time_potgen(torch, helium_system())
         return
     sysm, repr_, psi, base = build_workload(torch)
     nnz = sum(int(np.prod(p.source.shape)) for p in base.PotentialList)
@@ -472,6 +510,7 @@ def main():
         (lrn_sps, 1e3 * lrn_sec / a.steps, lrn_err, lrn_dn, lrn_de, lrn_st))
 
     batch = None if a.no_batch else bench_batch(torch)
+    potgen = time_potgen(torch, sysm)
 
     cpu = None
     cpu1 = None
@@ -518,6 +557,7 @@ def main():
                                 "loss_first": learned.loss_history[0], "loss_last": learned.loss_history[-1]},
                    "energy": e_plain, "norm_drift_plain": abs(n_plain - norm0),
                    "batched_h_atoms": batch,
+                   "potential_generation": potgen,
                    "cpu_baseline_single_thread": cpu1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),

@@ -133,7 +133,7 @@ typedef struct pp_bspline_projection pp_bspline_projection;
  * (core/transform/bspline/bspline.cpp:372-466): the sum over the common support of
  * w f B_i B_j^(d) with (i, j) = (min, max)(row, col), times (-1)^d when row > col, zero
  * when |row - col| >= order, halved on the diagonal when hermitian ("Herm" storage).
- * n_points <= 1200. */
+ * n_points <= 2000. */
 int pp_bspline_projection_create(int n_splines, int n_points, int order, const double* weights,
                                  const double* values, const double* deriv_values, const int* support_first,
                                  const int* support_count, int differentiation, int pair_count,
@@ -157,6 +157,21 @@ int pp_potential_grid_multipole(int64_t pair_count, int nlam, const int* lams, c
                                 const double* r1, int n2, const double* r2, double* grid, pp_stream stream);
 int pp_potential_grid_separable(int64_t pair_count, int terms, const double* coeffs, int n1, const double* f1,
                                 int n2, const double* f2, double* grid, pp_stream stream);
+/* Grid evaluation and both projections in one pass (the grid is never stored): dst
+ * (pair_count, P1, P2) complex128 (+)= the potential whose grid function is the
+ * multipole / separable sum above, proj1 for the first radial rank (r1, n1 = its points),
+ * proj2 for the last (r2).  work: caller-owned device buffer of pair_count * n1 * P2
+ * doubles (the last rank projected, first rank still on the grid).  Separable with
+ * proj2 = NULL: one radial rank, dst (pair_count, P1) (+)= projection of
+ * sum_t coeffs[t][a] f1[t][q] (f2 and work unused). */
+int pp_potential_represent_multipole(const pp_bspline_projection* proj1, const pp_bspline_projection* proj2,
+                                     int64_t pair_count, int nlam, const int* lams, const double* coeffs,
+                                     const double* r1, const double* r2, double* work, pp_cplx* dst,
+                                     int accumulate, pp_stream stream);
+int pp_potential_represent_separable(const pp_bspline_projection* proj1, const pp_bspline_projection* proj2,
+                                     int64_t pair_count, int terms, const double* coeffs, const double* f1,
+                                     const double* f2, double* work, pp_cplx* dst, int accumulate,
+                                     pp_stream stream);
 
 /* ------------------------------------------------------------------------------
  * Overlap (banded Cholesky of the B-spline overlap)

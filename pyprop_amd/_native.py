@@ -47,6 +47,8 @@ SIGNATURES = [
     ("pp_bspline_represent", I, [P, I64, I64, P, P, I, I, P]),
     ("pp_potential_grid_multipole", I, [I64, I, P, P, I, P, I, P, P, P]),
     ("pp_potential_grid_separable", I, [I64, I, P, I, P, I, P, P, P]),
+    ("pp_potential_represent_multipole", I, [P, P, I64, I, P, P, P, P, P, P, I, P]),
+    ("pp_potential_represent_separable", I, [P, P, I64, I, P, P, P, P, P, I, P]),
     ("pp_overlap_create", I, [I, I, P, P]),
     ("pp_overlap_get_factor", I, [P, P]),
     ("pp_overlap_destroy", I, [P]),

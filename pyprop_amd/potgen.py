@@ -144,7 +144,59 @@ def grid_function(term, geometries):
 def generate_device_potential(data):
     """The complex128 device potential of a TensorPotentialData from its grid terms
     (GeneratePotential), summed over the terms as ConsolidatePotentials sums potentials
-    (BasisPropagator.py:68-107)."""
+    (BasisPropagator.py:68-107).  Each term is evaluated on the grid inside the projection
+    of the last rank (pp_potential_represent_*): the grid itself is never stored."""
+    if not data.grid_terms:
+        raise ValueError("potential %r has no grid description" % data.name)
+    torch = D.torch
+    lib = N.load()
+    s = D.stream_handle()
+    radial = data.geometries[1:]
+    if len(radial) not in (1, 2):
+        raise ValueError("one or two B-spline ranks supported")
+    P0 = data.geometries[0].pair_count
+    out = torch.empty(data.shape, dtype=torch.complex128, device="cuda")
+    work = None
+    for k, term in enumerate(data.grid_terms):
+        if len(term.differentiation) != len(radial):
+            raise ValueError("one differentiation per radial rank")
+        if term.coeffs.shape[1] != P0:
+            raise ValueError("coefficients do not match the angular pairs")
+        projs = [projection_for(g, d) for g, d in zip(radial, term.differentiation)]
+        if len(radial) == 2 and work is None:
+            work = torch.empty((P0, projs[0].n_points, projs[1].pair_count), dtype=torch.float64, device="cuda")
+        coeffs = D.to_device(np.ascontiguousarray(term.coeffs, dtype=np.float64))
+        T = term.coeffs.shape[0]
+        if term.kind == "separable":
+            fs = [D.to_device(np.ascontiguousarray(f, dtype=np.float64).reshape(T, g.basis.grid.size))
+                  for f, g in zip(term.factors, radial)]
+            if len(radial) == 1:
+                N.check(lib.pp_potential_represent_separable(projs[0].h, None, P0, T, D.ptr(coeffs), D.ptr(fs[0]),
+                                                             None, None, D.ptr(out), int(k > 0), s))
+            else:
+                N.check(lib.pp_potential_represent_separable(projs[0].h, projs[1].h, P0, T, D.ptr(coeffs),
+                                                             D.ptr(fs[0]), D.ptr(fs[1]), D.ptr(work), D.ptr(out),
+                                                             int(k > 0), s))
+        elif term.kind == "multipole":
+            if len(radial) != 2:
+                raise ValueError("a multipole grid needs two radial ranks")
+            lams = np.asarray(term.lams, dtype=np.int32)
+            if np.any(np.diff(lams) < 0) or lams.size != T:
+                raise ValueError("multipole orders must be ascending, one per coefficient row")
+            dl = D.to_device(lams)
+            r1 = D.to_device(np.ascontiguousarray(radial[0].basis.grid, dtype=np.float64))
+            r2 = D.to_device(np.ascontiguousarray(radial[1].basis.grid, dtype=np.float64))
+            N.check(lib.pp_potential_represent_multipole(projs[0].h, projs[1].h, P0, T, D.ptr(dl), D.ptr(coeffs),
+                                                         D.ptr(r1), D.ptr(r2), D.ptr(work), D.ptr(out), int(k > 0),
+                                                         s))
+        else:
+            raise ValueError("unknown grid term kind %r" % term.kind)
+    return out
+
+
+def generate_device_potential_staged(data):
+    """The same potential with the grid stored between the steps (pp_potential_grid_*,
+    then pp_bspline_represent per rank), GeneratePotential's steps 4 and 5 one by one."""
     if not data.grid_terms:
         raise ValueError("potential %r has no grid description" % data.name)
     out = None

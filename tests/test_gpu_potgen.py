@@ -31,7 +31,7 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("which", ["hydrogen", "helium"])
 def test_device_generation_vs_oracle(torch, which):
-    from pyprop_amd.potgen import generate_device_potential
+    from pyprop_amd.potgen import generate_device_potential, generate_device_potential_staged
     from pyprop_amd.systems import helium_system, hydrogen_system
     say("device potential generation vs oracle: %s" % which)
     if which == "hydrogen":
@@ -41,13 +41,14 @@ def test_device_generation_vs_oracle(torch, which):
         s = helium_system(lmax=2, L=(0, 1, 2), order=5, n_breakpoints=12, rmax=15.0, n_inner=6, gamma=2.5)
         pots = s.potentials + s.precond_potentials
     for p in pots:
-        V = generate_device_potential(p).cpu().numpy()
         ref = oracle_generate(p)
-        err = _rel(V.real, ref)
-        say("  %-60s %s rel err %.2e" % (p.name[:60], V.shape, err))
-        assert V.shape == ref.shape
-        assert np.all(V.imag == 0)
-        assert err < 1e-13
+        for fn in (generate_device_potential, generate_device_potential_staged):
+            V = fn(p).cpu().numpy()
+            err = _rel(V.real, ref)
+            say("  %-50s %-12s %s rel err %.2e" % (p.name[:50], "staged" if fn is generate_device_potential_staged else "fused", V.shape, err))
+            assert V.shape == ref.shape
+            assert np.all(V.imag == 0)
+            assert err < 1e-13
 
 
 def test_device_projection_one_rank_all_storages(torch, rng):
