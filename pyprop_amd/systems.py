@@ -86,6 +86,8 @@ class TensorPotentialData:
     angular_coeffs: Optional[np.ndarray] = None   # (T, P0) real
     radial_factors: Optional[np.ndarray] = None   # (T, P1[, P2]) real
     grid_terms: Optional[list] = None             # GridTerm list: the same potential on the grid
+    filename: Optional[str] = None                # load the data from this HDF5 file instead
+    dataset: Optional[str] = None                 # (TensorPotential.py:56-71, group path in the file)
 
     @property
     def data(self) -> np.ndarray:

@@ -109,6 +109,10 @@ class TensorPotential:
     def device_potential(self):
         """complex128 potential data in the reference layout on the device."""
         d = self.source
+        if d.filename:
+            # cached potential (TensorPotential.py:56-71): basis pairs checked against the file
+            from .serialization import LoadTensorPotential
+            return D.to_device(LoadTensorPotential(d.filename, d.dataset, self))
         if self.generation == "grid":
             from .potgen import generate_device_potential
             return generate_device_potential(d)
