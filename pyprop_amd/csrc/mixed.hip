@@ -304,16 +304,125 @@ __global__ __launch_bounds__(256) void overlap_elem32_kernel(const float* __rest
   y[idx] = acc;
 }
 
+// S1 (x) S2 on one channel per workgroup: the four passes U1, U1^T (along r1), U2, U2^T
+// (along r2) of the per-pass kernel above, with the same per-element sums, on an n1 x n2
+// block held in LDS with both factors.  Rank 0 (the channel) has no overlap.  Only one
+// workgroup per channel runs, so each has 16 waves and reads only LDS after its loads.
+template <int SD>
+__global__ __launch_bounds__(1024) void overlap_ss32_kernel(const float* __restrict__ gu1, int n1,
+                                                            const float* __restrict__ gu2, int n2,
+                                                            const float2* __restrict__ x, float2* __restrict__ y) {
+  extern __shared__ float2 ssbuf[];  // two n1 x n2 blocks, then both factors
+  const int m = n1 * n2;
+  float2* A = ssbuf;
+  float2* B = ssbuf + m;
+  float* u1 = reinterpret_cast<float*>(ssbuf + 2 * m);
+  float* u2 = u1 + n1 * (SD + 1);
+  const long long off = (long long)blockIdx.x * m;
+  for (int e = threadIdx.x; e < m; e += blockDim.x) A[e] = x[off + e];
+  for (int e = threadIdx.x; e < n1 * (SD + 1); e += blockDim.x) u1[e] = gu1[e];
+  for (int e = threadIdx.x; e < n2 * (SD + 1); e += blockDim.x) u2[e] = gu2[e];
+  __syncthreads();
+  for (int e = threadIdx.x; e < m; e += blockDim.x) {  // B = U1 A
+    const int i = e / n2, c = e - i * n2;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) {
+      const int j = i + d;
+      if (j < n1) {
+        const float a = u1[j * (SD + 1) + SD - d];
+        const float2 v = A[j * n2 + c];
+        acc.x = fmaf(a, v.x, acc.x);
+        acc.y = fmaf(a, v.y, acc.y);
+      }
+    }
+    B[e] = acc;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < m; e += blockDim.x) {  // A = U1^T B
+    const int i = e / n2, c = e - i * n2;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) {
+      const int j = i - d;
+      if (j >= 0) {
+        const float a = u1[i * (SD + 1) + SD - d];
+        const float2 v = B[j * n2 + c];
+        acc.x = fmaf(a, v.x, acc.x);
+        acc.y = fmaf(a, v.y, acc.y);
+      }
+    }
+    A[e] = acc;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < m; e += blockDim.x) {  // B = U2 A (along the contiguous axis)
+    const int i1 = e / n2, i = e - i1 * n2;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) {
+      const int j = i + d;
+      if (j < n2) {
+        const float a = u2[j * (SD + 1) + SD - d];
+        const float2 v = A[i1 * n2 + j];
+        acc.x = fmaf(a, v.x, acc.x);
+        acc.y = fmaf(a, v.y, acc.y);
+      }
+    }
+    B[e] = acc;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < m; e += blockDim.x) {  // y = U2^T B
+    const int i1 = e / n2, i = e - i1 * n2;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) {
+      const int j = i - d;
+      if (j >= 0) {
+        const float a = u2[i * (SD + 1) + SD - d];
+        const float2 v = B[i1 * n2 + j];
+        acc.x = fmaf(a, v.x, acc.x);
+        acc.y = fmaf(a, v.y, acc.y);
+      }
+    }
+    y[off + e] = acc;
+  }
+}
+
 void MixedWork::overlap_multiply32(const Repr& r, const float2* in, float2* out, float2* scratch, hipStream_t s) {
+  for (int rank = 0; rank < r.ndim; ++rank) {
+    const OverlapFactor* ov = r.ov[rank];
+    if (ov && u32[rank].count == 0) {
+      std::vector<float> h(ov->u_host.begin(), ov->u_host.end());
+      u32[rank].upload(h.data(), h.size(), s);
+    }
+  }
+  // (channel, r1, r2) with both radial overlaps: one launch, one channel per workgroup
+  if (r.ndim == 3 && !r.ov[0] && r.ov[1] && r.ov[2] && r.ov[1]->sd == r.ov[2]->sd &&
+      r.ov[1]->n == r.extents[1] && r.ov[2]->n == r.extents[2] &&
+      2 * (size_t)r.extents[1] * r.extents[2] * sizeof(float2) +
+              (size_t)(r.extents[1] + r.extents[2]) * (r.ov[1]->sd + 1) * sizeof(float) <=
+          65536 &&
+      in != out) {
+    const int n1 = r.extents[1], n2 = r.extents[2];
+    const size_t lds = 2 * (size_t)n1 * n2 * sizeof(float2) + (size_t)(n1 + n2) * (r.ov[1]->sd + 1) * sizeof(float);
+    switch (r.ov[1]->sd) {
+#define PP_SS(K)                                                                                       \
+  case K:                                                                                              \
+    hipLaunchKernelGGL((overlap_ss32_kernel<K>), dim3(r.extents[0]), dim3(1024), lds, s, u32[1].ptr, n1, \
+                       u32[2].ptr, n2, in, out);                                                       \
+    PP_HIP(hipGetLastError());                                                                         \
+    return;
+      PP_SS(1) PP_SS(2) PP_SS(3) PP_SS(4) PP_SS(5) PP_SS(6) PP_SS(7) PP_SS(8)
+#undef PP_SS
+      default:
+        break;
+    }
+  }
   const float2* cur = in;
   bool any = false;
   for (int rank = 0; rank < r.ndim; ++rank) {
     const OverlapFactor* ov = r.ov[rank];
     if (!ov) continue;
-    if (u32[rank].count == 0) {
-      std::vector<float> h(ov->u_host.begin(), ov->u_host.end());
-      u32[rank].upload(h.data(), h.size(), s);
-    }
     long long post = 1;
     for (int q = rank + 1; q < r.ndim; ++q) post *= r.extents[q];
     const long long total = n;
