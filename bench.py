@@ -324,6 +324,9 @@ def main():
     ap.add_argument("--profile-factored", action="store_true", help="only time the factored multiply (for rocprof)")
     ap.add_argument("--profile-potgen", action="store_true",
                     help="only time the device generation of the helium potentials (for rocprof)")
+    ap.add_argument("--profile-learned", type=int, default=0,
+                    help="only train the network on a few trajectories, then run this many mixed steps with the "
+                         "learned preconditioner (for rocprof)")
     ap.add_argument("--profile-mixed", type=int, default=0,
                     help="only run this many mixed-precision steps with the Kronecker preconditioner (for rocprof)")
     a = ap.parse_args()
@@ -376,6 +379,23 @@ def main():
         for dtype in (torch.complex128, torch.complex64):
             t = time_factored(torch, fh, psi, dtype, reps=20)
             log("factored %s %.1f us" % (dtype, t * 1e6))
+        return
+    if a.profile_learned:
+        pc = RadialTwoElectronPreconditioner(psi, sysm)
+        plain = CayleyPropagator(psi, base, pc, krylov_basis_size=30, krylov_tolerance=1e-10)
+        plain.SetupStep(dt)
+        psi0 = psi.data.clone()
+        kron = KroneckerPreconditioner(psi, sysm)
+        learned = LearnedPreconditioner(kron, hidden=512, seed=1)
+        x, tt, ch = generate_training_pairs(learned, base, plain, psi, dt, 4, 4, seed=2024)
+        learned.fit(x, ch, tt, epochs=2, batch=256, lr=1e-3, seed=1)
+        psi.data.copy_(psi0)
+        prop = MixedCayleyPropagator(psi, base, kron, learned=learned, tolerance=1e-10,
+                                     factored=tuned_factored(base)[0])
+        prop.SetupStep(dt)
+        sec, _ = run_steps(torch, prop, psi, 0.0, dt, 2, a.profile_learned)
+        log("%d learned mixed steps %.2f ms/step, last %s" % (a.profile_learned, 1e3 * sec / a.profile_learned,
+                                                              prop.GetStats()))
         return
     if a.profile_mixed:
         prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-10,
