@@ -67,7 +67,10 @@ def projection_for(geometry, differentiation: int) -> BSplineProjection:
         raise ValueError("rank is not a B-spline rank")
     if geometry.storage_id not in ("Simp", "Herm", "Band", "BandNH"):
         raise ValueError("storage %r has no B-spline projection" % geometry.storage_id)
-    key = (id(geometry.basis), geometry.storage_id, geometry.pairs.tobytes(), int(differentiation))
+    b = geometry.basis
+    # keyed on the basis's content (an id could be reused by a new basis after collection)
+    key = (int(b.order), b.knots.tobytes(), b.grid.tobytes(), b.grid_weights.tobytes(), geometry.storage_id,
+           geometry.pairs.tobytes(), int(differentiation))
     p = _cache.get(key)
     if p is None:
         p = BSplineProjection(geometry.basis, geometry.pairs, differentiation, geometry.storage_id == "Herm")
