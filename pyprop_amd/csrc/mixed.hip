@@ -238,6 +238,8 @@ struct LincombCoef {
   float2 c[120];
 };
 
+constexpr int kDevLsMax = 16;  // inner iterations of a correction solved on the device
+
 // x = base + sum_j c_j Z_j (the combination in FP32, added in FP64; base may alias x)
 __global__ void lincomb_add32_kernel(long long n, int m, const float2* __restrict__ Z, long long ld, LincombCoef cs,
                                      const double2* base, double2* x) {
@@ -253,6 +255,92 @@ __global__ void lincomb_add32_kernel(long long n, int m, const float2* __restric
     const double2 b = base[i];
     x[i] = make_double2(b.x + (double)t.x, b.y + (double)t.y);
   }
+}
+
+// The same combination with the coefficients on the device (m <= kDevLsMax).
+__global__ void lincomb_add32_dev_kernel(long long n, int m, const float2* __restrict__ Z, long long ld,
+                                         const float2* __restrict__ cs, const double2* base, double2* x) {
+  __shared__ float2 c[kDevLsMax];
+  if (threadIdx.x < kDevLsMax) c[threadIdx.x] = threadIdx.x < m ? cs[threadIdx.x] : make_float2(0.f, 0.f);
+  __syncthreads();
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    float2 t = make_float2(0.f, 0.f);
+    for (int j = 0; j < m; ++j) {
+      const float2 v = Z[j * ld + i];
+      t.x = fmaf(v.x, c[j].x, t.x);
+      t.x = fmaf(-v.y, c[j].y, t.x);
+      t.y = fmaf(v.x, c[j].y, t.y);
+      t.y = fmaf(v.y, c[j].x, t.y);
+    }
+    const double2 b = base[i];
+    x[i] = make_double2(b.x + (double)t.x, b.y + (double)t.y);
+  }
+}
+
+__device__ inline double2 zmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ inline double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 zscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+
+// The inner least-squares problem min |beta e1 - Hbar y| on the device, from the
+// Gram-Schmidt results of k iterations (hres + j * stride holds iteration j's first dots and
+// |w|^2, second dots and |w|^2, final |w|^2: Hbar(i, j) = first + second dots, Hbar(j+1, j) =
+// |w| after both passes, the sums the host forms) and beta^2 = beta2->x.  Givens QR in
+// FP64 on one thread.  y -> ycoef (FP32, for the combination); the residual estimate
+// |g_{j+1}| / beta after each column j -> rel[j].x.  A column that the rotations leave
+// (numerically) zero, as after a breakdown, gets coefficient 0.
+__global__ void hessenberg_lsq_kernel(int k, int stride, const double2* __restrict__ hres,
+                                      const double2* __restrict__ beta2, float2* __restrict__ ycoef,
+                                      double2* __restrict__ rel) {
+  // the work arrays live in LDS (one thread uses them), not in per-lane scratch memory
+  __shared__ double2 Rm[kDevLsMax][kDevLsMax];
+  __shared__ double2 g[kDevLsMax + 1];
+  __shared__ double cs[kDevLsMax];
+  __shared__ double2 sn[kDevLsMax];
+  __shared__ double2 col[kDevLsMax + 1];
+  __shared__ double2 y[kDevLsMax];
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double beta = sqrt(fmax(0.0, beta2->x));
+  for (int i = 0; i <= k; ++i) g[i] = make_double2(i == 0 ? beta : 0.0, 0.0);
+  double scale = 0.0;
+  for (int j = 0; j < k; ++j) {
+    const double2* hs = hres + (long long)j * stride;
+    const int w1 = j + 2;
+    for (int i = 0; i <= j; ++i) col[i] = zadd(hs[i], hs[w1 + i]);
+    col[j + 1] = make_double2(sqrt(fmax(0.0, hs[2 * w1].x)), 0.0);
+    for (int i = 0; i <= j + 1; ++i) scale = fmax(scale, fabs(col[i].x) + fabs(col[i].y));
+    for (int i = 0; i < j; ++i) {
+      const double2 x = col[i], y = col[i + 1];
+      col[i] = zadd(zscale(cs[i], x), zmul(sn[i], y));
+      col[i + 1] = zadd(zscale(-1.0, zmul(zconj(sn[i]), x)), zscale(cs[i], y));
+    }
+    const double2 a = col[j], b = col[j + 1];
+    const double na = hypot(a.x, a.y), nb = hypot(b.x, b.y), rho = hypot(na, nb);
+    if (rho == 0.0) {
+      cs[j] = 1.0;
+      sn[j] = make_double2(0.0, 0.0);
+    } else if (na == 0.0) {
+      cs[j] = 0.0;
+      sn[j] = zscale(1.0 / nb, zconj(b));
+    } else {
+      cs[j] = na / rho;
+      sn[j] = zscale(1.0 / (na * rho), zmul(a, zconj(b)));
+    }
+    col[j] = zadd(zscale(cs[j], a), zmul(sn[j], b));
+    for (int i = 0; i <= j; ++i) Rm[i][j] = col[i];
+    const double2 x = g[j];
+    g[j] = zscale(cs[j], x);
+    g[j + 1] = zscale(-1.0, zmul(zconj(sn[j]), x));
+    rel[j] = make_double2(beta > 0 ? hypot(g[j + 1].x, g[j + 1].y) / beta : 0.0, 0.0);
+  }
+  for (int i = k - 1; i >= 0; --i) {
+    double2 t = g[i];
+    for (int l = i + 1; l < k; ++l) t = zadd(t, zscale(-1.0, zmul(Rm[i][l], y[l])));
+    const double2 d = Rm[i][i];
+    const double dd = d.x * d.x + d.y * d.y;
+    y[i] = (dd > 0.0 && sqrt(dd) > 1e-14 * scale) ? zscale(1.0 / dd, zmul(t, zconj(d))) : make_double2(0.0, 0.0);
+  }
+  for (int i = 0; i < k; ++i) ycoef[i] = make_float2((float)y[i].x, (float)y[i].y);
 }
 
 // partial[block][0] = |a|^2, partial[block][1] = |b|^2 over the block's share (FP64)
@@ -817,6 +905,7 @@ void MixedWork::ensure(long long n_, int m_) {
   h64.resize(n);
   norms.resize(4);
   hdev.resize(kSmall);
+  ycoef.resize(kDevLsMax);
 }
 
 void MixedWork::read_small(int count, hipStream_t s) {
@@ -972,6 +1061,34 @@ int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply
   return k;
 }
 
+// k inner iterations with no host read: iteration j normalises V_j from the device copy of
+// iteration j-1's |w|^2, the least-squares problem is solved on the device and
+// xout = base + Z y is formed from the device coefficients.  V_0 must hold the normalised
+// right-hand side, whose squared norm is beta2_dev->x.  The residual estimate after each
+// iteration goes to small[est .. est + k), readable after the caller's next synchronisation.
+void MixedWork::fgmres32_device(const std::function<void(const float2*, float2*)>& apply_A,
+                                const std::function<void(int, float2*, const double2*, float2*)>& apply_P, int k,
+                                const double2* beta2_dev, const double2* base, double2* xout, int est,
+                                hipStream_t s) {
+  PP_REQUIRE(k >= 1 && k <= kDevLsMax && k <= m, "device inner solve size out of range");
+  PP_REQUIRE(est >= 0 && est + k <= kSmall, "estimate slots outside the host result block");
+  const int stride = 2 * (k + 1) + 1;
+  for (int j = 0; j < k; ++j) {
+    float2* vj = V.ptr + (size_t)j * n;
+    float2* zj = Z.ptr + (size_t)j * n;
+    float2* vn = V.ptr + (size_t)(j + 1) * n;
+    const double2* n2 = j == 0 ? nullptr : hdev.ptr + (size_t)(j - 1) * stride + 2 * (j + 1);
+    apply_P(j, vj, n2, zj);
+    apply_A(zj, vn);
+    cgs2_32(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s);
+  }
+  hipLaunchKernelGGL(hessenberg_lsq_kernel, dim3(1), dim3(64), 0, s, k, stride, hdev.ptr, beta2_dev, ycoef.ptr,
+                     small_dev + est);
+  hipLaunchKernelGGL(lincomb_add32_dev_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, k, Z.ptr, n, ycoef.ptr, base,
+                     xout);
+  PP_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------------------------
 // mixed Cayley step
 // ------------------------------------------------------------------------------------
@@ -1016,6 +1133,7 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   // |b|^2 and |r|^2: written to the pinned block for the host and to the device for the
   // scaling of V_0; the host reads them at the first synchronisation of the first correction
   constexpr int kB = MixedWork::kSmall - 4, kR0 = MixedWork::kSmall - 3, kR = MixedWork::kSmall - 2;
+  constexpr int kEst = MixedWork::kSmall - 8 - 8 * kDevLsMax;  // device-mode residual estimates
   work.norm2_pair(work.b64.ptr, work.r64.ptr, work.small_dev + kB, work.norms.ptr, s);
 
   auto apply_A32 = [&](const float2* in, float2* out) {
@@ -1044,6 +1162,18 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   };
   bool converged = false;
   bool x_is_psi = true;  // x has not been formed yet: it equals psi
+  // r = b - (S x + i a H x) into r64 (the residual pass follows)
+  auto residual_multiply = [&]() {
+    if (fused_s) {
+      fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, nullptr, make_double2(0.0, a), s, make_double2(1.0, 0.0));
+    } else {
+      vec_copy(n, work.x64.ptr, work.t64.ptr, s);
+      R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
+      if (fham) fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
+      else H.apply(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
+    }
+    ++stats.outer_ops;
+  };
   double bnorm = -1.0, rn = -1.0;
   // a residual norm written by a kernel and not yet read: its index in the pinned block and
   // its device copy
@@ -1058,7 +1188,42 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     stats.rel_residual = bnorm > 0 ? rn / bnorm : 0.0;
     return rn <= tol_rel * bnorm;
   };
-  for (int outer = 0;; ++outer) {
+  // Device mode: when the previous step has shown how many corrections and inner iterations
+  // a step needs, run them with no host read in between (each inner least-squares problem
+  // solved on the device) and synchronise once, at the final residual.  A step that is not
+  // converged then continues below, one host-driven correction at a time.
+  int dev_corr = device_mode && prev_corrections > 0 && (int)pre_its.size() >= prev_corrections ? prev_corrections
+                                                                                                  : 0;
+  for (int c = 0; c < dev_corr; ++c)
+    if (pre_its[c] > kDevLsMax || pre_its[c] > m) dev_corr = 0;
+  if (dev_corr > 0) {
+    for (int c = 0; c < dev_corr; ++c) {
+      ++stats.outer_iterations;
+      hipLaunchKernelGGL(f64_to_f32_unit_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, pending_dev, work.r64.ptr,
+                         work.V.ptr);
+      work.fgmres32_device(apply_A32, apply_P32, pre_its[c], pending_dev, x_is_psi ? psi : work.x64.ptr,
+                           work.x64.ptr, kEst + c * kDevLsMax, s);
+      stats.inner_iterations += pre_its[c];
+      x_is_psi = false;
+      ++correction;
+      residual_multiply();
+      work.residual(work.b64.ptr, work.r64.ptr, work.small_dev + kR, work.norms.ptr + 2, s);
+      pending = kR;
+      pending_dev = work.norms.ptr + 2;
+    }
+    PP_HIP(hipStreamSynchronize(s));
+    for (int c = 0; c < dev_corr; ++c) {
+      // the next step runs as many iterations as this correction needed to reach inner_rtol
+      const int k = pre_its[c];
+      int needed = 0;
+      for (int j = 0; j < k && !needed; ++j)
+        if (work.small[kEst + c * kDevLsMax + j].x <= inner_rtol) needed = j + 1;
+      pre_its[c] = needed ? needed : std::min(k + 1, std::min(kDevLsMax, m));
+    }
+    if (read_pending()) converged = true;
+    predicted = HUGE_VAL;
+  }
+  for (int outer = stats.outer_iterations; !converged; ++outer) {
     const bool limit = outer >= max_outer || stats.inner_iterations > fallback_iterations;
     // Read a pending residual now when it is likely converged or a limit is reached; else
     // the next correction starts at once and the residual is read at that correction's
@@ -1097,21 +1262,13 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     stats.inner_iterations += its;
     x_is_psi = false;
     ++correction;
-    // r = b - (S x + i a H x)
-    if (fused_s) {
-      fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, nullptr, make_double2(0.0, a), s, make_double2(1.0, 0.0));
-    } else {
-      vec_copy(n, work.x64.ptr, work.t64.ptr, s);
-      R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
-      if (fham) fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
-      else H.apply(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
-    }
-    ++stats.outer_ops;
+    residual_multiply();
     work.residual(work.b64.ptr, work.r64.ptr, work.small_dev + kR, work.norms.ptr + 2, s);
     pending = kR;
     pending_dev = work.norms.ptr + 2;
     predicted = (bnorm > 0 ? rn / bnorm : 0.0) * rel;
   }
+  if (converged) prev_corrections = std::max(1, stats.outer_iterations);
   if (converged && fused_s && !x_is_psi) {
     // W64 now holds the radial products (overlap term included) of x64, which becomes psi below
     z_valid = true;
@@ -1245,6 +1402,13 @@ extern "C" int pp_mixed_set_fallback(pp_mixed* m, pp_cayley* plain, int fallback
     PP_REQUIRE(!plain || plain->c.n == m->c.ham->n, "fallback solver size does not match");
     m->c.fallback = plain ? &plain->c : nullptr;
     m->c.fallback_iterations = fallback_iterations;
+  });
+}
+
+extern "C" int pp_mixed_set_device_mode(pp_mixed* m, int device_mode) {
+  return guarded([&] {
+    PP_REQUIRE(m, "null argument");
+    m->c.device_mode = device_mode != 0;
   });
 }
 

@@ -71,6 +71,7 @@ struct MixedWork {
   void read_small(int count, hipStream_t s);  // synchronise s, copy small[0..count) to host_small
   DeviceBuffer<double2> norms;   // device copies of step-level squared norms (|b|^2, |r0|^2, |r|^2)
   DeviceBuffer<double2> hdev;    // device copy of the inner solve's results (the Arnoldi scaling)
+  DeviceBuffer<float2> ycoef;    // inner least-squares solution of the device mode
   // out[0] = |a|^2, out[1] = |b|^2 (pinned), the same into out_dev (device)
   void norm2_pair(const double2* a, const double2* b, double2* out, double2* out_dev, hipStream_t s);
   // Two-pass classical Gram-Schmidt of w against V_0..V_{k-1} (FP64 sums): out[0..k] = first
@@ -90,6 +91,10 @@ struct MixedWork {
   // device squared norm) is given, normalising the basis vector, then z = P_j v.  The first
   // pre_iters iterations run before the host reads anything; *needed (may be NULL) receives
   // the iteration count at which the tolerance was first met.
+  // k iterations with no host read; least squares and combination on the device
+  void fgmres32_device(const std::function<void(const float2*, float2*)>& apply_A,
+                       const std::function<void(int, float2*, const double2*, float2*)>& apply_P, int k,
+                       const double2* beta2_dev, const double2* base, double2* xout, int est, hipStream_t s);
   int fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
                const std::function<void(int, float2*, const double2*, float2*)>& apply_P, const float2* rhs,
                double rtol, int max_it, double* rel_out, hipStream_t s, double beta, const double2* base,
@@ -116,6 +121,8 @@ struct MixedCayley {
   const double2* z_psi = nullptr;
   std::vector<double> scal;
   std::vector<int> pre_its;  // per correction: inner iterations run before the first host read
+  bool device_mode = false;  // after a first host-driven step: one host synchronisation per step
+  int prev_corrections = 0;  // corrections the last converged step needed
   void step(double2* psi, double dt, const double* scalings, hipStream_t s);
 };
 

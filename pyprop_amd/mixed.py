@@ -319,8 +319,12 @@ class MixedCayleyPropagator:
 
     def __init__(self, psi, base, kron: KroneckerPreconditioner, learned: Optional[LearnedPreconditioner] = None,
                  fallback=None, fallback_iterations: int = 40, krylov_basis_size: int = 20,
-                 tolerance: float = 1e-10, inner_tolerance: float = 1e-5, max_outer: int = 8, factored=None):
+                 tolerance: float = 1e-10, inner_tolerance: float = 1e-5, max_outer: int = 8, factored=None,
+                 device_mode: bool = False):
+        """device_mode: after the first converged step, solve the inner least-squares problems
+        on the device and synchronise once per step (pp_mixed_set_device_mode)."""
         self.psi = psi
+        self.device_mode = device_mode
         self.factored = factored
         self.BasePropagator = base
         self.kron = kron
@@ -348,6 +352,7 @@ class MixedCayleyPropagator:
             N.check(N.load().pp_mixed_set_fallback(h, self.fallback.h, self.fallback_iterations))
         if self.factored is not None:
             N.check(N.load().pp_mixed_set_factored(h, self.factored.h))
+        N.check(N.load().pp_mixed_set_device_mode(h, int(self.device_mode)))
 
     def SetLearned(self, learned):
         self.learned = learned

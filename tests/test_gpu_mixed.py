@@ -224,7 +224,9 @@ def test_cayley_with_kron_preconditioner_vs_dense(torch, rng):
 def test_mixed_cayley_vs_dense(torch, rng, operator):
     """Mixed-precision Cayley steps against the dense FP64 solve, with the tensor-potential
     multiply, the factored multiply plus separate overlap kernels, and the factored multiply
-    carrying the overlap term (S + i a H in one multiply)."""
+    carrying the overlap term (S + i a H in one multiply); each with every correction
+    host-driven and in device mode (from the second step on: one synchronisation per step,
+    inner least squares on the device)."""
     from pyprop_amd.factored import FactoredHamiltonian
     from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
     say("mixed-precision Cayley vs dense FP64 solve (%s)" % operator)
@@ -233,19 +235,23 @@ def test_mixed_cayley_vs_dense(torch, rng, operator):
     x0 = crandn(rng, sysm.shape)
     psi.data.copy_(torch.from_numpy(x0).to("cuda"))
     fh = None if operator == "tensor" else FactoredHamiltonian(base, overlap=operator.endswith("overlap"))
-    prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-11, factored=fh)
-    dt, t = 0.01, 40.0
-    cur = x0
-    for k in range(3):
-        sc = base.scalings(t + dt / 2)
-        sol, A, b = _dense_cayley(sysm, cur, dt, sc, True)
-        prop.AdvanceStep(t, dt)
-        st = prop.GetStats()
-        say("  step %d: %s" % (k, st))
-        assert st["rel_residual"] <= 1e-11 and st["fallback"] == 0 and st["outer"] >= 1
-        cur = psi.numpy()
-        _check_solution(cur, sol, A, b, 2e-11, 1e-8)
-        t += dt
+    for device_mode in (False, True):
+        say("  device mode %s" % device_mode)
+        psi.data.copy_(torch.from_numpy(x0).to("cuda"))
+        prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-11, factored=fh,
+                                     device_mode=device_mode)
+        dt, t = 0.01, 40.0
+        cur = x0
+        for k in range(4):
+            sc = base.scalings(t + dt / 2)
+            sol, A, b = _dense_cayley(sysm, cur, dt, sc, True)
+            prop.AdvanceStep(t, dt)
+            st = prop.GetStats()
+            say("  step %d: %s" % (k, st))
+            assert st["rel_residual"] <= 1e-11 and st["fallback"] == 0 and st["outer"] >= 1
+            cur = psi.numpy()
+            _check_solution(cur, sol, A, b, 2e-11, 1e-8)
+            t += dt
 
 
 def test_mixed_cayley_fallback_to_plain(torch, rng):
@@ -612,7 +618,8 @@ def test_mixed_factored_reuse_detects_modified_psi(torch, rng):
     x0 = crandn(rng, sysm.shape)
     psi.data.copy_(torch.from_numpy(x0).to("cuda"))
     kp = KroneckerPreconditioner(psi, sysm)
-    prop = MixedCayleyPropagator(psi, base, kp, tolerance=1e-12, factored=fh)
+    # host-driven corrections in both solvers, so that they take the same iteration path
+    prop = MixedCayleyPropagator(psi, base, kp, tolerance=1e-12, factored=fh, device_mode=False)
     dt, t = 0.01, 40.0
     for k in range(3):
         prop.AdvanceStep(t, dt)
@@ -624,7 +631,7 @@ def test_mixed_factored_reuse_detects_modified_psi(torch, rng):
     # fresh solver on a copy of the modified state
     from pyprop_amd.propagator import Wavefunction
     psi2 = Wavefunction(repr_, start)
-    prop2 = MixedCayleyPropagator(psi2, base, kp, tolerance=1e-12, factored=fh)
+    prop2 = MixedCayleyPropagator(psi2, base, kp, tolerance=1e-12, factored=fh, device_mode=False)
     prop2.AdvanceStep(t, dt)
     assert rel(got, psi2.numpy()) < 1e-12
     # and the unmodified reuse path agrees with the dense solve
