@@ -129,11 +129,11 @@ def tuned_factored(base):
         name = lambda k: "%dx%d%s" % (k[0], k[1], "p" if k[2] else "")
         tune["fp%d" % prec] = {"best": name(fh.RADIAL_SHAPES[best]),
                                "us": {name(k): round(1e3 * v, 1) for k, v in tim.items()}}
-        tune["fp%d" % prec]["angular_waves"] = fh.angular_waves(prec)
+        tune["fp%d" % prec]["angular"] = fh.ANGULAR_VARIANTS[fh.angular_variant(prec)]
         tune["fp%d" % prec]["angular_us"] = {str(k): round(1e3 * v, 1) for k, v in fh.angular_timings.items()}
-        log("factored radial tune FP%d: best %s  (%s); angular %d waves (%s)" % (
+        log("factored radial tune FP%d: best %s  (%s); angular %s (%s)" % (
             prec, tune["fp%d" % prec]["best"], ", ".join("%s %.1f us" % (k, v) for k, v in tune["fp%d" % prec]["us"].items()),
-            fh.angular_waves(prec), ", ".join("%d: %.1f us" % (k, 1e3 * v) for k, v in fh.angular_timings.items())))
+            tune["fp%d" % prec]["angular"], ", ".join("%s: %.1f us" % (k, 1e3 * v) for k, v in fh.angular_timings.items())))
     return fh, tune
 
 

@@ -413,15 +413,21 @@ int pp_fham_multiply_f32(const pp_fham* f, const double* scalings, const pp_cplx
  * (1,1), (2,2), (3,2), (3,3) per wave, and 4..6 the shapes of 1..3 with the next band step's
  * operands prefetched into registers; precision is 64 (complex128 multiply) or 32
  * (complex64).  pp_fham_tune times every radial variant (trials multiplies each, on stream)
- * and keeps the fastest, then does the same for the angular stage's wave count (16, 8 or 4
- * waves splitting its K); timings_ms (may be NULL) receives PP_FHAM_RADIAL_VARIANTS radial
- * then PP_FHAM_ANGULAR_VARIANTS angular entries. */
+ * and keeps the fastest, then does the same for the angular stage's form: variants 0..2 run
+ * one row block of 16 channels per workgroup with 16, 8 or 4 waves splitting its K, 3..4 all
+ * row blocks of a 16-position tile per workgroup (each Z element read once; up to 64
+ * channels) with 8 or 16 waves.  timings_ms (may be NULL) receives PP_FHAM_RADIAL_VARIANTS
+ * radial then PP_FHAM_ANGULAR_VARIANTS angular entries.  set_angular_waves selects among
+ * variants 0..2 by wave count; get_angular_waves returns the wave count of the current
+ * variant. */
 #define PP_FHAM_RADIAL_VARIANTS 7
-#define PP_FHAM_ANGULAR_VARIANTS 3
+#define PP_FHAM_ANGULAR_VARIANTS 5
 int pp_fham_set_radial_variant(pp_fham* f, int precision, int variant);
 int pp_fham_get_radial_variant(const pp_fham* f, int precision, int* variant);
 int pp_fham_set_angular_waves(pp_fham* f, int precision, int waves);
 int pp_fham_get_angular_waves(const pp_fham* f, int precision, int* waves);
+int pp_fham_set_angular_variant(pp_fham* f, int precision, int variant);
+int pp_fham_get_angular_variant(const pp_fham* f, int precision, int* variant);
 int pp_fham_tune(pp_fham* f, int precision, int trials, int* best, double* timings_ms, pp_stream stream);
 /* Overlap term (no reference counterpart): appends S1 (x) S2 (dense n x n, row-major, zero
  * outside the operator's band) as one more radial term whose angular part is the identity,

@@ -117,6 +117,8 @@ SIGNATURES = [
     ("pp_fham_get_radial_variant", I, [P, I, P]),
     ("pp_fham_set_angular_waves", I, [P, I, I]),
     ("pp_fham_get_angular_waves", I, [P, I, P]),
+    ("pp_fham_set_angular_variant", I, [P, I, I]),
+    ("pp_fham_get_angular_variant", I, [P, I, P]),
     ("pp_fham_tune", I, [P, I, I, P, P, P]),
     ("pp_fham_set_overlap", I, [P, P, P]),
     ("pp_piram_create", I, [I64, I, I, D, I, I, P]),

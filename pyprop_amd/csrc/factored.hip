@@ -291,8 +291,11 @@ __global__ __launch_bounds__(64) void fham_radial_reuse_kernel(int nc, int ncp, 
 constexpr int kRadTG[kFhamRadialVariants] = {1, 2, 3, 3, 2, 3, 3};
 constexpr int kRadGG[kFhamRadialVariants] = {1, 2, 2, 3, 2, 2, 3};
 constexpr bool kRadPipe[kFhamRadialVariants] = {false, false, false, false, true, true, true};
-// waves splitting K in the angular stage, per angular variant
-constexpr int kAngWaveChoices[kFhamAngularVariants] = {16, 8, 4};
+// angular variants: waves splitting K, and whether one workgroup covers all row blocks of
+// its position tile (reading each Z element once) or one row block
+constexpr int kAngWaveChoices[kFhamAngularVariants] = {16, 8, 4, 8, 16};
+constexpr bool kAngAllRows[kFhamAngularVariants] = {false, false, false, true, true};
+constexpr int kAngMaxRowBlocks = 4;  // all-row variants: at most 64 channels
 
 template <class T, int B>
 static void launch_radial(int variant, int nc, int ncp, int n, int nterms, int ngroups, int tiles, const T* Xt,
@@ -326,6 +329,7 @@ constexpr int kAngBatch = 12;
 template <class T, int W>
 __global__ __launch_bounds__(64 * W) void fham_angular_kernel(int nc, int nrb, int K, int Kreal, long long P,
                                                                       const T* __restrict__ A,
+                                                                      const int* __restrict__ tk,
                                                                       const typename MfmaT<T>::C2* __restrict__ Z,
                                                                       FScales<T> sc,
                                                                       const typename MfmaT<T>::C2* addend,
@@ -364,7 +368,7 @@ __global__ __launch_bounds__(64 * W) void fham_angular_kernel(int nc, int nrb, i
       const int ks = kb + q;
       const int k = ks * 4 + lk;
       const bool ok = ks < ks1 && k < Kreal;
-      a[q] = ok ? A[(long long)(m0 + lr) * K + k] * sc.s[k / nc] : T(0);
+      a[q] = ok ? A[((long long)(m0 >> 4) * nks + ks) * 64 + lane] * sc.s[tk[k]] : T(0);
       if (ok && col_ok) z[q] = Z[(long long)k * P + pc];
       else {
         z[q].x = 0;
@@ -415,6 +419,123 @@ __global__ __launch_bounds__(64 * W) void fham_angular_kernel(int nc, int nrb, i
       o2.x = alpha2.x * vr - alpha2.y * vi + beta2.x * v.x - beta2.y * v.y;
       o2.y = alpha2.x * vi + alpha2.y * vr + beta2.x * v.y + beta2.y * v.x;
       out2[idx] = o2;
+    }
+  }
+}
+
+// Stage 2, all row blocks per workgroup: a workgroup of W waves owns one tile of 16
+// positions and every row block (MR of them), so each Z element is read once and feeds MR
+// MFMAs; the waves split K and reduce through LDS in a tree (log2 W steps, W/2 slots).
+constexpr int kAngBatchRows = 4;
+template <class T, int W, int MR>
+__global__ __launch_bounds__(64 * W) void fham_angular_rows_kernel(int nc, int K, int Kreal, long long P,
+                                                                   const T* __restrict__ A,
+                                                                   const int* __restrict__ tk,
+                                                                   const typename MfmaT<T>::C2* __restrict__ Z,
+                                                                   FScales<T> sc,
+                                                                   const typename MfmaT<T>::C2* addend,
+                                                                   typename MfmaT<T>::C2 alpha,
+                                                                   const typename MfmaT<T>::C2* __restrict__ zs,
+                                                                   typename MfmaT<T>::C2 beta,
+                                                                   typename MfmaT<T>::C2* out,
+                                                                   typename MfmaT<T>::C2* out2,
+                                                                   typename MfmaT<T>::C2 alpha2,
+                                                                   typename MfmaT<T>::C2 beta2,
+                                                                   int* __restrict__ reset) {
+  typedef typename MfmaT<T>::V4 V4;
+  typedef typename MfmaT<T>::C2 C2;
+  extern __shared__ char ang_red_raw[];
+  V4* red = reinterpret_cast<V4*>(ang_red_raw);  // [W/2][MR][2][64]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;
+  const long long p0 = (long long)blockIdx.x * 16;
+  const int nks = K >> 2;
+  const int per = (nks + W - 1) / W;
+  const int ks0 = wave * per, ks1 = min(nks, ks0 + per);
+  const long long pc = p0 + lr;
+  const bool col_ok = pc < P;
+  V4 are[MR], aim[MR];
+#pragma unroll
+  for (int r = 0; r < MR; ++r) {
+    are[r] = V4{0, 0, 0, 0};
+    aim[r] = V4{0, 0, 0, 0};
+  }
+  for (int kb = ks0; kb < ks1; kb += kAngBatchRows) {
+    T a[MR][kAngBatchRows];
+    C2 z[kAngBatchRows];
+#pragma unroll
+    for (int q = 0; q < kAngBatchRows; ++q) {
+      const int ks = kb + q;
+      const int k = ks * 4 + lk;
+      const bool ok = ks < ks1 && k < Kreal;
+      const T f = ok ? sc.s[tk[k]] : T(0);
+#pragma unroll
+      for (int r = 0; r < MR; ++r) a[r][q] = ok ? A[((long long)r * nks + ks) * 64 + lane] * f : T(0);
+      if (ok && col_ok) z[q] = Z[(long long)k * P + pc];
+      else {
+        z[q].x = 0;
+        z[q].y = 0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kAngBatchRows; ++q)
+#pragma unroll
+      for (int r = 0; r < MR; ++r) {
+        are[r] = MfmaT<T>::mma(a[r][q], z[q].x, are[r]);
+        aim[r] = MfmaT<T>::mma(a[r][q], z[q].y, aim[r]);
+      }
+  }
+  // tree: waves [h, 2h) hand their sums to waves [0, h)
+#pragma unroll
+  for (int h = W / 2; h >= 1; h /= 2) {
+    if (wave >= h && wave < 2 * h) {
+#pragma unroll
+      for (int r = 0; r < MR; ++r) {
+        red[(((wave - h) * MR + r) * 2 + 0) * 64 + lane] = are[r];
+        red[(((wave - h) * MR + r) * 2 + 1) * 64 + lane] = aim[r];
+      }
+    }
+    __syncthreads();
+    if (wave < h) {
+#pragma unroll
+      for (int r = 0; r < MR; ++r) {
+        are[r] += red[((wave * MR + r) * 2 + 0) * 64 + lane];
+        aim[r] += red[((wave * MR + r) * 2 + 1) * 64 + lane];
+      }
+    }
+    __syncthreads();
+  }
+  if (wave != 0 || !col_ok) return;
+#pragma unroll
+  for (int rb = 0; rb < MR; ++rb) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int c = rb * 16 + (MfmaT<T>::kRowMajorOut ? lk * 4 + r : r * 4 + lk);
+      if (c >= nc) continue;
+      const long long idx = (long long)c * P + pc;
+      C2 o;
+      if (addend) o = addend[idx];
+      else {
+        o.x = 0;
+        o.y = 0;
+      }
+      const T vr = are[rb][r], vi = aim[rb][r];
+      o.x += alpha.x * vr - alpha.y * vi;
+      o.y += alpha.x * vi + alpha.y * vr;
+      C2 v;
+      v.x = 0;
+      v.y = 0;
+      if (zs) v = zs[idx];  // overlap term: its angular part is the identity
+      o.x += beta.x * v.x - beta.y * v.y;
+      o.y += beta.x * v.y + beta.y * v.x;
+      out[idx] = o;
+      if (out2) {
+        C2 o2;
+        o2.x = alpha2.x * vr - alpha2.y * vi + beta2.x * v.x - beta2.y * v.y;
+        o2.y = alpha2.x * vi + alpha2.y * vr + beta2.x * v.y + beta2.y * v.x;
+        out2[idx] = o2;
+      }
     }
   }
 }
@@ -474,10 +595,31 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   typedef typename MfmaT<T>::C2 C2;
   const C2* Zc = reinterpret_cast<const C2*>(Zb.ptr);
   const dim3 agrid((unsigned)((ablocks + 7) / 8 * 8));
-  const int waves = kAngWaveChoices[sizeof(T) == 8 ? F.ang_variant64 : F.ang_variant32];
+  const int avar = sizeof(T) == 8 ? F.ang_variant64 : F.ang_variant32;
+  const int waves = kAngWaveChoices[avar];
+  if (kAngAllRows[avar] && nrb <= kAngMaxRowBlocks) {
+    typedef typename MfmaT<T>::V4 V4;
+    const dim3 rgrid((unsigned)ceil_div(P, 16));
+    const size_t lds = (size_t)(waves / 2) * nrb * 2 * 64 * sizeof(V4);
+#define PP_ANGR(W, MR)                                                                                           \
+  if (waves == W && nrb == MR) {                                                                                 \
+    PP_HIP(hipFuncSetAttribute((const void*)fham_angular_rows_kernel<T, W, MR>,                                  \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                           \
+    hipLaunchKernelGGL((fham_angular_rows_kernel<T, W, MR>), rgrid, dim3(64 * W), lds, s, nc, F.Kp, nterms * nc, \
+                       P, Ab.ptr, F.tk.ptr, Zc, sc, o1.addend, to_c2<T>(o1.alpha),                               \
+                       with_s ? Zc + (size_t)nterms * nc * P : nullptr, to_c2<T>(o1.beta), o1.dst,              \
+                       o2 ? o2->dst : nullptr, to_c2<T>(o2 ? o2->alpha : make_double2(0, 0)),                    \
+                       to_c2<T>(o2 ? o2->beta : make_double2(0, 0)), differs);                                   \
+  }
+    PP_ANGR(8, 1) PP_ANGR(8, 2) PP_ANGR(8, 3) PP_ANGR(8, 4)
+    PP_ANGR(16, 1) PP_ANGR(16, 2) PP_ANGR(16, 3) PP_ANGR(16, 4)
+#undef PP_ANGR
+    PP_HIP(hipGetLastError());
+    return;
+  }
 #define PP_ANG(W)                                                                                                    \
   hipLaunchKernelGGL((fham_angular_kernel<T, W>), agrid, dim3(64 * W), 0, s, nc, nrb, F.Kp, nterms * nc, P, Ab.ptr, \
-                     Zc, sc, o1.addend, to_c2<T>(o1.alpha), with_s ? Zc + (size_t)nterms * nc * P : nullptr,       \
+                     F.tk.ptr, Zc, sc, o1.addend, to_c2<T>(o1.alpha), with_s ? Zc + (size_t)nterms * nc * P : nullptr,       \
                      to_c2<T>(o1.beta), o1.dst, o2 ? o2->dst : nullptr,                                              \
                      to_c2<T>(o2 ? o2->alpha : make_double2(0, 0)), to_c2<T>(o2 ? o2->beta : make_double2(0, 0)),   \
                      differs)
@@ -585,16 +727,25 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
     F.Kp = (nterms * nchan + 3) / 4 * 4;  // angular GEMM K, zero-padded to the MFMA K step
     F.term_pot.assign(term_potential, term_potential + nterms);
     const int Mp = (nchan + 15) / 16 * 16;
-    // A[c'][t*nc + c] = A_t[c'][c], rows padded to a multiple of 16 with zeros
+    // A(c', t*nc + c) = A_t[c'][c], rows padded to a multiple of 16 with zeros, stored in
+    // MFMA operand order: [row block][K step][lane], lane l holding A(16 rb + l % 16,
+    // 4 ks + l / 16), so that a wave reads its 64 operands as one coalesced load
+    const int nks = F.Kp / 4;
     std::vector<double> a((size_t)Mp * F.Kp, 0.0);
     for (int t = 0; t < nterms; ++t)
       for (int r = 0; r < nchan; ++r)
-        for (int c = 0; c < nchan; ++c) a[(size_t)r * F.Kp + (size_t)t * nchan + c] = A[((size_t)t * nchan + r) * nchan + c];
+        for (int c = 0; c < nchan; ++c) {
+          const int k = t * nchan + c;
+          a[((size_t)(r / 16) * nks + k / 4) * 64 + (k % 4) * 16 + r % 16] = A[((size_t)t * nchan + r) * nchan + c];
+        }
+    std::vector<int> tk(F.Kp);
+    for (int k = 0; k < F.Kp; ++k) tk[k] = std::min(k / nchan, nterms - 1);  // term of column k
     std::vector<double> bt = band_tiles(nterms, n, bw, dia);
     const size_t nbt = bt.size();
     try {
       hipStream_t s = nullptr;
       F.A64.upload(a.data(), a.size(), s);
+      F.tk.upload(tk.data(), tk.size(), s);
       std::vector<float> af(a.begin(), a.end());
       F.A32.upload(af.data(), af.size(), s);
       F.bt64.upload(bt.data(), nbt, s);
@@ -651,9 +802,24 @@ extern "C" int pp_fham_set_angular_waves(pp_fham* f, int precision, int waves) {
     PP_REQUIRE(f && (precision == 64 || precision == 32), "precision must be 64 or 32");
     int v = -1;
     for (int i = 0; i < kFhamAngularVariants; ++i)
-      if (kAngWaveChoices[i] == waves) v = i;
+      if (!kAngAllRows[i] && kAngWaveChoices[i] == waves) v = i;
     PP_REQUIRE(v >= 0, "angular wave count must be 16, 8 or 4");
     (precision == 64 ? f->f.ang_variant64 : f->f.ang_variant32) = v;
+  });
+}
+
+extern "C" int pp_fham_set_angular_variant(pp_fham* f, int precision, int variant) {
+  return guarded([&] {
+    PP_REQUIRE(f && (precision == 64 || precision == 32), "precision must be 64 or 32");
+    PP_REQUIRE(variant >= 0 && variant < kFhamAngularVariants, "angular variant out of range");
+    (precision == 64 ? f->f.ang_variant64 : f->f.ang_variant32) = variant;
+  });
+}
+
+extern "C" int pp_fham_get_angular_variant(const pp_fham* f, int precision, int* variant) {
+  return guarded([&] {
+    PP_REQUIRE(f && variant && (precision == 64 || precision == 32), "invalid argument");
+    *variant = precision == 64 ? f->f.ang_variant64 : f->f.ang_variant32;
   });
 }
 

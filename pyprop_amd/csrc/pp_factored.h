@@ -11,7 +11,7 @@ namespace pp {
 
 constexpr int kMaxFTerms = 32;
 constexpr int kFhamRadialVariants = 7;  // radial-stage kernel shapes (terms x channel groups per wave, prefetch)
-constexpr int kFhamAngularVariants = 3;  // angular-stage wave counts (16, 8, 4 waves split K)
+constexpr int kFhamAngularVariants = 5;  // angular-stage forms (waves splitting K, row blocks per workgroup)
 
 struct FHamOut {
   double2* dst = nullptr;
@@ -22,7 +22,8 @@ struct FHamOut {
 struct FHam {
   int nc = 0, n = 0, nterms = 0, bw = 0, npot = 0, Kp = 0;  // Kp: angular GEMM K = nterms * nc rounded up to 4
   std::vector<int> term_pot;          // potential (time scaling) index of each term
-  DeviceBuffer<double> A64, bt64;     // A: [ceil16(nc)][Kp] (row c, column t*nc + c);
+  DeviceBuffer<int> tk;               // term index of each angular GEMM column k (k / nc)
+  DeviceBuffer<double> A64, bt64;     // A: MFMA operand order [ceil16(nc)/16][Kp/4][64] of A(c, t*nc + c');
   DeviceBuffer<float> A32, bt32;      // bt: band tiles in MFMA lane order [T][n][w][tiles][KS][64]
   mutable DeviceBuffer<double> W64;   // radial products Z [T][nc][n][n] complex
   mutable DeviceBuffer<float> W32;

@@ -133,13 +133,24 @@ class FactoredHamiltonian:
         stage's wave count; returns (best radial variant, {shape: ms per multiply}) and leaves
         the angular timings in self.angular_timings."""
         best = C.c_int()
-        tim = np.zeros(len(self.RADIAL_SHAPES) + len(self.ANGULAR_WAVES))
+        tim = np.zeros(len(self.RADIAL_SHAPES) + len(self.ANGULAR_VARIANTS))
         N.check(N.load().pp_fham_tune(self.h, precision, trials, C.byref(best), D.hptr(tim), D.stream_handle()))
         nr = len(self.RADIAL_SHAPES)
-        self.angular_timings = dict(zip(self.ANGULAR_WAVES, tim[nr:].tolist()))
+        self.angular_timings = dict(zip(self.ANGULAR_VARIANTS, tim[nr:].tolist()))
         return best.value, dict(zip(self.RADIAL_SHAPES, tim[:nr].tolist()))
 
-    ANGULAR_WAVES = (16, 8, 4)  # waves splitting K in the angular stage, per angular variant
+    ANGULAR_WAVES = (16, 8, 4)  # waves splitting K in the angular stage, one row block per workgroup
+    # every angular variant: "<waves>" (one row block of 16 channels per workgroup) or
+    # "rows<waves>" (all row blocks of a position tile per workgroup, each Z element read once)
+    ANGULAR_VARIANTS = ("16", "8", "4", "rows8", "rows16")
+
+    def set_angular_variant(self, precision: int, variant: int):
+        N.check(N.load().pp_fham_set_angular_variant(self.h, precision, variant))
+
+    def angular_variant(self, precision: int) -> int:
+        v = C.c_int()
+        N.check(N.load().pp_fham_get_angular_variant(self.h, precision, C.byref(v)))
+        return v.value
 
     def set_angular_waves(self, precision: int, waves: int):
         N.check(N.load().pp_fham_set_angular_waves(self.h, precision, waves))

@@ -514,18 +514,22 @@ def test_factored_radial_variants_vs_dense(torch, rng):
         fh.set_angular_waves(64, waves)
         fh.set_angular_waves(32, waves)
         assert fh.angular_waves(64) == waves and fh.angular_waves(32) == waves
+    for v, name in enumerate(fh.ANGULAR_VARIANTS):
+        fh.set_angular_variant(64, v)
+        fh.set_angular_variant(32, v)
+        assert fh.angular_variant(64) == v and fh.angular_variant(32) == v
         d = torch.empty_like(src)
         fh.Multiply(src, d, t)
         d32 = torch.empty_like(s32)
         fh.Multiply(s32, d32, t)
         e64 = rel(d.cpu().numpy().ravel(), ref)
         e32 = rel(d32.cpu().numpy().ravel(), ref)
-        say("  angular %d waves: FP64 %.2e  FP32 %.2e" % (waves, e64, e32))
+        say("  angular %s: FP64 %.2e  FP32 %.2e" % (name, e64, e32))
         assert e64 < 1e-12 and e32 < 2e-6
     best, tim = fh.tune(64, trials=2)
     assert 0 <= best < len(fh.RADIAL_SHAPES) and fh.radial_variant(64) == best
     assert all(ms > 0 for ms in tim.values()) and all(ms > 0 for ms in fh.angular_timings.values())
-    assert fh.angular_waves(64) in fh.ANGULAR_WAVES
+    assert 0 <= fh.angular_variant(64) < len(fh.ANGULAR_VARIANTS)
 
 
 def test_factored_multiply_full_size_vs_tensor_potentials(torch, rng):
@@ -557,6 +561,7 @@ def test_factored_multiply_full_size_vs_tensor_potentials(torch, rng):
         best, tim = fh.tune(prec, trials=3)
         say("  tune %d: best %s, %s" % (prec, fh.RADIAL_SHAPES[best],
                                         ", ".join("%s %.1f us" % (k, 1e3 * v) for k, v in tim.items())))
+    say("  angular: %s" % ", ".join("%s %.1f us" % (k, 1e3 * v) for k, v in fh.angular_timings.items()))
     for v in range(len(fh.RADIAL_SHAPES)):
         fh.set_radial_variant(64, v)
         fh.set_radial_variant(32, v)
@@ -565,6 +570,16 @@ def test_factored_multiply_full_size_vs_tensor_potentials(torch, rng):
         e64 = (torch.linalg.norm(got - ref) / torch.linalg.norm(ref)).item()
         e32 = (torch.linalg.norm(g32.to(torch.complex128) - ref) / torch.linalg.norm(ref)).item()
         assert e64 < 1e-12 and e32 < 2e-6, (v, e64, e32)
+    # every angular form at 42 channels (3 row blocks)
+    for v, name in enumerate(fh.ANGULAR_VARIANTS):
+        fh.set_angular_variant(64, v)
+        fh.set_angular_variant(32, v)
+        fh.Multiply(x, got, t)
+        fh.Multiply(x.to(torch.complex64), g32, t)
+        e64 = (torch.linalg.norm(got - ref) / torch.linalg.norm(ref)).item()
+        e32 = (torch.linalg.norm(g32.to(torch.complex128) - ref) / torch.linalg.norm(ref)).item()
+        say("  angular %s: FP64 %.2e  FP32 %.2e" % (name, e64, e32))
+        assert e64 < 1e-12 and e32 < 2e-6, (name, e64, e32)
 
 
 def test_mixed_cayley_factored_full_size_vs_plain(torch):
