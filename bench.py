@@ -226,7 +226,7 @@ def host_threads():
     return max(1, n)
 
 
-def cpu_baseline(sysm, n_ops_per_step, sample_frac=0.1, threads=1):
+def cpu_baseline(sysm, n_ops_per_step, sample_frac=0.1, threads=1, reps=3):
     """The oracle (sequential C++ restatement of the reference loops) timed on the host:
     the tensor-potential multiply on `sample_frac` of every potential's angular pairs and
     the block-preconditioner solve of two channels, scaled to one Cayley step (n_ops
@@ -234,7 +234,7 @@ def cpu_baseline(sysm, n_ops_per_step, sample_frac=0.1, threads=1):
     every thread runs the same kind of sample on its own slice of the angular pairs and its
     own two channels at the same time (the calls release the GIL), the reference's
     one-process-per-core model: the step time is the wall time per sample unit divided by
-    the number of threads."""
+    the number of threads.  Each sample runs `reps` times; the median wall time counts."""
     import threading
     import numpy as np
     import oracle
@@ -265,7 +265,7 @@ def cpu_baseline(sysm, n_ops_per_step, sample_frac=0.1, threads=1):
             dst = np.zeros(sysm.shape, complex)
             jobs.append(lambda V=V, pairs0=pairs0, dst=dst, p=p: oracle.tpmul(
                 p.storage_ids, V, 1.0, psi, dst, [pairs0, p.geometries[1].pairs, None]))
-        wall = run_parallel(jobs)
+        wall = float(np.median([run_parallel(jobs) for _ in range(reps)]))
         t_mul += wall * P0 / (k * threads)
     # preconditioner: each thread factors (not timed) and solves (timed) two channels
     b1 = sysm.bases[1]
@@ -278,15 +278,14 @@ def cpu_baseline(sysm, n_ops_per_step, sample_frac=0.1, threads=1):
         s.set_preconditioner(S1, S1, b1.order - 1, P, g[1].pairs, g[2].pairs, 0.005j)
         systems.append(s)
         xs.append(psi[:2].copy())
-    reps = 3
     wall = run_parallel([lambda s=s, x=x: [s.precond_solve(x) for _ in range(reps)] for s, x in zip(systems, xs)])
     t_pc = wall / reps * nc / (2 * threads)
     step = (n_ops_per_step + 1) * (t_mul + t_pc)
     return dict(value=1.0 / step, unit="steps/s", cores=threads, kind="port",
-                sample="oracle multiply on %d%% of each potential's angular pairs per thread (full multiply %.3fs) "
-                       "and 2-channel preconditioner solves per thread (full solve %.3fs), %d thread(s) at once, "
-                       "scaled to one step of %.1f operator applications + right-hand side"
-                       % (int(sample_frac * 100), t_mul, t_pc, threads, n_ops_per_step))
+                sample="oracle multiply on %d%% of each potential's angular pairs per thread, median of %d runs "
+                       "(full multiply %.3fs), and 2-channel preconditioner solves per thread, %d runs (full solve "
+                       "%.3fs), %d thread(s) at once, scaled to one step of %.1f operator applications + right-hand "
+                       "side" % (int(sample_frac * 100), reps, t_mul, reps, t_pc, threads, n_ops_per_step))
 
 
 def run_steps(torch, prop, psi, t0, dt, warmup, steps, stats=None):
@@ -536,7 +535,7 @@ def main():
     cpu1 = None
     if not a.no_cpu_baseline:
         t0 = time.time()
-        cpu1 = cpu_baseline(sysm, ops_plain)
+        cpu1 = cpu_baseline(sysm, ops_plain, sample_frac=0.3)
         log("cpu baseline, 1 thread: %.4f steps/s (%.1fs)" % (cpu1["value"], time.time() - t0))
         nt = host_threads()
         t0 = time.time()
