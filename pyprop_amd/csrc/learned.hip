@@ -139,10 +139,14 @@ int gemm_bf16(int M, int N, int K, const uint16_t* a, long long sam, long long s
   return z;
 }
 
+// Split-K factor: enough workgroups for the chip (about 256), at most 32 splits and at
+// least 128 of K per split.  Skinny products (the inference of a few dozen channel rows)
+// split most.
+constexpr int kOutSplitsMax = 4;  // split-K of the output layer (its partials are rows x in)
 static int pick_splits(int M, int N, int K) {
   const long long tiles = ceil_div(M, GBM) * ceil_div(N, GBN);
   int sp = (int)std::max(1LL, 256 / std::max(1LL, tiles));
-  sp = std::min(sp, 16);
+  sp = std::min(sp, 32);
   sp = std::min(sp, std::max(1, K / 128));
   return sp;
 }
@@ -326,7 +330,7 @@ void Learned::ensure(int rows) {
   y.resize(ri);
   gf.resize(ri);
   dzf.resize(rh);
-  part.resize(std::max((size_t)splits_max * rh, ri));
+  part.resize(std::max((size_t)splits_max * rh, (size_t)kOutSplitsMax * ri));
   loss_part.resize(1024);
 }
 
@@ -351,7 +355,8 @@ void Learned::forward(int rows, const int* chan, hipStream_t s) {
   hipLaunchKernelGGL(bias_act_kernel, dim3(grid_for_n(rh)), dim3(256), 0, s, rows, hidden, part.ptr, sp,
                      params.ptr + o_b2, (const float*)nullptr, (const int*)nullptr, 0, z2.ptr, h2h.ptr, (float*)nullptr,
                      1);
-  sp = gemm_bf16(rows, in, hidden, h2h.ptr, hidden, 1, w3h.ptr, 1, hidden, part.ptr, 1, s);
+  sp = gemm_bf16(rows, in, hidden, h2h.ptr, hidden, 1, w3h.ptr, 1, hidden, part.ptr,
+                 std::min(kOutSplitsMax, pick_splits(rows, in, hidden)), s);
   hipLaunchKernelGGL(bias_act_kernel, dim3(grid_for_n(ri)), dim3(256), 0, s, rows, in, part.ptr, sp,
                      params.ptr + o_b3, (const float*)nullptr, (const int*)nullptr, 0, (float*)nullptr,
                      (uint16_t*)nullptr, y.ptr, 0);

@@ -31,7 +31,7 @@ struct Learned {
   DeviceBuffer<int> iota;
   DeviceBuffer<double> nrm;          // per-block norms, [nchan] = global norm
   DeviceBuffer<double2> tmp64;
-  int splits_max = 16;
+  int splits_max = 32;
   void ensure(int rows);
   void refresh_bf16(hipStream_t s);
   // y (rows x in, FP32) = network(x) for BF16 inputs already in xh; keeps activations
