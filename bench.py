@@ -331,7 +331,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md)", file=sys.stderr)
+              "(the helium problem would shard by angular channel, one slice per GPU)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
