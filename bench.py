@@ -330,7 +330,7 @@ def main():
                     help="only run this many mixed-precision steps with the Kronecker preconditioner (for rocprof)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+        print("bench.py runs on one GPU only: pyprop_amd has no multi-GPU path "
               "(the helium problem would shard by angular channel, one slice per GPU)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
