@@ -555,12 +555,79 @@ static typename MfmaT<T>::C2 to_c2(double2 v) {
   return c;
 }
 
+// Radial products of the separable terms R = a (x) b: Z_c = a X_c b^T for one (channel,
+// term) per workgroup, with X_c and the r2 pass held in LDS.  a, b: band rows [n][2 bw + 1]
+// (a(i, i - bw + jj)).  Term j writes product slot first + j.  Skipped, like the band
+// kernels, when the input equals the one whose products are held.
+template <class T>
+__global__ __launch_bounds__(1024) void fham_separable_kernel(int nc, int n, int bw, int first,
+                                                              const T* __restrict__ SA, const T* __restrict__ SB,
+                                                              const typename MfmaT<T>::C2* __restrict__ X,
+                                                              typename MfmaT<T>::C2* __restrict__ Z,
+                                                              const int* __restrict__ differs) {
+  typedef typename MfmaT<T>::C2 C2;
+  if (differs && *differs == 0) return;
+  extern __shared__ char sep_raw[];
+  const int w = 2 * bw + 1, m = n * n;
+  C2* Xs = reinterpret_cast<C2*>(sep_raw);
+  C2* Ts = Xs + m;
+  T* as = reinterpret_cast<T*>(Ts + m);
+  T* bs = as + n * w;
+  const int c = blockIdx.x, j = blockIdx.y;
+  const C2* x = X + (size_t)c * m;
+  for (int e = threadIdx.x; e < m; e += blockDim.x) Xs[e] = x[e];
+  for (int e = threadIdx.x; e < n * w; e += blockDim.x) {
+    as[e] = SA[(size_t)j * n * w + e];
+    bs[e] = SB[(size_t)j * n * w + e];
+  }
+  __syncthreads();
+  // thread -> (row i1 = threadIdx.x / 64 + 16 k, column i2 = threadIdx.x % 64): no division
+  const int col = threadIdx.x & 63, row0 = threadIdx.x >> 6, rstep = blockDim.x >> 6;
+  if (col < n) {
+    for (int i1 = row0; i1 < n; i1 += rstep) {  // T = X b^T (along r2)
+      T re = 0, im = 0;
+      for (int d = 0; d < w; ++d) {
+        const int j2 = col - bw + d;
+        if (j2 < 0 || j2 >= n) continue;
+        const T f = bs[col * w + d];
+        const C2 v = Xs[i1 * n + j2];
+        re = fma(f, v.x, re);
+        im = fma(f, v.y, im);
+      }
+      C2 o;
+      o.x = re;
+      o.y = im;
+      Ts[i1 * n + col] = o;
+    }
+  }
+  __syncthreads();
+  C2* z = Z + ((size_t)(first + j) * nc + c) * m;
+  if (col < n) {
+    for (int i1 = row0; i1 < n; i1 += rstep) {  // Z = a T (along r1)
+      T re = 0, im = 0;
+      for (int jj = 0; jj < w; ++jj) {
+        const int j1 = i1 - bw + jj;
+        if (j1 < 0 || j1 >= n) continue;
+        const T f = as[i1 * w + jj];
+        const C2 v = Ts[j1 * n + col];
+        re = fma(f, v.x, re);
+        im = fma(f, v.y, im);
+      }
+      C2 o;
+      o.x = re;
+      o.y = im;
+      z[i1 * n + col] = o;
+    }
+  }
+}
+
 // o1.dst = o1.addend + o1.alpha Y + o1.beta Zs, o2 likewise without addend (may be NULL);
 // Y = sum_t f_t A_t Z_t, Zs = the overlap product.  ref / differs: reuse of the radial
 // products when src equals ref (FP64 only).
 template <class T>
 static void fham_run(const FHam& F, const double* scalings, const typename MfmaT<T>::C2* src, const FOut<T>& o1,
-                     const FOut<T>* o2, const DeviceBuffer<T>& Ab, const DeviceBuffer<T>& Db, DeviceBuffer<T>& Zb,
+                     const FOut<T>* o2, const DeviceBuffer<T>& Ab, const DeviceBuffer<T>& Db,
+                     const DeviceBuffer<T>& SAb, const DeviceBuffer<T>& SBb, DeviceBuffer<T>& Zb,
                      DeviceBuffer<T>& Xb, const T* ref, int* differs, hipStream_t s) {
   const int nc = F.nc, n = F.n, nterms = F.nterms;
   const long long P = (long long)n * n;
@@ -578,15 +645,31 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
                      dim3(256), 0, s, nc, ncp, n, reinterpret_cast<const T*>(src), Xb.ptr, ref, differs);
   const int tiles = (n + 15) / 16;
   const int variant = sizeof(T) == 8 ? F.rad_variant64 : F.rad_variant32;
-  switch (F.bw) {
+  // band-tile terms first; with separable terms those (and the overlap term) follow
+  const int nrad_band = F.sep_enabled ? F.nband : nrad;
+  if (nrad_band > 0) {
+    switch (F.bw) {
 #define PP_RAD(B)                                                                                                    \
   case B:                                                                                                            \
-    launch_radial<T, B>(variant, nc, ncp, n, nrad, ngroups, tiles, Xb.ptr, Db.ptr, Zb.ptr, differs, s);             \
+    launch_radial<T, B>(variant, nc, ncp, n, nrad_band, ngroups, tiles, Xb.ptr, Db.ptr, Zb.ptr, differs, s);        \
     break;
-    PP_RAD(1) PP_RAD(2) PP_RAD(3) PP_RAD(4) PP_RAD(5) PP_RAD(6) PP_RAD(7) PP_RAD(8)
+      PP_RAD(1) PP_RAD(2) PP_RAD(3) PP_RAD(4) PP_RAD(5) PP_RAD(6) PP_RAD(7) PP_RAD(8)
 #undef PP_RAD
-    default:
-      throw Error(PP_ERR_INVALID, "factored multiply supports band half-widths 1..8");
+      default:
+        throw Error(PP_ERR_INVALID, "factored multiply supports band half-widths 1..8");
+    }
+  }
+  if (F.sep_enabled) {
+    const int nsep_run = (nterms - F.nband) + (with_s ? 1 : 0);
+    if (nsep_run > 0) {
+      typedef typename MfmaT<T>::C2 C2s;
+      const size_t lds = 2 * (size_t)P * sizeof(C2s) + 2 * (size_t)n * (2 * F.bw + 1) * sizeof(T);
+      PP_HIP(hipFuncSetAttribute((const void*)fham_separable_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds));
+      hipLaunchKernelGGL(fham_separable_kernel<T>, dim3(nc, nsep_run), dim3(1024), lds, s, nc, n, F.bw, F.nband,
+                         SAb.ptr, SBb.ptr, src, reinterpret_cast<C2s*>(Zb.ptr), differs);
+      PP_HIP(hipGetLastError());
+    }
   }
   FScales<T> sc;
   for (int t = 0; t < kMaxFTerms; ++t) sc.s[t] = t < nterms ? (T)scalings[F.term_pot[t]] : T(0);
@@ -666,7 +749,7 @@ void FHam::apply64_ex(const double* scalings, const double2* src, const FHamOut&
     a2.alpha = o2->alpha;
     a2.beta = o2->beta;
   }
-  fham_run<double>(*this, scalings, src, a1, o2 ? &a2 : nullptr, A64, bt64, W64, X64,
+  fham_run<double>(*this, scalings, src, a1, o2 ? &a2 : nullptr, A64, bt64, sa64, sb64, W64, X64,
                    reinterpret_cast<const double*>(reuse_ref), reuse_ref ? reuse_flag.ptr : nullptr, s);
   // when the products were reused they cover at least nrad terms; when recomputed, exactly nrad
   w64_terms = nrad;
@@ -679,7 +762,7 @@ void FHam::apply32(const double* scalings, const float2* src, float2* dst, const
   o.addend = addend;
   o.alpha = alpha;
   o.beta = beta;
-  fham_run<float>(*this, scalings, src, o, nullptr, A32, bt32, W32, X32, nullptr, nullptr, s);
+  fham_run<float>(*this, scalings, src, o, nullptr, A32, bt32, sa32, sb32, W32, X32, nullptr, nullptr, s);
 }
 
 }  // namespace pp
@@ -708,6 +791,44 @@ static std::vector<double> band_tiles(int nterms, int n, int bw, const double* d
   return bt;
 }
 
+// A term whose band D[i1][jj][d][i2] equals a[i1][jj] b[i2][d] (entries within 1e-13 of the
+// largest) is separable; a, b are read from the row and column through its largest entry.
+static bool separable_factors(const double* D, int n, int bw, std::vector<double>& a, std::vector<double>& b) {
+  const int w = 2 * bw + 1;
+  const size_t cnt = (size_t)n * w * w * n;
+  size_t best = 0;
+  double mx = 0.0;
+  for (size_t e = 0; e < cnt; ++e)
+    if (std::abs(D[e]) > mx) {
+      mx = std::abs(D[e]);
+      best = e;
+    }
+  a.assign((size_t)n * w, 0.0);
+  b.assign((size_t)n * w, 0.0);
+  if (mx == 0.0) return true;
+  const int i2s = (int)(best % n), ds = (int)((best / n) % w), jjs = (int)((best / ((size_t)n * w)) % w);
+  const int i1s = (int)(best / ((size_t)n * w * w));
+  const double piv = D[best];
+  for (int i1 = 0; i1 < n; ++i1)
+    for (int jj = 0; jj < w; ++jj) a[(size_t)i1 * w + jj] = D[(((size_t)i1 * w + jj) * w + ds) * n + i2s];
+  for (int d = 0; d < w; ++d)
+    for (int i2 = 0; i2 < n; ++i2) b[(size_t)i2 * w + d] = D[(((size_t)i1s * w + jjs) * w + d) * n + i2] / piv;
+  for (int i1 = 0; i1 < n; ++i1)
+    for (int jj = 0; jj < w; ++jj)
+      for (int d = 0; d < w; ++d)
+        for (int i2 = 0; i2 < n; ++i2) {
+          const double v = D[(((size_t)i1 * w + jj) * w + d) * n + i2];
+          if (std::abs(v - a[(size_t)i1 * w + jj] * b[(size_t)i2 * w + d]) > 1e-13 * mx) return false;
+        }
+  return true;
+}
+
+// the separable kernel holds X_c, one pass and both factors in LDS, and maps one thread to
+// each of at most 64 columns
+static bool separable_fits(int n) {
+  return n <= 64 && 2 * (size_t)n * n * sizeof(double2) + 2 * (size_t)n * 17 * sizeof(double) <= 160 * 1024;
+}
+
 extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double* A, const int* term_potential,
                               const double* dia, int npotentials, pp_fham** out) {
   return guarded([&] {
@@ -725,7 +846,29 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
     F.bw = bw;
     F.npot = npotentials;
     F.Kp = (nterms * nchan + 3) / 4 * 4;  // angular GEMM K, zero-padded to the MFMA K step
-    F.term_pot.assign(term_potential, term_potential + nterms);
+    // separable terms go last, in their own kernel; the others keep band tiles
+    const int w = 2 * bw + 1;
+    const size_t dstride = (size_t)n * w * w * n;
+    std::vector<std::vector<double>> fa(nterms), fb(nterms);
+    std::vector<int> sep(nterms, 0), order;
+    const bool sep_ok = separable_fits(n);
+    for (int t = 0; t < nterms; ++t) sep[t] = sep_ok && separable_factors(dia + t * dstride, n, bw, fa[t], fb[t]);
+    for (int t = 0; t < nterms; ++t)
+      if (!sep[t]) order.push_back(t);
+    F.nband = (int)order.size();
+    for (int t = 0; t < nterms; ++t)
+      if (sep[t]) order.push_back(t);
+    F.sep_enabled = sep_ok;
+    F.term_pot.resize(nterms);
+    for (int k = 0; k < nterms; ++k) F.term_pot[k] = term_potential[order[k]];
+    std::vector<double> dband((size_t)F.nband * dstride);
+    for (int k = 0; k < F.nband; ++k) std::copy(dia + order[k] * dstride, dia + (order[k] + 1) * dstride,
+                                                dband.begin() + (size_t)k * dstride);
+    std::vector<double> sav, sbv;
+    for (int k = F.nband; k < nterms; ++k) {
+      sav.insert(sav.end(), fa[order[k]].begin(), fa[order[k]].end());
+      sbv.insert(sbv.end(), fb[order[k]].begin(), fb[order[k]].end());
+    }
     const int Mp = (nchan + 15) / 16 * 16;
     // A(c', t*nc + c) = A_t[c'][c], rows padded to a multiple of 16 with zeros, stored in
     // MFMA operand order: [row block][K step][lane], lane l holding A(16 rb + l % 16,
@@ -736,11 +879,12 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
       for (int r = 0; r < nchan; ++r)
         for (int c = 0; c < nchan; ++c) {
           const int k = t * nchan + c;
-          a[((size_t)(r / 16) * nks + k / 4) * 64 + (k % 4) * 16 + r % 16] = A[((size_t)t * nchan + r) * nchan + c];
+          a[((size_t)(r / 16) * nks + k / 4) * 64 + (k % 4) * 16 + r % 16] =
+              A[((size_t)order[t] * nchan + r) * nchan + c];
         }
     std::vector<int> tk(F.Kp);
     for (int k = 0; k < F.Kp; ++k) tk[k] = std::min(k / nchan, nterms - 1);  // term of column k
-    std::vector<double> bt = band_tiles(nterms, n, bw, dia);
+    std::vector<double> bt = band_tiles(F.nband, n, bw, dband.data());
     const size_t nbt = bt.size();
     try {
       hipStream_t s = nullptr;
@@ -751,6 +895,11 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
       F.bt64.upload(bt.data(), nbt, s);
       std::vector<float> bf(bt.begin(), bt.end());
       F.bt32.upload(bf.data(), nbt, s);
+      F.sa64.upload(sav.data(), sav.size(), s);
+      F.sb64.upload(sbv.data(), sbv.size(), s);
+      std::vector<float> saf(sav.begin(), sav.end()), sbf(sbv.begin(), sbv.end());
+      F.sa32.upload(saf.data(), saf.size(), s);
+      F.sb32.upload(sbf.data(), sbf.size(), s);
       PP_HIP(hipStreamSynchronize(s));
     } catch (...) {
       delete p;
@@ -914,6 +1063,31 @@ extern "C" int pp_fham_set_overlap(pp_fham* f, const double* S1, const double* S
       for (int j = 0; j < n; ++j)
         PP_REQUIRE(std::abs(i - j) <= bw || (S1[(size_t)i * n + j] == 0.0 && S2[(size_t)i * n + j] == 0.0),
                    "overlap matrix is wider than the operator's band");
+    if (F.sep_enabled) {
+      // S1 (x) S2 as separable factors after those of the terms: band rows of S1 and S2
+      std::vector<double> a1((size_t)n * w, 0.0), b2((size_t)n * w, 0.0);
+      for (int i = 0; i < n; ++i)
+        for (int jj = 0; jj < w; ++jj) {
+          const int j = i - bw + jj;
+          if (j < 0 || j >= n) continue;
+          a1[(size_t)i * w + jj] = S1[(size_t)i * n + j];
+          b2[(size_t)i * w + jj] = S2[(size_t)i * n + j];
+        }
+      const size_t nsep = (size_t)(F.nterms - F.nband) * n * w;
+      auto append = [&](DeviceBuffer<double>& d64, DeviceBuffer<float>& d32, const std::vector<double>& v) {
+        std::vector<double> h(nsep + v.size());
+        if (nsep) PP_HIP(hipMemcpy(h.data(), d64.ptr, nsep * sizeof(double), hipMemcpyDeviceToHost));
+        std::copy(v.begin(), v.end(), h.begin() + nsep);
+        d64.upload(h.data(), h.size(), nullptr);
+        std::vector<float> hf(h.begin(), h.end());
+        d32.upload(hf.data(), hf.size(), nullptr);
+        PP_HIP(hipStreamSynchronize(nullptr));
+      };
+      append(F.sa64, F.sa32, a1);
+      append(F.sb64, F.sb32, b2);
+      F.has_overlap = true;
+      return;
+    }
     // dia[i1][jj][d][i2] = S1[i1][i1 - bw + jj] S2[i2][i2 - bw + d]
     std::vector<double> dia((size_t)n * w * w * n, 0.0);
     for (int i1 = 0; i1 < n; ++i1)

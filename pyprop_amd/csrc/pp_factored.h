@@ -32,6 +32,13 @@ struct FHam {
   // Optional overlap term: radial term index nterms holds S1 (x) S2 (identity angular part),
   // added with its own coefficient beta outside the angular contraction.
   bool has_overlap = false;
+  // Separable terms R_t = a (x) b run as two 1-D band passes (fham_separable_kernel): terms
+  // [0, nband) use band tiles, terms [nband, nterms) and the overlap term the factors
+  // sa/sb (band rows [n][2 bw + 1] per term, the overlap's last).
+  bool sep_enabled = false;
+  int nband = 0;
+  DeviceBuffer<double> sa64, sb64;
+  DeviceBuffer<float> sa32, sb32;
   int rad_variant64 = 0, rad_variant32 = 0;  // radial kernel shape per precision
   int ang_variant64 = 0, ang_variant32 = 0;  // angular stage wave count per precision
   mutable unsigned long long gen64 = 0;      // FP64 multiplies issued
