@@ -157,12 +157,14 @@ def time_factored(torch, fh, psi, dtype, reps=50):
 
 def factored_flops(fh, nc, n):
     """(issued MFMA flops incl. padding, useful flops) of one factored multiply:
-    radial stage, one 16x16x4 MFMA per (term, i1, jj, i2 tile, channel group, K step);
-    angular stage, two MFMAs (re, im) per (16 channels, 16 positions, K step of 4)."""
+    radial stage, one 16x16x4 MFMA per (band-tile term, i1, jj, i2 tile, channel group, K
+    step) -- the separable terms run as two 1-D band passes on the vector units and are not
+    counted as issued MFMA work; angular stage, two MFMAs (re, im) per (16 channels, 16
+    positions, K step of 4)."""
     T, bw = fh.nterms, fh.bw
     pairs1 = sum(min(n, i + bw + 1) - max(0, i - bw) for i in range(n))
     ks = (16 + 2 * bw + 3) // 4
-    rad_issued = T * pairs1 * ks * math.ceil(n / 16) * math.ceil(nc / 8) * 2048
+    rad_issued = fh.band_terms * pairs1 * ks * math.ceil(n / 16) * math.ceil(nc / 8) * 2048
     kp = (T * nc + 3) // 4 * 4
     ang_issued = math.ceil(n * n / 16) * math.ceil(nc / 16) * (kp // 4) * 2 * 2048
     nnz_r = pairs1 * pairs1

@@ -403,6 +403,10 @@ typedef struct pp_fham pp_fham;
 int pp_fham_create(int nchan, int n, int nterms, int bw, const double* A, const int* term_potential,
                    const double* dia, int npotentials, pp_fham** out);
 int pp_fham_destroy(pp_fham* f);
+/* Terms whose band is an outer product a (x) b run as two 1-D band passes instead of the
+ * MFMA band tiles; band_terms receives the count left on the tiles (nterms when none is
+ * separable). */
+int pp_fham_band_terms(const pp_fham* f, int* band_terms);
 /* dst = addend + alpha * sum_p scalings[p] V_p src (addend may be NULL; src may alias dst) */
 int pp_fham_multiply(const pp_fham* f, const double* scalings, const pp_cplx* src, pp_cplx* dst,
                      const pp_cplx* addend, pp_cplx alpha, pp_stream stream);

@@ -118,6 +118,7 @@ SIGNATURES = [
     ("pp_fham_set_angular_waves", I, [P, I, I]),
     ("pp_fham_get_angular_waves", I, [P, I, P]),
     ("pp_fham_set_angular_variant", I, [P, I, I]),
+    ("pp_fham_band_terms", I, [P, P]),
     ("pp_fham_get_angular_variant", I, [P, I, P]),
     ("pp_fham_tune", I, [P, I, I, P, P, P]),
     ("pp_fham_set_overlap", I, [P, P, P]),

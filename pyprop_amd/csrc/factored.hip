@@ -909,6 +909,13 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
   });
 }
 
+extern "C" int pp_fham_band_terms(const pp_fham* f, int* band_terms) {
+  return guarded([&] {
+    PP_REQUIRE(f && band_terms, "null argument");
+    *band_terms = f->f.sep_enabled ? f->f.nband : f->f.nterms;
+  });
+}
+
 extern "C" int pp_fham_destroy(pp_fham* f) {
   return guarded([&] { delete f; });
 }

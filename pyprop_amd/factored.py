@@ -87,6 +87,9 @@ class FactoredHamiltonian:
         N.check(N.load().pp_fham_create(nchan, n, self.nterms, bw, D.hptr(A), D.hptr(tpot), D.hptr(dia),
                                         len(base.PotentialList), C.byref(h)))
         self.h = h
+        nb = C.c_int()
+        N.check(N.load().pp_fham_band_terms(h, C.byref(nb)))
+        self.band_terms = nb.value  # terms on the MFMA band tiles; the rest are separable
         self.has_overlap = False
         bases = base.psi.repr.bases
         if overlap and bases[1] is not None and bases[2] is not None:
