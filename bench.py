@@ -119,6 +119,28 @@ def time_matvec(torch, base, psi, reps=20):
     return e0.elapsed_time(e1) / reps / 1e3
 
 
+def time_stream_copy(torch, nbytes=1 << 30, reps=20):
+    """Device-to-device copy rate (read + write bytes / time) of torch's copy kernel on a
+    1 GiB buffer: the practical HBM ceiling that the multiply's rate is read against."""
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record(st)
+    e1.synchronize()
+    t = e0.elapsed_time(e1) / reps / 1e3
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * nbytes / t / 1e9
+
+
 def tuned_factored(base):
     """The factored operator with its radial kernel shape tuned for both precisions."""
     from pyprop_amd.factored import FactoredHamiltonian
@@ -410,6 +432,8 @@ def main():
     achieved = alg_bytes / t_mv / 1e9
     log("fused H multiply: %.3f ms  alg %.3f GB  -> %.1f GB/s (%.1f%% of 8 TB/s)" %
         (t_mv * 1e3, alg_bytes / 1e9, achieved, 100 * achieved / HBM_PEAK_GBS))
+    copy_gbs = time_stream_copy(torch)
+    log("device copy (1 GiB): %.1f GB/s; the multiply reaches %.1f%% of it" % (copy_gbs, 100 * achieved / copy_gbs))
     fh, fh_tune = tuned_factored(base)
     nc, n, _ = sysm.shape
     t_f64 = time_factored(torch, fh, psi, torch.complex128)
@@ -585,7 +609,7 @@ def main():
                      "traffic_source": "profiles/tp_dia_kernel_helium_fp64_matvec.md (rocprofv3 FETCH_SIZE + "
                                        "WRITE_SIZE, scaled by a 1 GiB copy)",
                      "kernel": "tp_dia_kernel (fused H0+R12+dipole FP64 multiply)", "ms": t_mv * 1e3,
-                     "alg_bytes": alg_bytes},
+                     "alg_bytes": alg_bytes, "copy_GBps": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
