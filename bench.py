@@ -148,9 +148,8 @@ def tuned_factored(base):
     tune = {}
     for prec in (64, 32):
         best, tim = fh.tune(prec, trials=5)
-        name = lambda k: "%dx%d%s" % (k[0], k[1], "p" if k[2] else "")
-        tune["fp%d" % prec] = {"best": name(fh.RADIAL_SHAPES[best]),
-                               "us": {name(k): round(1e3 * v, 1) for k, v in tim.items()}}
+        tune["fp%d" % prec] = {"best": fh.RADIAL_SHAPES[best],
+                               "us": {k: round(1e3 * v, 1) for k, v in tim.items()}}
         tune["fp%d" % prec]["angular"] = fh.ANGULAR_VARIANTS[fh.angular_variant(prec)]
         tune["fp%d" % prec]["angular_us"] = {str(k): round(1e3 * v, 1) for k, v in fh.angular_timings.items()}
         log("factored radial tune FP%d: best %s  (%s); angular %s (%s)" % (
@@ -177,16 +176,26 @@ def time_factored(torch, fh, psi, dtype, reps=50):
     return e0.elapsed_time(e1) / reps / 1e3
 
 
-def factored_flops(fh, nc, n):
-    """(issued MFMA flops incl. padding, useful flops) of one factored multiply:
-    radial stage, one 16x16x4 MFMA per (band-tile term, i1, jj, i2 tile, channel group, K
-    step) -- the separable terms run as two 1-D band passes on the vector units and are not
-    counted as issued MFMA work; angular stage, two MFMAs (re, im) per (16 channels, 16
-    positions, K step of 4)."""
+def factored_flops(fh, nc, n, precision):
+    """(issued MFMA flops incl. padding, useful flops) of one factored multiply in the radial
+    variant chosen for `precision`.  Radial stage: band-tile variants issue one 16x16x4 MFMA
+    per (band-tile term, i1, jj, i2 tile, channel group, K step); the position variants
+    ("pos<g>") one per (16 band terms, position, channel group, K step of the w x w window),
+    channel groups rounded up to a multiple of g.  The separable terms run as two 1-D band
+    passes on the vector units and are not counted.  Angular stage: two MFMAs (re, im) per
+    (16 channels, 16 positions, K step of 4)."""
     T, bw = fh.nterms, fh.bw
+    w = 2 * bw + 1
     pairs1 = sum(min(n, i + bw + 1) - max(0, i - bw) for i in range(n))
-    ks = (16 + 2 * bw + 3) // 4
-    rad_issued = fh.band_terms * pairs1 * ks * math.ceil(n / 16) * math.ceil(nc / 8) * 2048
+    groups = math.ceil(nc / 8)
+    name = fh.RADIAL_SHAPES[fh.radial_variant(precision)]
+    if name.startswith("pos"):
+        g = int(name[3:])
+        rad_issued = (math.ceil(fh.band_terms / 16) * n * n * math.ceil(groups / g) * g *
+                      ((w * w + 3) // 4) * 2048)
+    else:
+        ks = (16 + 2 * bw + 3) // 4
+        rad_issued = fh.band_terms * pairs1 * ks * math.ceil(n / 16) * groups * 2048
     kp = (T * nc + 3) // 4 * 4
     ang_issued = math.ceil(n * n / 16) * math.ceil(nc / 16) * (kp // 4) * 2 * 2048
     nnz_r = pairs1 * pairs1
@@ -438,10 +447,12 @@ def main():
     nc, n, _ = sysm.shape
     t_f64 = time_factored(torch, fh, psi, torch.complex128)
     t_f32 = time_factored(torch, fh, psi, torch.complex64)
-    issued, useful = factored_flops(fh, nc, n)
+    issued64, useful = factored_flops(fh, nc, n, 64)
+    issued32, _ = factored_flops(fh, nc, n, 32)
     fact = {"fp64_ms": t_f64 * 1e3, "fp32_ms": t_f32 * 1e3, "fp64_matvecs_per_s": 1 / t_f64,
-            "fp32_matvecs_per_s": 1 / t_f32, "mfma_flops_issued": issued, "useful_flops": useful,
-            "fp64_mfma_util": issued / t_f64 / 78.6e12, "fp32_mfma_util": issued / t_f32 / 157.3e12,
+            "fp32_matvecs_per_s": 1 / t_f32, "fp64_mfma_flops_issued": issued64,
+            "fp32_mfma_flops_issued": issued32, "useful_flops": useful,
+            "fp64_mfma_util": issued64 / t_f64 / 78.6e12, "fp32_mfma_util": issued32 / t_f32 / 157.3e12,
             "radial_tune": fh_tune}
     log("factored multiply: FP64 %.1f us (%.1f%% of FP64 MFMA peak), FP32 %.1f us (%.1f%% of FP32 MFMA peak)" %
         (t_f64 * 1e6, 100 * fact["fp64_mfma_util"], t_f32 * 1e6, 100 * fact["fp32_mfma_util"]))

@@ -413,9 +413,12 @@ int pp_fham_multiply(const pp_fham* f, const double* scalings, const pp_cplx* sr
 int pp_fham_multiply_f32(const pp_fham* f, const double* scalings, const pp_cplxf* src, pp_cplxf* dst,
                          const pp_cplxf* addend, pp_cplx alpha, pp_stream stream);
 /* Radial-stage kernel shape of the factored multiply (no reference counterpart: the
- * reference has no factored form).  Variants 0..6 compute (terms x groups of 8 channels) =
- * (1,1), (2,2), (3,2), (3,3) per wave, and 4..6 the shapes of 1..3 with the next band step's
- * operands prefetched into registers; precision is 64 (complex128 multiply) or 32
+ * reference has no factored form).  Variants 0..6 multiply band tiles: (terms x groups of 8
+ * channels) = (1,1), (2,2), (3,2), (3,3) per wave, and 4..6 the shapes of 1..3 with the next
+ * band step's operands prefetched into registers.  Variants 7..9 form each position's
+ * products of all band terms as one small GEMM over the w x w window (no band padding), with
+ * 6, 3 or 2 channel groups per wave; they need the separable split (n <= 64) and
+ * pp_fham_tune reports -1 for them without it.  precision is 64 (complex128 multiply) or 32
  * (complex64).  pp_fham_tune times every radial variant (trials multiplies each, on stream)
  * and keeps the fastest, then does the same for the angular stage's form: variants 0..2 run
  * one row block of 16 channels per workgroup with 16, 8 or 4 waves splitting its K, 3..4 all
@@ -424,7 +427,7 @@ int pp_fham_multiply_f32(const pp_fham* f, const double* scalings, const pp_cplx
  * radial then PP_FHAM_ANGULAR_VARIANTS angular entries.  set_angular_waves selects among
  * variants 0..2 by wave count; get_angular_waves returns the wave count of the current
  * variant. */
-#define PP_FHAM_RADIAL_VARIANTS 7
+#define PP_FHAM_RADIAL_VARIANTS 10
 #define PP_FHAM_ANGULAR_VARIANTS 5
 int pp_fham_set_radial_variant(pp_fham* f, int precision, int variant);
 int pp_fham_get_radial_variant(const pp_fham* f, int precision, int* variant);

@@ -286,11 +286,139 @@ __global__ __launch_bounds__(64) void fham_radial_reuse_kernel(int nc, int ncp, 
   }
 }
 
-// (terms, channel groups, prefetch) per wave of each radial variant; variant 0 is the
-// one-term, one-group kernel
-constexpr int kRadTG[kFhamRadialVariants] = {1, 2, 3, 3, 2, 3, 3};
-constexpr int kRadGG[kFhamRadialVariants] = {1, 2, 2, 3, 2, 2, 3};
-constexpr bool kRadPipe[kFhamRadialVariants] = {false, false, false, false, true, true, true};
+// Stage 1, position form.  At one position (i1, i2) the radial products of all band terms
+// are one small GEMM: Z_t[(c, part)][i1][i2] = sum_k X[(c, part)][k] R[k][t] over the
+// w x w window k = jj w + d, X[.][k] = X[c][i1 - BW + jj][i2 - BW + d] (zero outside the
+// grid), R[k][t] = dia_t[i1][jj][d][i2].  No band padding enters the MFMAs (the band-tile
+// form multiplies (16 + 2 BW)-wide tiles of which w diagonals are non-zero).  RP: [term
+// tile of 16][position i1 n + i2][KP][64] in MFMA lane order, lane l of step ks holding
+// R[4 ks + l / 16][16 tile + l % 16], zero past w^2 and past the last term.  A wave forms one
+// position's products for MG channel groups of 8 (16 operand rows each), sharing each R
+// operand among them.  The R operands of KC steps are loaded at once (a wave's R stream
+// comes from HBM, so it waits once per chunk instead of once per step batch); the X
+// operands, read through L2, are loaded KB steps ahead of their MFMAs.  Waves: (term tile,
+// position, group chunk), group chunk fastest, four per workgroup.
+template <class T, int BW, int MG>
+__global__ __launch_bounds__(256) void fham_radial_pos_kernel(int nc, int ncp, int n, int nterms, int ngroups,
+                                                              long long nwaves, const T* __restrict__ Xt,
+                                                              const T* __restrict__ RP, T* __restrict__ Z,
+                                                              const int* __restrict__ differs) {
+  if (differs && *differs == 0) return;  // input equals the one whose products Z holds
+  typedef typename MfmaT<T>::V4 V4;
+  typedef typename MfmaT<T>::C2 C2;
+  constexpr int w = 2 * BW + 1, KK = w * w, KP = (KK + 3) / 4;
+  constexpr int KB = 4;                  // steps per X operand batch
+  constexpr int KC = sizeof(T) == 8 ? 24 : 44;  // steps per R operand chunk (a multiple of KB)
+  constexpr int NA = MG >= 4 ? 1 : 2;    // independent accumulator chains per group
+  const long long nblocks = (nwaves + 3) / 4;
+  const long long per = (nblocks + 7) / 8;
+  const long long hw = blockIdx.x;
+  const long long L = (hw % 8) * per + hw / 8;  // XCD-aware: consecutive blocks share an L2
+  const long long wv = L * 4 + (threadIdx.x >> 6);
+  if (L >= nblocks || wv >= nwaves) return;  // whole waves; the kernel has no barrier
+  const int nmg = (ngroups + MG - 1) / MG;
+  const long long P = (long long)n * n;
+  const int mgc = (int)(wv % nmg);
+  const long long pos = (wv / nmg) % P;
+  const int tn = (int)(wv / ((long long)nmg * P));
+  const int i1 = (int)(pos / n), i2 = (int)(pos - (long long)i1 * n);
+  const int lane = threadIdx.x & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int cl = MfmaT<T>::kRowMajorOut ? (lr >> 1) : (lr & 7);
+  const int part = MfmaT<T>::kRowMajorOut ? (lr & 1) : (lr >> 3);
+  int roff[MG];  // this lane's operand row within a position's ncp * 2 values, per group
+#pragma unroll
+  for (int g = 0; g < MG; ++g) roff[g] = (min(mgc * MG + g, ngroups - 1) * 8 + cl) * 2 + part;
+  const T* Rb = RP + ((long long)tn * P + pos) * KP * 64 + lane;
+  V4 acc[NA][MG];
+#pragma unroll
+  for (int h = 0; h < NA; ++h)
+#pragma unroll
+    for (int g = 0; g < MG; ++g) acc[h][g] = V4{0, 0, 0, 0};
+  // X operands, step after step (k = 4 ks + lk = jj w + d advances by 4 per step): the
+  // lane's window position (jj, d) and its offset in Xt are carried from step to step.
+  // Outside the grid (or past k = w^2, where jj reaches w) the load reads the position-0
+  // element and the select zeroes it.
+  const int S2 = ncp * 2, S1 = n * S2;  // Xt strides of j2 and j1 (Xt has n n ncp 2 < 2^31 values)
+  const int jlo = max(0, BW - i1), jhi = min(w - 1, n - 1 - i1 + BW);
+  const int dlo = max(0, BW - i2), dhi = min(w - 1, n - 1 - i2 + BW);
+  int jj = lk / w, d = lk - (lk / w) * w;  // k = lk at step 0 (w = 3 puts lk = 3 in row 1)
+  int off = (i1 - BW + jj) * S1 + (i2 - BW + d) * S2;
+  auto load_x = [&](T (&a)[MG]) {
+    const bool ok = jj >= jlo && jj <= jhi && d >= dlo && d <= dhi;
+    const int o = ok ? off : 0;
+#pragma unroll
+    for (int g = 0; g < MG; ++g) {
+      const T v = Xt[o + roff[g]];
+      a[g] = ok ? v : T(0);
+    }
+    d += 4;
+    off += 4 * S2;
+    const bool wrap = d >= w;
+    d = wrap ? d - w : d;
+    jj = wrap ? jj + 1 : jj;
+    off = wrap ? off + S1 - w * S2 : off;
+    if constexpr (w < 4) {  // w = 3: a step of 4 can cross two rows
+      const bool wrap2 = d >= w;
+      d = wrap2 ? d - w : d;
+      jj = wrap2 ? jj + 1 : jj;
+      off = wrap2 ? off + S1 - w * S2 : off;
+    }
+  };
+  T a0[KB][MG];
+#pragma unroll
+  for (int u = 0; u < KB; ++u) load_x(a0[u]);  // steps 0 .. KB - 1
+  for (int c0 = 0; c0 < KP; c0 += KC) {
+    T bc[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) bc[u] = Rb[(long long)min(c0 + u, KP - 1) * 64];  // past the end: unused
+#pragma unroll
+    for (int kb = 0; kb < KC; kb += KB) {
+      T a1[KB][MG];
+#pragma unroll
+      for (int u = 0; u < KB; ++u) load_x(a1[u]);  // steps c0 + kb + KB + u (past KP: zeros, unused)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+        if (c0 + kb + u < KP) {
+#pragma unroll
+          for (int g = 0; g < MG; ++g) acc[u % NA][g] = MfmaT<T>::mma(a0[u][g], bc[kb + u], acc[u % NA][g]);
+        }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+#pragma unroll
+        for (int g = 0; g < MG; ++g) a0[u][g] = a1[u][g];
+    }
+  }
+  const int t = tn * 16 + lr;  // result column: the term
+  if (t >= nterms) return;
+#pragma unroll
+  for (int g = 0; g < MG; ++g) {
+    const int grp = mgc * MG + g;
+    if (grp >= ngroups) continue;
+    V4 r = acc[0][g];
+    if constexpr (NA == 2) r = r + acc[1][g];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = grp * 8 + (MfmaT<T>::kRowMajorOut ? lk * 2 + q : lk + 4 * q);
+      if (c >= nc) continue;
+      C2 v;
+      v.x = MfmaT<T>::kRowMajorOut ? r[2 * q] : r[q];
+      v.y = MfmaT<T>::kRowMajorOut ? r[2 * q + 1] : r[q + 2];
+      reinterpret_cast<C2*>(Z)[(((long long)t * nc + c) * n + i1) * n + i2] = v;
+    }
+  }
+}
+
+// (terms, channel groups of 8, prefetch) per wave of each band-tile radial variant; variant
+// 0 is the one-term, one-group kernel; variants with kRadPosMG > 0 are the position form
+// with that many channel groups per wave (only with separable terms split off, which fixes
+// the set of terms its tiles hold)
+constexpr int kRadTG[kFhamRadialVariants] = {1, 2, 3, 3, 2, 3, 3, 1, 1, 1};
+constexpr int kRadGG[kFhamRadialVariants] = {1, 2, 2, 3, 2, 2, 3, 1, 1, 1};
+constexpr bool kRadPipe[kFhamRadialVariants] = {false, false, false, false, true, true, true, false, false, false};
+constexpr int kRadPosMG[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 6, 3, 2};
 // angular variants: waves splitting K, and whether one workgroup covers all row blocks of
 // its position tile (reading each Z element once) or one row block
 constexpr int kAngWaveChoices[kFhamAngularVariants] = {16, 8, 4, 8, 16};
@@ -299,7 +427,25 @@ constexpr int kAngMaxRowBlocks = 4;  // all-row variants: at most 64 channels
 
 template <class T, int B>
 static void launch_radial(int variant, int nc, int ncp, int n, int nterms, int ngroups, int tiles, const T* Xt,
-                          const T* BT, T* Z, const int* differs, hipStream_t s) {
+                          const T* BT, const T* RP, T* Z, const int* differs, hipStream_t s) {
+  if (kRadPosMG[variant] > 0) {
+    const int MG = kRadPosMG[variant];
+    if (!RP) throw Error(PP_ERR_INVALID, "position-form radial variant without position tiles");
+    const long long nwaves = (long long)((nterms + 15) / 16) * n * n * ((ngroups + MG - 1) / MG);
+    const long long nblocks = (nwaves + 3) / 4;
+    const dim3 grid((unsigned)((nblocks + 7) / 8 * 8));
+    if (MG == 6)
+      hipLaunchKernelGGL((fham_radial_pos_kernel<T, B, 6>), grid, dim3(256), 0, s, nc, ncp, n, nterms, ngroups,
+                         nwaves, Xt, RP, Z, differs);
+    else if (MG == 3)
+      hipLaunchKernelGGL((fham_radial_pos_kernel<T, B, 3>), grid, dim3(256), 0, s, nc, ncp, n, nterms, ngroups,
+                         nwaves, Xt, RP, Z, differs);
+    else
+      hipLaunchKernelGGL((fham_radial_pos_kernel<T, B, 2>), grid, dim3(256), 0, s, nc, ncp, n, nterms, ngroups,
+                         nwaves, Xt, RP, Z, differs);
+    PP_HIP(hipGetLastError());
+    return;
+  }
   const int TG = kRadTG[variant], GG = kRadGG[variant];
   const long long nblocks =
       (long long)n * tiles * ((nterms + TG - 1) / TG) * ((ngroups + GG - 1) / GG);
@@ -627,7 +773,7 @@ __global__ __launch_bounds__(1024) void fham_separable_kernel(int nc, int n, int
 template <class T>
 static void fham_run(const FHam& F, const double* scalings, const typename MfmaT<T>::C2* src, const FOut<T>& o1,
                      const FOut<T>* o2, const DeviceBuffer<T>& Ab, const DeviceBuffer<T>& Db,
-                     const DeviceBuffer<T>& SAb, const DeviceBuffer<T>& SBb, DeviceBuffer<T>& Zb,
+                     const DeviceBuffer<T>& RPb, const DeviceBuffer<T>& SAb, const DeviceBuffer<T>& SBb, DeviceBuffer<T>& Zb,
                      DeviceBuffer<T>& Xb, const T* ref, int* differs, hipStream_t s) {
   const int nc = F.nc, n = F.n, nterms = F.nterms;
   const long long P = (long long)n * n;
@@ -647,11 +793,13 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   const int variant = sizeof(T) == 8 ? F.rad_variant64 : F.rad_variant32;
   // band-tile terms first; with separable terms those (and the overlap term) follow
   const int nrad_band = F.sep_enabled ? F.nband : nrad;
+  PP_REQUIRE(kRadPosMG[variant] == 0 || F.sep_enabled, "position-form radial variant needs the separable split");
   if (nrad_band > 0) {
     switch (F.bw) {
 #define PP_RAD(B)                                                                                                    \
   case B:                                                                                                            \
-    launch_radial<T, B>(variant, nc, ncp, n, nrad_band, ngroups, tiles, Xb.ptr, Db.ptr, Zb.ptr, differs, s);        \
+    launch_radial<T, B>(variant, nc, ncp, n, nrad_band, ngroups, tiles, Xb.ptr, Db.ptr, RPb.ptr, Zb.ptr, differs,   \
+                        s);                                                                                          \
     break;
       PP_RAD(1) PP_RAD(2) PP_RAD(3) PP_RAD(4) PP_RAD(5) PP_RAD(6) PP_RAD(7) PP_RAD(8)
 #undef PP_RAD
@@ -749,7 +897,7 @@ void FHam::apply64_ex(const double* scalings, const double2* src, const FHamOut&
     a2.alpha = o2->alpha;
     a2.beta = o2->beta;
   }
-  fham_run<double>(*this, scalings, src, a1, o2 ? &a2 : nullptr, A64, bt64, sa64, sb64, W64, X64,
+  fham_run<double>(*this, scalings, src, a1, o2 ? &a2 : nullptr, A64, bt64, rp64, sa64, sb64, W64, X64,
                    reinterpret_cast<const double*>(reuse_ref), reuse_ref ? reuse_flag.ptr : nullptr, s);
   // when the products were reused they cover at least nrad terms; when recomputed, exactly nrad
   w64_terms = nrad;
@@ -762,7 +910,7 @@ void FHam::apply32(const double* scalings, const float2* src, float2* dst, const
   o.addend = addend;
   o.alpha = alpha;
   o.beta = beta;
-  fham_run<float>(*this, scalings, src, o, nullptr, A32, bt32, sa32, sb32, W32, X32, nullptr, nullptr, s);
+  fham_run<float>(*this, scalings, src, o, nullptr, A32, bt32, rp32, sa32, sb32, W32, X32, nullptr, nullptr, s);
 }
 
 }  // namespace pp
@@ -789,6 +937,26 @@ static std::vector<double> band_tiles(int nterms, int n, int bw, const double* d
                   dia[((((size_t)t * n + i1) * w + jj) * w + d) * n + i2];
             }
   return bt;
+}
+
+// Position tiles [term tile][i1 n + i2][ks][lane] from dia[t][i1][jj][d][i2]: lane l of step
+// ks holds R[k][t] = dia[t][i1][jj][d][i2] at k = 4 ks + l / 16 = jj w + d, t = 16 tile + l % 16.
+static std::vector<double> position_tiles(int nterms, int n, int bw, const double* dia) {
+  const int w = 2 * bw + 1;
+  const int KP = (w * w + 3) / 4;
+  const int ntn = (nterms + 15) / 16;
+  std::vector<double> rp((size_t)ntn * n * n * KP * 64, 0.0);
+  for (int t = 0; t < nterms; ++t)
+    for (int i1 = 0; i1 < n; ++i1)
+      for (int jj = 0; jj < w; ++jj)
+        for (int d = 0; d < w; ++d) {
+          const int k = jj * w + d;
+          const int lane = (k % 4) * 16 + t % 16;
+          for (int i2 = 0; i2 < n; ++i2)
+            rp[((((size_t)(t / 16) * n + i1) * n + i2) * KP + k / 4) * 64 + lane] =
+                dia[((((size_t)t * n + i1) * w + jj) * w + d) * n + i2];
+        }
+  return rp;
 }
 
 // A term whose band D[i1][jj][d][i2] equals a[i1][jj] b[i2][d] (entries within 1e-13 of the
@@ -886,6 +1054,8 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
     for (int k = 0; k < F.Kp; ++k) tk[k] = std::min(k / nchan, nterms - 1);  // term of column k
     std::vector<double> bt = band_tiles(F.nband, n, bw, dband.data());
     const size_t nbt = bt.size();
+    std::vector<double> rp;
+    if (F.sep_enabled && F.nband > 0) rp = position_tiles(F.nband, n, bw, dband.data());
     try {
       hipStream_t s = nullptr;
       F.A64.upload(a.data(), a.size(), s);
@@ -895,6 +1065,11 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
       F.bt64.upload(bt.data(), nbt, s);
       std::vector<float> bf(bt.begin(), bt.end());
       F.bt32.upload(bf.data(), nbt, s);
+      if (!rp.empty()) {
+        F.rp64.upload(rp.data(), rp.size(), s);
+        std::vector<float> rf(rp.begin(), rp.end());
+        F.rp32.upload(rf.data(), rf.size(), s);
+      }
       F.sa64.upload(sav.data(), sav.size(), s);
       F.sb64.upload(sbv.data(), sbv.size(), s);
       std::vector<float> saf(sav.begin(), sav.end()), sbf(sbv.begin(), sbv.end());
@@ -942,6 +1117,8 @@ extern "C" int pp_fham_set_radial_variant(pp_fham* f, int precision, int variant
   return guarded([&] {
     PP_REQUIRE(f && (precision == 64 || precision == 32), "precision must be 64 or 32");
     PP_REQUIRE(variant >= 0 && variant < kFhamRadialVariants, "radial variant out of range");
+    PP_REQUIRE(kRadPosMG[variant] == 0 || f->f.sep_enabled,
+               "position-form radial variants need the separable split (n <= 64)");
     (precision == 64 ? f->f.rad_variant64 : f->f.rad_variant32) = variant;
   });
 }
@@ -1015,6 +1192,10 @@ extern "C" int pp_fham_tune(pp_fham* f, int precision, int trials, int* best, do
     double best_ms = 1e300;
     int best_v = var;
     for (int v = 0; v < kFhamRadialVariants; ++v) {
+      if (kRadPosMG[v] > 0 && !F.sep_enabled) {
+        if (timings_ms) timings_ms[v] = -1.0;  // not available for this operator
+        continue;
+      }
       var = v;
       run();  // warm-up
       PP_HIP(hipEventRecord(e0, s));

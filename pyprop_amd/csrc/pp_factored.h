@@ -10,7 +10,7 @@
 namespace pp {
 
 constexpr int kMaxFTerms = 32;
-constexpr int kFhamRadialVariants = 7;  // radial-stage kernel shapes (terms x channel groups per wave, prefetch)
+constexpr int kFhamRadialVariants = 10;  // radial-stage kernel shapes (band-tile and position forms)
 constexpr int kFhamAngularVariants = 5;  // angular-stage forms (waves splitting K, row blocks per workgroup)
 
 struct FHamOut {
@@ -39,6 +39,11 @@ struct FHam {
   int nband = 0;
   DeviceBuffer<double> sa64, sb64;
   DeviceBuffer<float> sa32, sb32;
+  // Position tiles of the band terms (with the separable split only): [ceil(nband / 16)][n n]
+  // [ceil(w^2 / 4)][64], lane l of step ks holding R_t[i1][jj][d][i2] at 4 ks + l / 16 =
+  // jj w + d, t = 16 tile + l % 16
+  DeviceBuffer<double> rp64;
+  DeviceBuffer<float> rp32;
   int rad_variant64 = 0, rad_variant32 = 0;  // radial kernel shape per precision
   int ang_variant64 = 0, ang_variant32 = 0;  // angular stage wave count per precision
   mutable unsigned long long gen64 = 0;      // FP64 multiplies issued

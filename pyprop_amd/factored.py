@@ -107,7 +107,6 @@ class FactoredHamiltonian:
         fn = N.load().pp_fham_multiply_f32 if s.dtype == D.torch.complex64 else N.load().pp_fham_multiply
         N.check(fn(self.h, D.hptr(sc), D.ptr(s), D.ptr(d), D.ptr(a), N.cplx(alpha), D.stream_handle()))
 
-    # (terms, channel groups of 8, next band step prefetched) per wave of each radial variant
     def MultiplyOverlap(self, src, dst, t, alpha=1.0, beta=1.0, addend=None):
         """dst = addend + alpha * H(t) src + beta * (S (x) S) src (needs the overlap term)."""
         sc = self.base.scalings(t)
@@ -119,7 +118,10 @@ class FactoredHamiltonian:
         N.check(fn(self.h, D.hptr(sc), D.ptr(s), D.ptr(d), D.ptr(a), N.cplx(alpha), N.cplx(beta),
                    D.stream_handle()))
 
-    RADIAL_SHAPES = ((1, 1, 0), (2, 2, 0), (3, 2, 0), (3, 3, 0), (2, 2, 1), (3, 2, 1), (3, 3, 1))
+    # radial variants: "<terms>x<channel groups>" per wave on band tiles ("p": next band step
+    # prefetched), "pos<groups>": each position's products of all band terms as one GEMM over
+    # the w x w window (needs the separable split; tune reports -1 ms without it)
+    RADIAL_SHAPES = ("1x1", "2x2", "3x2", "3x3", "2x2p", "3x2p", "3x3p", "pos6", "pos3", "pos2")
 
     def set_radial_variant(self, precision: int, variant: int):
         """Radial-stage kernel shape for the complex128 (precision 64) or complex64 (32)
