@@ -160,6 +160,10 @@ class BasisPropagator:
         device, as GeneratePotential does)."""
         self.psi = psi
         self.generation = generation
+        # a potential with no basis pairs on some rank (a dipole coupling when only one total
+        # L is present) adds nothing to H, as the reference's loop over zero pairs does; it
+        # is left out instead of being given an empty device plan
+        potentials = [p for p in potentials if all(g.pair_count > 0 for g in p.geometries)]
         self.PotentialList = [TensorPotential(p, psi.repr.shape, generation) for p in potentials]
         if consolidate:
             self.ConsolidatePotentials()

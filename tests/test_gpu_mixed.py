@@ -582,6 +582,48 @@ def test_factored_multiply_full_size_vs_tensor_potentials(torch, rng):
         assert e64 < 1e-12 and e32 < 2e-6, (name, e64, e32)
 
 
+def test_factored_without_separable_split(torch, rng):
+    """69 radial functions (more than the separable kernel's 64): every term stays on the
+    band tiles, the position-form radial variants are refused and the tuner reports -1 ms
+    for them, and every band-tile variant equals the fused tensor-potential multiply.  With
+    the single total L = 0 the dipole potential has no basis pairs and is left out."""
+    from pyprop_amd._native import NativeError
+    from pyprop_amd.factored import FactoredHamiltonian
+    from pyprop_amd.systems import helium_system
+    say("factored multiply without the separable split (n = 69)")
+    sysm = helium_system(lmax=1, L=(0,), order=5, n_breakpoints=68, rmax=40.0, n_inner=6, gamma=2.5)
+    assert sysm.shape[1] > 64
+    repr_, psi, base = _device(sysm, torch)
+    assert len(base.PotentialList) == 2  # H0, R12 (the dipole has no pairs)
+    base.SetupStep()
+    fh = FactoredHamiltonian(base)
+    assert fh.band_terms == fh.nterms
+    x = torch.from_numpy(crandn(rng, sysm.shape)).to("cuda")
+    ref = torch.empty_like(x)
+    got = torch.empty_like(x)
+    g32 = torch.empty_like(x, dtype=torch.complex64)
+    t = 55.0
+    base.MultiplyHamiltonianNoOverlap(x, ref, t)
+    for v, name in enumerate(fh.RADIAL_SHAPES):
+        if name.startswith("pos"):
+            for prec in (64, 32):
+                with pytest.raises(NativeError, match="separable split"):
+                    fh.set_radial_variant(prec, v)
+            continue
+        fh.set_radial_variant(64, v)
+        fh.set_radial_variant(32, v)
+        fh.Multiply(x, got, t)
+        fh.Multiply(x.to(torch.complex64), g32, t)
+        e64 = (torch.linalg.norm(got - ref) / torch.linalg.norm(ref)).item()
+        e32 = (torch.linalg.norm(g32.to(torch.complex128) - ref) / torch.linalg.norm(ref)).item()
+        say("  %s: FP64 %.2e  FP32 %.2e" % (name, e64, e32))
+        assert e64 < 1e-12 and e32 < 2e-6, (name, e64, e32)
+    best, tim = fh.tune(64, trials=1)
+    assert not fh.RADIAL_SHAPES[best].startswith("pos")
+    assert all(ms == -1.0 for k, ms in tim.items() if k.startswith("pos"))
+    assert all(ms > 0 for k, ms in tim.items() if not k.startswith("pos"))
+
+
 def test_mixed_cayley_factored_full_size_vs_plain(torch):
     """Full helium, 20 steps: mixed FP32 + factored multiply (with its FP64 overlap term)
     against the plain FP64 path converged to 1e-14.  The bar, 2e-10 after 20 steps, is about
