@@ -178,29 +178,37 @@ def time_factored(torch, fh, psi, dtype, reps=50):
 
 def factored_flops(fh, nc, n, precision):
     """(issued MFMA flops incl. padding, useful flops) of one factored multiply in the radial
-    variant chosen for `precision`.  Radial stage: band-tile variants issue one 16x16x4 MFMA
-    per (band-tile term, i1, jj, i2 tile, channel group, K step); the position variants
-    ("pos<g>") one per (16 band terms, position, channel group, K step of the w x w window),
-    channel groups rounded up to a multiple of g.  The separable terms run as two 1-D band
-    passes on the vector units and are not counted.  Angular stage: two MFMAs (re, im) per
-    (16 channels, 16 positions, K step of 4)."""
+    and angular variants chosen for `precision`.  Radial stage: band-tile variants with
+    register reuse issue one 16x16x4 MFMA per (band-tile term, i1, jj, i2 tile, channel
+    group, K step); the position variants ("pos<g>") one per (16 band terms, position,
+    channel group, K step of the w x w window), channel groups rounded up to a multiple of
+    g.  The separable terms run as two 1-D band passes on the vector units and are not
+    counted.  Angular stage: two MFMAs (re, im) per (16 channels, 16 positions, K step of 4),
+    each wave's share of the K steps rounded up to whole batches (12 steps, 4 in the
+    all-rows form).  At helium these are the SQ_INSTS_MFMA counts the hardware reports
+    (band tiles 1,487,808, position form 928,800, angular 259,200 per multiply)."""
     T, bw = fh.nterms, fh.bw
     w = 2 * bw + 1
     pairs1 = sum(min(n, i + bw + 1) - max(0, i - bw) for i in range(n))
     groups = math.ceil(nc / 8)
     name = fh.RADIAL_SHAPES[fh.radial_variant(precision)]
     if name.startswith("pos"):
-        g = int(name[3:])
-        rad_issued = (math.ceil(fh.band_terms / 16) * n * n * math.ceil(groups / g) * g *
-                      ((w * w + 3) // 4) * 2048)
+        g = int(name[3])
+        rad_mfma = math.ceil(fh.band_terms / 16) * n * n * math.ceil(groups / g) * g * ((w * w + 3) // 4)
     else:
         ks = (16 + 2 * bw + 3) // 4
-        rad_issued = fh.band_terms * pairs1 * ks * math.ceil(n / 16) * groups * 2048
-    kp = (T * nc + 3) // 4 * 4
-    ang_issued = math.ceil(n * n / 16) * math.ceil(nc / 16) * (kp // 4) * 2 * 2048
+        rad_mfma = fh.band_terms * pairs1 * ks * math.ceil(n / 16) * groups
+    av = fh.ANGULAR_VARIANTS[fh.angular_variant(precision)]
+    waves = int(av.replace("rows", ""))
+    batch = 4 if av.startswith("rows") else 12
+    nks = (T * nc + 3) // 4
+    per = math.ceil(nks / waves)
+    ang_steps = sum(math.ceil(max(0, min(nks, k0 + per) - k0) / batch) * batch
+                    for k0 in range(0, waves * per, per))
+    ang_mfma = math.ceil(n * n / 16) * math.ceil(nc / 16) * ang_steps * 2
     nnz_r = pairs1 * pairs1
     useful = T * nnz_r * 4 * nc + 2 * nc * T * nc * 2 * n * n
-    return rad_issued + ang_issued, useful
+    return (rad_mfma + ang_mfma) * 2048, useful
 
 
 def bench_batch(torch, steps=1000, per_graph=10, natoms=64, seed=2024, dt=0.01):
