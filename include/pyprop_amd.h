@@ -555,7 +555,8 @@ int pp_batch_create(int natoms, const int* nl, const int* n, int band, const dou
                     const double* cpl, const double* s_upper, const double* pulse, int krylov, int pade_order,
                     int fp32_matvec, pp_batch** out);
 int pp_batch_destroy(pp_batch* b);
-/* host psi (concatenated [nl][N] complex128 per atom) and per-atom times */
+/* host psi (concatenated [nl][N] complex128 per atom) and per-atom times; waits for the
+ * device first (a step or replay on any stream finishes before the state is replaced) */
 int pp_batch_set_state(pp_batch* b, const pp_cplx* psi, const double* t);
 /* any of psi, t, err (per-atom error estimate E(m-1, 0)) may be NULL */
 int pp_batch_get_state(const pp_batch* b, pp_cplx* psi, double* t, pp_cplx* err);

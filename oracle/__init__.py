@@ -49,6 +49,8 @@ def lib():
         L = C.CDLL(str(LIB))
         P = C.c_void_p
         L.oracle_last_error.restype = C.c_char_p
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_get_threads.restype = C.c_int
         L.oracle_tpmul.argtypes = [C.c_int, P, P, P, C.c_double, P, P, P, P]
         L.oracle_overlap_factor.argtypes = [C.c_int, C.c_int, P, P]
         L.oracle_overlap_apply.argtypes = [C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P]
@@ -74,6 +76,17 @@ def lib():
         L.oracle_sys_inner_product.argtypes = [P, P, P, P]
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Host threads for the multiply, overlap passes and preconditioner solve (1 = the
+    reference's sequential loops).  Results are bitwise the same for every count: each
+    output element is summed by one thread in the sequential order."""
+    lib().oracle_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().oracle_get_threads())
 
 
 def _check(rc):

@@ -21,6 +21,10 @@
 //                            pyprop/propagator/CayleyPropagator.py:71-150,
 //                            pyprop/propagator/PamPropagator.py:50-75,
 //                            examples/tensor/helium_stabilization/preconditioner.py:113-198 (exact solve)
+//
+// oracle_set_threads(n) splits the multiply by rank-0 destination row, the overlap passes by
+// fibre and the preconditioner solve by channel; each output element keeps the sequential
+// summation order, so the results do not depend on n.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -51,6 +55,37 @@ static int guard(F&& f) {
   }
 }
 
+// Host threads for the multiply, the overlap passes and the preconditioner solve.  Work is
+// split so that every output element is computed by one thread in the sequential order,
+// so any thread count gives bitwise the same result as one thread.
+static int g_threads = 1;
+
+extern "C" void oracle_set_threads(int n) { g_threads = std::max(1, std::min(n, 256)); }
+extern "C" int oracle_get_threads() { return g_threads; }
+
+// f(part, nparts) on g_threads threads (inline when 1); the first exception is rethrown
+template <class F>
+static void run_parts(F&& f) {
+  const int nt = g_threads;
+  if (nt <= 1) {
+    f(0, 1);
+    return;
+  }
+  std::vector<std::thread> pool;
+  std::vector<std::string> errs(nt);
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&, t] {
+      try {
+        f(t, nt);
+      } catch (const std::exception& e) {
+        errs[t] = e.what();
+      }
+    });
+  for (auto& th : pool) th.join();
+  for (auto& e : errs)
+    if (!e.empty()) throw std::runtime_error(e);
+}
+
 // ------------------------------------------------------------------------------------
 // tensor potential multiply
 // ------------------------------------------------------------------------------------
@@ -66,6 +101,7 @@ struct TP {
   const int* sext;
   cd* dst;
   const int32_t* const* pairs;
+  int part = 0, nparts = 1;  // this thread computes the rank-0 destination rows == part (mod nparts)
 };
 
 // reference zgbmv('N', m=n, n, kl=b, ku=b, alpha, A, lda=2b+1, x, 1, beta=1, y, 1)
@@ -105,6 +141,7 @@ void loop(const TP& t, int r, bool conj, size_t dOff, size_t sOff, size_t pOff) 
   const int N = t.sext[r];
   const bool inner = (r == t.R - 1);
   auto rec = [&](int row, int col, int i, bool cj) {
+    if (r == 0 && t.nparts > 1 && row % t.nparts != t.part) return;
     if (inner) innermost_entry(t, cj, dOff * N, sOff * N, pOff * P, row, col, i);
     else loop(t, r + 1, cj, dOff * N + row, sOff * N + col, pOff * P + i);
   };
@@ -164,7 +201,16 @@ extern "C" int oracle_tpmul(int rank, const int* ids, const double* potential, c
   return guard([&] {
     TP t{rank, ids, reinterpret_cast<const cd*>(potential), pot_ext, scaling, reinterpret_cast<const cd*>(src),
          src_ext, reinterpret_cast<cd*>(dst), pairs};
-    loop(t, 0, false, 0, 0, 0);
+    if (rank < 2) {  // one fibre: the innermost BLAS call is not split
+      loop(t, 0, false, 0, 0, 0);
+      return;
+    }
+    run_parts([&](int part, int nparts) {
+      TP tt = t;
+      tt.part = part;
+      tt.nparts = nparts;
+      loop(tt, 0, false, 0, 0, 0);
+    });
   });
 }
 
@@ -246,8 +292,12 @@ extern "C" int oracle_overlap_apply(int n, int sd, const double* u, int op, int 
     for (int r = rank + 1; r < ndim; ++r) post *= ext[r];
     if (ext[rank] != n) throw std::runtime_error("overlap size mismatch");
     cd* x = reinterpret_cast<cd*>(data);
-    for (size_t p = 0; p < pre; ++p)
-      for (size_t q = 0; q < post; ++q) tb_fibre(n, sd, u, op, conj != 0, x + p * n * post + q, post);
+    run_parts([&](int part, int nparts) {
+      for (size_t f = part; f < pre * post; f += nparts) {
+        const size_t p = f / post, q = f % post;
+        tb_fibre(n, sd, u, op, conj != 0, x + p * n * post + q, post);
+      }
+    });
   });
 }
 
@@ -780,7 +830,8 @@ extern "C" int oracle_sys_set_preconditioner(void* p, const double* s1, const do
 
 static void sys_precond(OracleSystem* s, cd* psi) {
   const int n = s->pc_n1 * s->pc_n2, w = s->pc_w, ld = 2 * w + 1;
-  for (int c = 0; c < s->pc_nchan; ++c) {
+  run_parts([&](int part, int nparts) {
+  for (int c = part; c < s->pc_nchan; c += nparts) {
     const cd* L = s->pc_lu.data() + (size_t)c * n * ld;
     auto E = [&](int i, int j) { return L[(size_t)i * ld + w + j - i]; };
     cd* x = psi + (size_t)c * n;
@@ -791,6 +842,7 @@ static void sys_precond(OracleSystem* s, cd* psi) {
       x[i] /= E(i, i);
     }
   }
+  });
 }
 
 extern "C" int oracle_sys_precond_solve(void* p, double* psi) {

@@ -591,6 +591,9 @@ extern "C" int pp_batch_set_state(pp_batch* p, const pp_cplx* psi, const double*
   return guarded([&] {
     PP_REQUIRE(p && psi && t, "null argument");
     Batch& b = p->b;
+    // a step or graph replay may still run on a non-blocking stream: wait for it before
+    // overwriting the state it reads and writes
+    PP_HIP(hipDeviceSynchronize());
     PP_HIP(hipMemcpy(b.psi.ptr, psi, b.psi_total * sizeof(double2), hipMemcpyHostToDevice));
     PP_HIP(hipMemcpy(b.t.ptr, t, b.natoms * sizeof(double), hipMemcpyHostToDevice));
   });

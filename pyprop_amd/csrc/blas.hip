@@ -171,14 +171,19 @@ void block_update(long long n, int m, const double2* V, long long ld, const doub
   block_update_if(n, m, V, ld, h_dev, w, norm2_dev, red, nullptr, s);
 }
 
-void vec_dot(long long n, const double2* x, const double2* y, double2* out_dev, Reducer& red, hipStream_t s) {
-  // single inner product: one "row" of V
+void vec_dot_into(long long n, const double2* x, const double2* y, double2* out_dev, double2* partial,
+                  hipStream_t s) {
+  // single inner product: one "row" of V; partial holds 2 reduction_blocks(n) values
   const int blocks = reduction_blocks(n);
-  red.ensure(blocks, 2);
-  hipLaunchKernelGGL(block_dot_kernel, dim3(blocks), dim3(kRedThreads), 0, s, n, 1, x, n, y, red.partial.ptr, 2,
+  hipLaunchKernelGGL(block_dot_kernel, dim3(blocks), dim3(kRedThreads), 0, s, n, 1, x, n, y, partial, 2,
                      (const int*)nullptr);
-  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, red.partial.ptr, blocks, 2, 1, out_dev, (const int*)nullptr);
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, partial, blocks, 2, 1, out_dev, (const int*)nullptr);
   PP_HIP(hipGetLastError());
+}
+
+void vec_dot(long long n, const double2* x, const double2* y, double2* out_dev, Reducer& red, hipStream_t s) {
+  red.ensure(reduction_blocks(n), 2);
+  vec_dot_into(n, x, y, out_dev, red.partial.ptr, s);
 }
 
 void vec_norm2(long long n, const double2* x, double2* out_dev, Reducer& red, hipStream_t s) {

@@ -33,9 +33,9 @@ using cd = std::complex<double>;
 // ----------------------------------------------------------------------------------
 // representation
 // ----------------------------------------------------------------------------------
-void Repr::overlap(int op, bool conj, double2* psi, hipStream_t s) const {
+void Repr::overlap(int op, bool conj, double2* psi, hipStream_t s, double2* work) const {
   for (int r = 0; r < ndim; ++r)
-    if (ov[r]) overlap_apply(*ov[r], op, conj, ndim, extents.data(), r, psi, s);
+    if (ov[r]) overlap_apply(*ov[r], op, conj, ndim, extents.data(), r, psi, s, work);
 }
 
 // ----------------------------------------------------------------------------------

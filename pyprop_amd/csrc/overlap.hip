@@ -245,7 +245,7 @@ static void launch_window(int op, bool conj, const double* u, int n, long long p
 }
 
 void overlap_apply(const OverlapFactor& ov, int op, bool conj, int ndim, const int* extents, int rank,
-                   double2* data, hipStream_t s) {
+                   double2* data, hipStream_t s, double2* work) {
   PP_REQUIRE(rank >= 0 && rank < ndim, "rank out of range");
   PP_REQUIRE(extents[rank] == ov.n, "overlap size does not match the rank extent");
   long long pre = 1, post = 1;
@@ -258,9 +258,9 @@ void overlap_apply(const OverlapFactor& ov, int op, bool conj, int ndim, const i
   PP_REQUIRE(op >= 0 && op <= 3, "unknown overlap op");
   if (ov.sd >= 1 && ov.sd <= 8 && (op == PP_OVERLAP_MULTIPLY || op == PP_OVERLAP_SQRT_MULTIPLY)) {
     const long long total = pre * ov.n * post;
-    if (ov.work.count < (size_t)total) ov.work.resize(total);
+    if (!work && ov.work.count < (size_t)total) ov.work.resize(total);
     const int eb = ceil_div(total, 256);
-    double2* w = ov.work.ptr;
+    double2* w = work ? work : ov.work.ptr;
     auto pass = [&](int mode, const double2* in, double2* out) {
       switch (ov.sd * 2 + mode) {
 #define PP_EL(K) \

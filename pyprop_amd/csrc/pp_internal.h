@@ -88,6 +88,9 @@ void block_dot(long long n, int m, const double2* V, long long ld, const double2
 void block_update(long long n, int m, const double2* V, long long ld, const double2* h_dev,
                   double2* w, double2* norm2_dev, Reducer& red, hipStream_t s);
 void vec_dot(long long n, const double2* x, const double2* y, double2* out_dev, Reducer& red, hipStream_t s);
+// the same dot with caller-provided reduction scratch (2 reduction_blocks(n) values)
+void vec_dot_into(long long n, const double2* x, const double2* y, double2* out_dev, double2* partial,
+                  hipStream_t s);
 void vec_norm2(long long n, const double2* x, double2* out_dev, Reducer& red, hipStream_t s);
 void vec_axpy_dev(long long n, const double2* alpha_dev, bool negate, const double2* x, double2* y, hipStream_t s);
 void vec_combine(long long n, double2 a, const double2* x, double2 b, const double2* y, double2* out, hipStream_t s);
@@ -104,8 +107,10 @@ struct OverlapFactor {
   DeviceBuffer<double> u;        // same layout on device
   mutable DeviceBuffer<double2> work;  // out-of-place passes of the multiply ops (grown on demand)
 };
+// work: out-of-place buffer of the multiply ops (as many elements as data); NULL uses ov.work,
+// grown on demand
 void overlap_apply(const OverlapFactor& ov, int op, bool conj, int ndim, const int* extents, int rank,
-                   double2* data, hipStream_t s);
+                   double2* data, hipStream_t s, double2* work = nullptr);
 
 }  // namespace pp
 

@@ -28,7 +28,8 @@ struct Repr {
     for (int e : extents) s *= e;
     return s;
   }
-  void overlap(int op, bool conj, double2* psi, hipStream_t s) const;
+  // work: out-of-place buffer of size() elements for the multiply ops (NULL: each factor's own)
+  void overlap(int op, bool conj, double2* psi, hipStream_t s, double2* work = nullptr) const;
 };
 
 struct Hamiltonian {

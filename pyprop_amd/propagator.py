@@ -398,9 +398,13 @@ class GmresWrapper:
         self.n = n
         self.m = krylov_basis_size
 
-    def Solve(self, callback, rhs, sol):
-        """callback(src_tensor, dst_tensor) computes dst = A src on the device."""
-        import torch
+    def Solve(self, callback, rhs, sol, usePypropIntegration: bool = False):
+        """sol = A^-1 rhs; callback(src_tensor, dst_tensor) computes dst = A src on the
+        device (GmresWrapper::Solve, gmreswrapper.cpp:84-110).  Inner products are the
+        plain vector ones; the S-metric integration functor (usePypropIntegration True) is
+        not provided and raises."""
+        if usePypropIntegration:
+            raise NotImplementedError("GMRES with the S-metric integration functor is not provided")
         n = self.n
 
         def cb(ctx, pin, pout, stream):
@@ -438,7 +442,7 @@ class GmresWrapper:
 
 class PampWrapper:
     """pAMP on device vectors with a Python operator callback
-    (krylov_PampWrapper, pampwrapper.cpp:83-108)."""
+    (krylov_PampWrapper, pampwrapper.cpp:18-108)."""
 
     def __init__(self, n: int, krylov_basis_size: int = 20, pade_order: int = 6,
                  krylov_double_orthogonalization: bool = False):
@@ -448,18 +452,39 @@ class PampWrapper:
         self.h = h
         self.n = n
 
-    def AdvanceStep(self, callback, psi, dt):
+    def AdvanceStep(self, callback, psi, tempPsi, dt, t: float, usePypropIntegration: bool = False):
+        """psi <- exp(-i dt A) psi, A applied by callback(srcPsi, destPsi, dt, t)
+        (PampWrapper::AdvanceStep, pampwrapper.cpp:84-108).  As in ApplyOperator
+        (pampwrapper.cpp:18-46) the callback sees psi and tempPsi pointed at the Krylov
+        input and output vectors, the output zeroed first; psi and tempPsi are restored
+        afterwards.  psi / tempPsi: Wavefunction objects of the same representation.
+        The Krylov inner products are the plain vector ones (usePypropIntegration False,
+        the balanced-overlap form PamPropagator.py:50-55 uses); the S-metric
+        integration of the non-balanced form is not provided and raises."""
+        if usePypropIntegration:
+            raise NotImplementedError("pAMP with the S-metric integration functor is not provided; "
+                                      "propagate in the balanced-overlap form instead")
+        shape = tuple(psi.data.shape)
         n = self.n
+        if int(np.prod(shape)) != n or tuple(tempPsi.data.shape) != shape:
+            raise ValueError("psi and tempPsi must have %d elements of the same shape" % n)
 
         def cb(ctx, pin, pout, stream):
+            old_in, old_out = psi.data, tempPsi.data
             try:
-                callback(_tensor_view(pin, n), _tensor_view(pout, n))
+                src = _tensor_view(pin, n).view(shape)
+                dst = _tensor_view(pout, n).view(shape)
+                dst.zero_()
+                psi.data, tempPsi.data = src, dst
+                callback(psi, tempPsi, dt, t)
                 return 0
             except Exception:
                 return 1
+            finally:
+                psi.data, tempPsi.data = old_in, old_out
 
         fn = N.OP_FN(cb)
-        N.check(N.load().pp_pamp_propagate(self.h, fn, None, D.ptr(psi), N.cplx(dt), D.stream_handle()))
+        N.check(N.load().pp_pamp_propagate(self.h, fn, None, D.ptr(psi.data), N.cplx(dt), D.stream_handle()))
 
     def GetPropagationErrorEstimate(self) -> complex:
         e = N.PPCplx()

@@ -340,6 +340,59 @@ def helium_system(lmax: int = 5, L=(0, 1, 2), M: int = 0, order: int = 7,
     return sysm
 
 
+def _restrict_potential(p: TensorPotentialData, keep_map: dict, n_sub: int):
+    """p with its rank-0 (angular) pairs restricted to the channels in keep_map
+    (old index -> new index) and renumbered; None when no pair is left."""
+    g0 = p.geometries[0]
+    if g0.storage_id not in ("Simp", "Diag"):
+        raise ValueError("channel restriction needs a Simp or Diag angular rank, not %s" % g0.storage_id)
+    rows, cols = g0.pairs[:, 0], g0.pairs[:, 1]
+    sel = np.array([int(r) in keep_map and int(c) in keep_map for r, c in zip(rows, cols)], dtype=bool)
+    if not sel.any():
+        return None
+    new_pairs = np.array([(keep_map[int(r)], keep_map[int(c)]) for r, c in g0.pairs[sel]], dtype=np.int32).reshape(-1, 2)
+    if g0.storage_id == "Diag":
+        if not np.array_equal(new_pairs[:, 0], np.arange(n_sub)):
+            raise ValueError("a Diag rank must keep every channel of the subset")
+    g = Geometry(g0.storage_id, new_pairs, g0.use_grid, n_sub, g0.basis)
+    terms = None
+    if p.grid_terms:
+        terms = [GridTerm(t.kind, np.ascontiguousarray(t.coeffs[:, sel]), t.differentiation, t.factors, t.lams)
+                 for t in p.grid_terms]
+    q = TensorPotentialData(p.name, [g] + list(p.geometries[1:]), None, p.time_function,
+                            None if p.angular_coeffs is None else np.ascontiguousarray(p.angular_coeffs[:, sel]),
+                            p.radial_factors, terms)
+    if p.angular_coeffs is None:
+        q._data = np.ascontiguousarray(p.data[sel])
+    return q
+
+
+def restrict_channels(system: System, channels, field_free: bool = True) -> System:
+    """The same Hamiltonian on a subset of the rank-0 channels (for helium one total-L
+    block): every potential keeps the angular pairs whose row and column both lie in the
+    subset, renumbered in subset order; the radial data are shared, not recomputed.  With
+    field_free the time-dependent potentials are left out (the field-free Hamiltonian whose
+    bound states define ionisation, examples/tensor/helium_stabilization/example.py:567-570)."""
+    channels = [int(c) for c in channels]
+    if len(set(channels)) != len(channels) or not channels:
+        raise ValueError("channels must be distinct and non-empty")
+    keep_map = {c: i for i, c in enumerate(channels)}
+    n_sub = len(channels)
+    pots = []
+    for p in system.potentials:
+        if field_free and p.is_time_dependent:
+            continue
+        q = _restrict_potential(p, keep_map, n_sub)
+        if q is not None:
+            pots.append(q)
+    prec = [q for q in (_restrict_potential(p, keep_map, n_sub) for p in system.precond_potentials) if q is not None]
+    sub_channels = [system.channels[c] for c in channels] if system.channels else []
+    info = dict(system.info)
+    info["parent_channels"] = channels
+    return System(system.name, (n_sub,) + tuple(system.shape[1:]), system.bases, pots,
+                  None if field_free else system.pulse, sub_channels, prec, info)
+
+
 def hydrogenic_1s(basis: BSplineBasis, Z: float, l: int = 0) -> np.ndarray:
     """Lowest generalized eigenvector of the 1-D radial Hamiltonian
     -1/2 d2/dr2 + l(l+1)/2r^2 - Z/r in the B-spline basis, S-normalised."""

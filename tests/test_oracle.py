@@ -476,3 +476,46 @@ def test_preconditioner_exact_vs_dense(rng):
                 M[i1 * n1 + i2, j1 * n1 + j2] += 0.5j * dt * P[c, q1, q2]
         ref = np.linalg.solve(M, x[c].ravel())
         assert np.allclose(got[c].ravel(), ref, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("ids", [("Simp", "Simp", "BandNH"), ("Herm", "Band", "Diag"), ("Diag", "BandNH"),
+                                 ("Band", "Simp", "Dense")], ids=lambda c: "_".join(c))
+def test_threaded_multiply_is_bitwise_sequential(ids, rng):
+    """The multiply split over host threads by rank-0 destination row sums every element in
+    the sequential loop order, so it equals the one-thread result bit for bit."""
+    shape = {2: (7, 23), 3: (9, 6, 11)}[len(ids)]
+    pairs = [pairs_for(s, n, rng) for s, n in zip(ids, shape)]
+    V = crandn(rng, tuple(len(p) for p in pairs))
+    src = crandn(rng, shape)
+    outs = []
+    try:
+        for nt in (1, 3, 8):
+            oracle.set_threads(nt)
+            dst = crandn(np.random.default_rng(5), shape)
+            outs.append(oracle.tpmul(ids, V, 0.7, src, dst, pairs).copy())
+    finally:
+        oracle.set_threads(1)
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_threaded_cayley_step_is_bitwise_sequential(rng):
+    """Overlap passes (by fibre), preconditioner solve (by channel) and the multiply (by row)
+    on host threads: a preconditioned Cayley step equals the one-thread step bit for bit."""
+    sysm = small_helium()
+    s = oracle_system(sysm)
+    dt = 0.01
+    set_exact_preconditioner(s, sysm, dt)
+    psi = crandn(rng, sysm.shape)
+    scal = np.array([1.0, 1.0, 0.04])
+    res = []
+    try:
+        for nt in (1, 4):
+            oracle.set_threads(nt)
+            res.append((s.cayley_step(psi, dt, scal, m=30, tol=1e-12), s.overlap(1, False, psi),
+                        s.precond_solve(psi), s.multiply(psi, scal, mode=2)))
+    finally:
+        oracle.set_threads(1)
+    (a, b) = res
+    assert np.array_equal(a[0][0], b[0][0]) and a[0][2] == b[0][2]
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)
