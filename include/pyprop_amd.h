@@ -489,6 +489,11 @@ int pp_shift_invert_set_scalings(pp_shift_invert* si, const double* scalings);
 int pp_shift_invert_set_factored(pp_shift_invert* si, const pp_fham* f);
 int pp_shift_invert_solve(pp_shift_invert* si, const pp_cplx* src, pp_cplx* dst, pp_stream stream);
 int pp_shift_invert_apply(void* ctx, const pp_cplx* in, pp_cplx* out, pp_stream stream);
+/* negate != 0: pp_shift_invert_apply returns -(H - shift S)^-1 S.  With a shift below the
+ * spectrum the Ritz values -1 / (E - shift) are then all negative and pIRAM's wanted set
+ * (smallest real part) holds the lowest energies, lowest first (E = shift - 1 / theta).
+ * pp_shift_invert_solve is not affected. */
+int pp_shift_invert_set_negate(pp_shift_invert* si, int negate);
 int pp_shift_invert_stats(const pp_shift_invert* si, int* solves, int* operator_count, double* last_error);
 
 /* The Hamiltonian as a pp_operator_fn (ctx = the pp_ham_op*): mode 0 = H
@@ -521,6 +526,10 @@ int pp_mixed_create(pp_hamiltonian* ham, const pp_kron* k, int basis_size, doubl
                     double inner_rtol, int max_outer, pp_mixed** out);
 int pp_mixed_destroy(pp_mixed* m);
 int pp_mixed_set_learned(pp_mixed* m, pp_learned* l);
+/* where the learned correction acts (flexible GMRES allows a different preconditioner per
+ * direction): 0 = the first direction of the first correction (default), 1 = the first
+ * direction of every correction, 2 = every direction */
+int pp_mixed_set_learned_mode(pp_mixed* m, int mode);
 int pp_mixed_set_fallback(pp_mixed* m, pp_cayley* plain, int fallback_iterations);
 /* use the factored multiply for the FP64 residuals and the FP32 inner products (NULL detaches) */
 int pp_mixed_set_factored(pp_mixed* m, const pp_fham* f);

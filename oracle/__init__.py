@@ -56,8 +56,9 @@ def lib():
         L.oracle_overlap_apply.argtypes = [C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P]
         L.oracle_gmres_solve.argtypes = [C.c_int64, C.c_int, C.c_double, C.c_int, OP, P, P, P, P, P, P]
         L.oracle_pamp_propagate.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int, OP, P, P, C.c_double, C.c_double,
-                                            P, P, P]
+                                            P, P, P, C.c_int]
         L.oracle_pade_exp.argtypes = [C.c_int, P, C.c_int, P]
+        L.oracle_eig_exp.argtypes = [C.c_int, P, P]
         L.oracle_bspline_tables.argtypes = [C.c_int, P, C.c_int, C.c_int, P, C.c_int, P, P, P, P, P, P, P]
         L.oracle_bspline_represent.argtypes = [C.c_int, P, C.c_int, C.c_int, P, C.c_int, P, P, C.c_int, C.c_int,
                                                P, C.c_int, C.c_int64, C.c_int64, P, P]
@@ -72,7 +73,7 @@ def lib():
                                                     C.c_double, C.c_double, C.c_double, C.c_double]
         L.oracle_sys_precond_solve.argtypes = [P, P]
         L.oracle_sys_cayley_step.argtypes = [P, P, C.c_double, P, C.c_int, C.c_double, C.c_int, P, P]
-        L.oracle_sys_pamp_step.argtypes = [P, P, C.c_double, C.c_double, P, C.c_int, C.c_int, C.c_int, P]
+        L.oracle_sys_pamp_step.argtypes = [P, P, C.c_double, C.c_double, P, C.c_int, C.c_int, C.c_int, P, C.c_int]
         L.oracle_sys_inner_product.argtypes = [P, P, P, P]
         _lib = L
     return _lib
@@ -168,7 +169,9 @@ def gmres_solve(apply, rhs, m, tol, dorth=True):
     return sol, float(err[0]), int(ops[0]), elist
 
 
-def pamp_propagate(apply, v, m, dt, order=6, dorth=False):
+def pamp_propagate(apply, v, m, dt, order=6, dorth=False, method=0):
+    """method 0: Pade exponential of the Hessenberg matrix, 1: eigenvector exponential
+    (krylov_exponentiation_method, pampwrapper.cpp:56-59)."""
     n = v.size
     v = np.ascontiguousarray(v, dtype=np.complex128).copy()
     est = np.zeros(2)
@@ -177,8 +180,17 @@ def pamp_propagate(apply, v, m, dt, order=6, dorth=False):
     cb = _wrap_op(apply, n)
     dt = complex(dt)
     _check(lib().oracle_pamp_propagate(n, m, order, int(dorth), cb, None, _ptr(v), dt.real, dt.imag, _ptr(est),
-                                       _ptr(ops), _ptr(bd)))
+                                       _ptr(ops), _ptr(bd), int(method)))
     return v, complex(est[0], est[1]), int(ops[0]), bool(bd[0])
+
+
+def eig_exp(A):
+    """V diag(exp(w)) V^H for an upper Hessenberg A (unit-norm eigenvectors)."""
+    m = A.shape[0]
+    Af = np.asfortranarray(A, dtype=np.complex128)
+    out = np.zeros((m, m), dtype=np.complex128, order="F")
+    _check(lib().oracle_eig_exp(m, _ptr(Af), _ptr(out)))
+    return np.asarray(out)
 
 
 def pade_exp(A, order=6):
@@ -293,13 +305,13 @@ class System:
                                             _ptr(ops)))
         return psi, float(err[0]), int(ops[0])
 
-    def pamp_step(self, psi, dt, scalings, m=20, order=6, dorth=False):
+    def pamp_step(self, psi, dt, scalings, m=20, order=6, dorth=False, method=0):
         psi = np.ascontiguousarray(psi, dtype=np.complex128).copy()
         sc = np.asarray(scalings, dtype=np.float64)
         est = np.zeros(2)
         dt = complex(dt)
         _check(lib().oracle_sys_pamp_step(self.h, _ptr(psi), dt.real, dt.imag, _ptr(sc), m, order, int(dorth),
-                                          _ptr(est)))
+                                          _ptr(est), int(method)))
         return psi, complex(est[0], est[1])
 
     def inner_product(self, a, b):

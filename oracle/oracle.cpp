@@ -11,7 +11,8 @@
 //   oracle_overlap_factor    core/representation/overlapmatrix.cpp:9-37 (zpbtrf 'U', unblocked)
 //   oracle_overlap_apply     core/representation/overlapmatrix.cpp:111-229 (ztbmv/ztbsv loops)
 //   oracle_gmres_solve       core/krylov/gmres/gmres.h:260-505 (+ zgels by Householder QR, zlarfg convention)
-//   oracle_pamp_propagate    core/krylov/pamp/pamp/pamp.h:270-670 (Pade via zgetrf/zgetrs loops)
+//   oracle_pamp_propagate    core/krylov/pamp/pamp/pamp.h:270-670 (Pade via zgetrf/zgetrs loops;
+//                            EigenvectorExponential :567-608 via zgeev restated for Hessenberg input)
 //   oracle_bspline_*         core/transform/bspline/bspline.cpp:26-161 (value/derivative
 //                            recursions), :372-466 (BSplineGlobalOverlapIntegral),
 //                            core/tensorpotential/basis_bspline.h:10-47 (one rank of
@@ -566,6 +567,131 @@ void pade_exp(int m, std::vector<cd> A, int order, std::vector<cd>& out) {
     out.swap(T);
   }
 }
+// zgeev restated for the upper Hessenberg case: eigenvalues w and unit-norm eigenvectors
+// (column k of V, row-major V[i*m + k]) of the m x m upper Hessenberg matrix a (row-major).
+// Schur form by the single-shift QR algorithm (Wilkinson shift, an exceptional shift every 11th
+// iteration, Givens bulge chase), eigenvectors of the triangular factor by back substitution
+// (ztrevc, small divisors replaced by eps |T|), then v = Q y, normalised.
+void hess_eig(int m, std::vector<cd> a, std::vector<cd>& w, std::vector<cd>& V) {
+  auto A = [&](int i, int j) -> cd& { return a[(size_t)i * m + j]; };
+  std::vector<cd> Q((size_t)m * m, 0);
+  for (int i = 0; i < m; ++i) Q[(size_t)i * m + i] = 1;
+  const double eps = std::numeric_limits<double>::epsilon();
+  double anorm = 0;
+  for (auto& z : a) anorm = std::max(anorm, std::abs(z));
+  auto givens = [&](cd x, cd y, double& c, cd& sn) {
+    const double ax = std::abs(x), ay = std::abs(y), r = std::hypot(ax, ay);
+    if (r == 0) { c = 1; sn = 0; return; }
+    if (ax == 0) { c = 0; sn = std::conj(y) / ay; return; }
+    const cd al = x / ax;
+    c = ax / r;
+    sn = al * std::conj(y) / r;
+  };
+  // rows k, k+1 <- G rows, G = [c s; -conj(s) c]; columns k, k+1 <- columns G^H; Q <- Q G^H
+  auto rotate = [&](int k, double c, cd sn, int col0, int row1) {
+    for (int j = col0; j < m; ++j) {
+      const cd x = A(k, j), y = A(k + 1, j);
+      A(k, j) = c * x + sn * y;
+      A(k + 1, j) = -std::conj(sn) * x + c * y;
+    }
+    for (int i = 0; i <= row1; ++i) {
+      const cd x = A(i, k), y = A(i, k + 1);
+      A(i, k) = c * x + std::conj(sn) * y;
+      A(i, k + 1) = -sn * x + c * y;
+    }
+    for (int i = 0; i < m; ++i) {
+      cd& x = Q[(size_t)i * m + k];
+      cd& y = Q[(size_t)i * m + k + 1];
+      const cd qx = x, qy = y;
+      x = c * qx + std::conj(sn) * qy;
+      y = -sn * qx + c * qy;
+    }
+  };
+  int hi = m - 1, iter = 0, total = 0;
+  while (hi > 0) {
+    int lo = hi;
+    while (lo > 0) {
+      double sc = std::abs(A(lo - 1, lo - 1)) + std::abs(A(lo, lo));
+      if (sc == 0) sc = anorm;
+      if (std::abs(A(lo, lo - 1)) <= eps * sc) {
+        A(lo, lo - 1) = 0;
+        break;
+      }
+      --lo;
+    }
+    if (lo == hi) {
+      --hi;
+      iter = 0;
+      continue;
+    }
+    if (++total > 60 * m) throw std::runtime_error("hess_eig: QR iteration did not converge");
+    ++iter;
+    cd mu;
+    if (iter % 11 == 0) {  // exceptional shift
+      mu = A(hi, hi) + 0.75 * std::abs(A(hi, hi - 1));
+    } else {  // the eigenvalue of the trailing 2 x 2 block closer to its last diagonal entry
+      const cd p = A(hi - 1, hi - 1), q = A(hi - 1, hi), r = A(hi, hi - 1), t = A(hi, hi);
+      const cd half = 0.5 * (p - t);
+      const cd disc = std::sqrt(half * half + q * r);
+      const cd m1 = t + half + disc, m2 = t + half - disc;
+      mu = std::abs(m1 - t) < std::abs(m2 - t) ? m1 : m2;
+    }
+    for (int k = lo; k < hi; ++k) {
+      double c;
+      cd sn;
+      if (k == lo) givens(A(k, k) - mu, A(k + 1, k), c, sn);
+      else givens(A(k, k - 1), A(k + 1, k - 1), c, sn);
+      rotate(k, c, sn, k == lo ? lo : k - 1, std::min(hi, k + 2));
+      if (k > lo) A(k + 1, k - 1) = 0;
+    }
+  }
+  w.resize(m);
+  for (int i = 0; i < m; ++i) w[i] = A(i, i);
+  double tnorm = 0;
+  for (int i = 0; i < m; ++i)
+    for (int j = i; j < m; ++j) tnorm = std::max(tnorm, std::abs(A(i, j)));
+  const double smin = std::max(eps * tnorm, std::numeric_limits<double>::min());
+  V.assign((size_t)m * m, 0);
+  std::vector<cd> y(m);
+  for (int k = 0; k < m; ++k) {
+    std::fill(y.begin(), y.end(), cd(0));
+    y[k] = 1;
+    for (int j = k - 1; j >= 0; --j) {
+      cd acc = 0;
+      for (int i = j + 1; i <= k; ++i) acc += A(j, i) * y[i];
+      cd d = A(j, j) - w[k];
+      if (std::abs(d) < smin) d = smin;
+      y[j] = -acc / d;
+    }
+    double nrm = 0;
+    for (int i = 0; i < m; ++i) {
+      cd v = 0;
+      for (int j = 0; j <= k; ++j) v += Q[(size_t)i * m + j] * y[j];
+      V[(size_t)i * m + k] = v;
+      nrm += std::norm(v);
+    }
+    nrm = std::sqrt(nrm);
+    for (int i = 0; i < m; ++i) V[(size_t)i * m + k] /= nrm;
+  }
+}
+
+// EigenvectorExponential (pamp.h:567-608): exp(A) ~ V diag(exp(w)) V^H with the unit-norm
+// eigenvectors of A (exact for a normal A, as -i dt H_m is in exact arithmetic); A and out in
+// the Hessenberg storage of pamp_propagate (element (row, col) at [col * m + row])
+void eig_exp(int m, const std::vector<cd>& Acm, std::vector<cd>& out) {
+  std::vector<cd> a((size_t)m * m);
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c < m; ++c) a[(size_t)r * m + c] = Acm[(size_t)c * m + r];
+  std::vector<cd> w, V;
+  hess_eig(m, a, w, V);
+  out.assign((size_t)m * m, 0);
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c < m; ++c) {
+      cd acc = 0;
+      for (int k = 0; k < m; ++k) acc += V[(size_t)r * m + k] * std::exp(w[k]) * std::conj(V[(size_t)c * m + k]);
+      out[(size_t)c * m + r] = acc;
+    }
+}
 }  // namespace
 
 extern "C" int oracle_pade_exp(int m, const double* A, int order, double* out) {
@@ -576,8 +702,17 @@ extern "C" int oracle_pade_exp(int m, const double* A, int order, double* out) {
   });
 }
 
+// V diag(exp(w)) V^H of an upper Hessenberg A, both in column-major storage
+extern "C" int oracle_eig_exp(int m, const double* A, double* out) {
+  return guard([&] {
+    std::vector<cd> a(reinterpret_cast<const cd*>(A), reinterpret_cast<const cd*>(A) + (size_t)m * m), o;
+    eig_exp(m, a, o);
+    std::copy(o.begin(), o.end(), reinterpret_cast<cd*>(out));
+  });
+}
+
 static cd pamp_propagate(size_t N, int m, int order, bool dorth, oracle_op op, void* ctx, cd* v, cd dt,
-                         int* op_count, int* breakdown) {
+                         int* op_count, int* breakdown, int method = 0) {
   const double tol = std::sqrt(std::numeric_limits<double>::epsilon());
   const int maxOrth = 3;
   std::vector<cd> V((size_t)m * N, 0), r(v, v + N), tmp(N);
@@ -631,7 +766,8 @@ static cd pamp_propagate(size_t N, int m, int order, bool dorth, oracle_op op, v
   }
   for (auto& z : H) z *= -dt * cd(0, 1);
   std::vector<cd> E;
-  pade_exp(m, H, order, E);
+  if (method == 1) eig_exp(m, H, E);  // ExponentiationEigenvector (pamp.h:640-643)
+  else pade_exp(m, H, order, E);
   std::fill(v, v + N, cd(0));
   for (int i = 0; i < m; ++i) zaxpy(N, E[i], V.data() + (size_t)i * N, v);  // column 0
   zscal(N, norm, v);
@@ -641,10 +777,11 @@ static cd pamp_propagate(size_t N, int m, int order, bool dorth, oracle_op op, v
 }
 
 extern "C" int oracle_pamp_propagate(int64_t n, int m, int order, int dorth, oracle_op op, void* ctx, double* v,
-                                     double dt_re, double dt_im, double* err_est, int* ops, int* breakdown) {
+                                     double dt_re, double dt_im, double* err_est, int* ops, int* breakdown,
+                                     int method) {
   return guard([&] {
     cd e = pamp_propagate((size_t)n, m, order, dorth != 0, op, ctx, reinterpret_cast<cd*>(v), cd(dt_re, dt_im), ops,
-                          breakdown);
+                          breakdown, method);
     if (err_est) {
       err_est[0] = e.real();
       err_est[1] = e.imag();
@@ -886,13 +1023,13 @@ static int balanced_cb(void* ctx, const double* in, double* out) {
 }
 
 extern "C" int oracle_sys_pamp_step(void* p, double* psi, double dt_re, double dt_im, const double* scal, int m,
-                                    int order, int dorth, double* err_est) {
+                                    int order, int dorth, double* err_est, int method) {
   return guard([&] {
     auto* s = static_cast<OracleSystem*>(p);
     s->scal.assign(scal, scal + s->terms.size());
     cd* x = reinterpret_cast<cd*>(psi);
     sys_overlap(s, 2, false, x);
-    cd e = pamp_propagate(s->N, m, order, dorth != 0, &balanced_cb, s, x, cd(dt_re, dt_im), nullptr, nullptr);
+    cd e = pamp_propagate(s->N, m, order, dorth != 0, &balanced_cb, s, x, cd(dt_re, dt_im), nullptr, nullptr, method);
     sys_overlap(s, 3, false, x);
     if (err_est) {
       err_est[0] = e.real();

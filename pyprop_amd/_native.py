@@ -103,6 +103,7 @@ SIGNATURES = [
     ("pp_mixed_create", I, [P, P, I, D, D, I, P]),
     ("pp_mixed_destroy", I, [P]),
     ("pp_mixed_set_learned", I, [P, P]),
+    ("pp_mixed_set_learned_mode", I, [P, I]),
     ("pp_mixed_set_fallback", I, [P, P, I]),
     ("pp_mixed_step", I, [P, P, D, P, P]),
     ("pp_mixed_stats", I, [P, P, P]),
@@ -134,6 +135,7 @@ SIGNATURES = [
     ("pp_shift_invert_set_factored", I, [P, P]),
     ("pp_shift_invert_solve", I, [P, P, P, P]),
     ("pp_shift_invert_apply", I, [P, P, P, P]),
+    ("pp_shift_invert_set_negate", I, [P, I]),
     ("pp_shift_invert_stats", I, [P, P, P, P]),
     ("pp_ham_op_create", I, [P, P, I, P]),
     ("pp_ham_op_destroy", I, [P]),

@@ -387,6 +387,7 @@ struct ShiftInvert {
   Gmres gmres;
   DeviceBuffer<double2> tmp, rhs;
   int solves = 0, op_total = 0;
+  bool negate = false;  // the operator form (pp_shift_invert_apply) returns -(H - shift S)^-1 S
   ShiftInvert(Hamiltonian* h, const Precond* p, int m, double tol, bool dorth, double sh)
       : ham(h), pc(p), shift(sh), scalings(h->terms.size(), 1.0), gmres(h->n, m, tol, dorth) {
     tmp.resize(h->n);
@@ -577,7 +578,17 @@ extern "C" int pp_shift_invert_apply(void* ctx, const pp_cplx* in, pp_cplx* out,
   auto* si = static_cast<pp_shift_invert*>(ctx);
   return guarded([&] {
     PP_REQUIRE(si && in && out, "null argument");
-    si->si.solve(reinterpret_cast<const double2*>(in), reinterpret_cast<double2*>(out), (hipStream_t)stream);
+    double2* o = reinterpret_cast<double2*>(out);
+    si->si.solve(reinterpret_cast<const double2*>(in), o, (hipStream_t)stream);
+    if (si->si.negate)
+      vec_combine(si->si.ham->n, make_double2(-1.0, 0.0), o, make_double2(0.0, 0.0), nullptr, o, (hipStream_t)stream);
+  });
+}
+
+extern "C" int pp_shift_invert_set_negate(pp_shift_invert* si, int negate) {
+  return guarded([&] {
+    PP_REQUIRE(si, "null argument");
+    si->si.negate = negate != 0;
   });
 }
 

@@ -408,14 +408,216 @@ void Learned::adam(float lr, float b1, float b2, float eps, hipStream_t s) {
   refresh_bf16(s);
 }
 
+// ------------------------------------------------------------------------------------
+// Inference: one row per channel block (rows <= 48), weights streamed once
+// ------------------------------------------------------------------------------------
+// The training GEMM tiles 64 x 64 outputs and splits K; with a few dozen rows most of its
+// tile is padding and each weight tile is loaded for one row block.  Inference instead
+// gives each wave a 16-column strip of the output and all rows (three 16-row MFMA tiles),
+// so every weight element is read from HBM once and used for all rows.  The inputs of a
+// workgroup's K range are staged once in LDS, as BF16, for its four waves.  Rounding
+// points are those of the training forward pass: x = bf16(v_c / |v|), h1 and h2 in BF16,
+// y in FP32.
+constexpr int IMR = 48;        // rows covered (3 x 16)
+constexpr int IKC = 256;       // K range of one first-layer workgroup
+constexpr int IKMAX = 640;     // K of the hidden layers staged whole in LDS
+
+union BfPack {  // eight BF16 values as one MFMA operand
+  bf16x8 v;
+  uint16_t u[8];
+};
+
+// part[slice][m][n] = sum_{k in slice} bf16(v[m][k] / |v|) W1[n][k]   (v: rows x K floats)
+__global__ __launch_bounds__(256) void infer_l1_kernel(int M, int N, int K, const float* __restrict__ v,
+                                                       double* __restrict__ nrm,
+                                                       const uint16_t* __restrict__ w, float* __restrict__ part) {
+  __shared__ __align__(16) uint16_t As[IMR * (IKC + 8)];
+  double g2 = 0;  // the global norm, summed in the same order as block_pack_kernel
+  for (int c = 0; c < M; ++c) g2 += nrm[c] * nrm[c];
+  const double g = std::sqrt(g2);
+  const float inv = g > 0 ? (float)(1.0 / g) : 0.f;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) nrm[M] = g;  // read by the Kronecker combination
+  const int k0 = blockIdx.y * IKC;
+  const int kn = min(IKC, K - k0);
+  // stage bf16(v / |v|): each thread converts groups of 8 consecutive values (two 16-byte
+  // loads, all issued before any is used), 6 groups per thread
+  constexpr int GROUPS = IMR * IKC / 8 / 256;
+  float4 lo[GROUPS], hi[GROUPS];
+  const bool vec = (K & 3) == 0;
+#pragma unroll
+  for (int q = 0; q < GROUPS; ++q) {
+    const int gidx = t + q * 256, m = gidx / (IKC / 8), kk = (gidx % (IKC / 8)) * 8;
+    const float* src = v + (long long)m * K + k0 + kk;
+    if (m < M && kk + 8 <= kn && vec) {
+      lo[q] = *reinterpret_cast<const float4*>(src);
+      hi[q] = *reinterpret_cast<const float4*>(src + 4);
+    } else {
+      float tmp[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) tmp[e] = (m < M && kk + e < kn) ? src[e] : 0.f;
+      lo[q] = make_float4(tmp[0], tmp[1], tmp[2], tmp[3]);
+      hi[q] = make_float4(tmp[4], tmp[5], tmp[6], tmp[7]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < GROUPS; ++q) {
+    const int gidx = t + q * 256, m = gidx / (IKC / 8), kk = (gidx % (IKC / 8)) * 8;
+    BfPack pk;
+    pk.u[0] = f2bf(lo[q].x * inv);
+    pk.u[1] = f2bf(lo[q].y * inv);
+    pk.u[2] = f2bf(lo[q].z * inv);
+    pk.u[3] = f2bf(lo[q].w * inv);
+    pk.u[4] = f2bf(hi[q].x * inv);
+    pk.u[5] = f2bf(hi[q].y * inv);
+    pk.u[6] = f2bf(hi[q].z * inv);
+    pk.u[7] = f2bf(hi[q].w * inv);
+    *reinterpret_cast<bf16x8*>(&As[m * (IKC + 8) + kk]) = pk.v;
+  }
+  __syncthreads();
+  const int n0 = (blockIdx.x * 4 + wave) * 16;
+  if (n0 >= N) return;
+  const int col = n0 + (lane & 15);
+  f32x4 acc[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int ks = 0; ks < kn; ks += 32) {
+    const int kq = ks + (lane >> 4) * 8;
+    BfPack b;
+    if (col < N && kq + 8 <= kn && (((long long)col * K + k0 + kq) & 7) == 0) {
+      b.v = *reinterpret_cast<const bf16x8*>(w + (long long)col * K + k0 + kq);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) b.u[q] = (col < N && kq + q < kn) ? w[(long long)col * K + k0 + kq + q] : (uint16_t)0;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(&As[(r * 16 + (lane & 15)) * (IKC + 8) + kq]);
+      acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b.v, acc[r], 0, 0, 0);
+    }
+  }
+  float* out = part + (long long)blockIdx.y * M * N;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = r * 16 + (lane >> 4) * 4 + q;
+      if (row < M && col < N) out[(long long)row * N + col] = acc[r][q];
+    }
+}
+
+// h[m][n] = bf16(GELU(sum_slices part + b[n] + emb[m][n]))  (the training bias_act order)
+__global__ void infer_act1_kernel(int M, int N, int slices, const float* __restrict__ part,
+                                  const float* __restrict__ bias, const float* __restrict__ emb,
+                                  uint16_t* __restrict__ h) {
+  const int total = M * N;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+    const int n = idx % N;
+    float v = 0.f;
+#pragma unroll 8
+    for (int p = 0; p < slices; ++p) v += part[(long long)p * total + idx];
+    v += bias[n];
+    v += emb[idx];  // row m is channel m
+    h[idx] = f2bf(gelu(v));
+  }
+}
+
+// out[m][n] = act(sum_k A[m][k] W[n][k] + b[n]) for K <= IKMAX: A (BF16 rows x K) staged in
+// LDS, each wave one 16-column strip over the whole K.  GELU_BF16: bf16(GELU(.)) into hout,
+// else the FP32 value into fout.
+template <bool GELU_BF16>
+__global__ __launch_bounds__(256) void infer_layer_kernel(int M, int N, int K, const uint16_t* __restrict__ a,
+                                                          const uint16_t* __restrict__ w,
+                                                          const float* __restrict__ bias,
+                                                          uint16_t* __restrict__ hout, float* __restrict__ fout) {
+  __shared__ __align__(16) uint16_t As[IMR * (IKMAX + 8)];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int kp = (K + 31) / 32 * 32;
+  // stage A in 16-byte groups of 8 values, all loads of a thread issued before the stores
+  constexpr int GROUPS = IMR * IKMAX / 8 / 256;  // upper bound; groups past IMR x kp are skipped
+  BfPack g[GROUPS];
+  const int gpr = kp / 8;  // groups per row
+  const bool vec = (K & 7) == 0;
+#pragma unroll
+  for (int q = 0; q < GROUPS; ++q) {
+    const int gidx = t + q * 256, m = gidx / gpr, kk = (gidx % gpr) * 8;
+    if (gidx < IMR * gpr && m < M && kk + 8 <= K && vec) {
+      g[q].v = *reinterpret_cast<const bf16x8*>(a + (long long)m * K + kk);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) g[q].u[e] = (gidx < IMR * gpr && m < M && kk + e < K) ? a[(long long)m * K + kk + e]
+                                                                                   : (uint16_t)0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < GROUPS; ++q) {
+    const int gidx = t + q * 256, m = gidx / gpr, kk = (gidx % gpr) * 8;
+    if (gidx < IMR * gpr) *reinterpret_cast<bf16x8*>(&As[m * (IKMAX + 8) + kk]) = g[q].v;
+  }
+  __syncthreads();
+  const int n0 = (blockIdx.x * 4 + wave) * 16;
+  if (n0 >= N) return;
+  const int col = n0 + (lane & 15);
+  f32x4 acc[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int ks = 0; ks < K; ks += 32) {
+    const int kq = ks + (lane >> 4) * 8;
+    BfPack b;
+    if (col < N && kq + 8 <= K && (((long long)col * K + kq) & 7) == 0) {
+      b.v = *reinterpret_cast<const bf16x8*>(w + (long long)col * K + kq);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) b.u[q] = (col < N && kq + q < K) ? w[(long long)col * K + kq + q] : (uint16_t)0;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(&As[(r * 16 + (lane & 15)) * (IKMAX + 8) + kq]);
+      acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b.v, acc[r], 0, 0, 0);
+    }
+  }
+  if (col >= N) return;
+  const float bn = bias[col];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = r * 16 + (lane >> 4) * 4 + q;
+      if (row >= M) continue;
+      const float v = acc[r][q] + bn;
+      if (GELU_BF16) hout[(long long)row * N + col] = f2bf(gelu(v));
+      else fout[(long long)row * N + col] = v;
+    }
+}
+
+// y (rows = channels) from the block norms and v, with the streaming kernels above
+static void infer_streaming(Learned& L, const float2* in, hipStream_t s) {
+  const int M = L.nchan, H = L.hidden, I = L.in;
+  const int slices = (int)ceil_div(I, IKC);
+  PP_REQUIRE((size_t)slices * M * H <= L.part.count, "learned: partial buffer too small");
+  hipLaunchKernelGGL(infer_l1_kernel, dim3((unsigned)ceil_div(H, 64), slices), dim3(256), 0, s, M, H, I,
+                     reinterpret_cast<const float*>(in), L.nrm.ptr, L.w1h.ptr, L.part.ptr);
+  hipLaunchKernelGGL(infer_act1_kernel, dim3((unsigned)ceil_div((long long)M * H, 256)), dim3(256), 0, s, M, H, slices,
+                     L.part.ptr, L.params.ptr + L.o_b1, L.params.ptr + L.o_emb, L.h1h.ptr);
+  hipLaunchKernelGGL(infer_layer_kernel<true>, dim3((unsigned)ceil_div(H, 64)), dim3(256), 0, s, M, H, H, L.h1h.ptr,
+                     L.w2h.ptr, L.params.ptr + L.o_b2, L.h2h.ptr, (float*)nullptr);
+  hipLaunchKernelGGL(infer_layer_kernel<false>, dim3((unsigned)ceil_div(I, 64)), dim3(256), 0, s, M, I, H, L.h2h.ptr,
+                     L.w3h.ptr, L.params.ptr + L.o_b3, (uint16_t*)nullptr, L.y.ptr);
+  PP_HIP(hipGetLastError());
+}
+
 void learned_apply(Learned& L, const Kron& K, double2 a, const float2* in, float2* out, hipStream_t s) {
   PP_REQUIRE(K.nchan == L.nchan && K.n == L.n, "learned preconditioner and Kronecker shapes differ");
   const int nb = L.n * L.n;
   L.ensure(L.nchan);
   const long long n = (long long)L.nchan * nb;
   hipLaunchKernelGGL((block_norm_kernel<float2>), dim3(L.nchan), dim3(256), 0, s, nb, in, L.nrm.ptr);
-  hipLaunchKernelGGL(block_pack_kernel, dim3(grid_for_n(n)), dim3(256), 0, s, L.nchan, nb, in, L.nrm.ptr, L.xh.ptr);
-  L.forward(L.nchan, L.iota.ptr, s);
+  if (L.nchan <= IMR && L.hidden <= IKMAX && L.hidden % 8 == 0) {
+    infer_streaming(L, in, s);
+  } else {
+    hipLaunchKernelGGL(block_pack_kernel, dim3(grid_for_n(n)), dim3(256), 0, s, L.nchan, nb, in, L.nrm.ptr, L.xh.ptr);
+    L.forward(L.nchan, L.iota.ptr, s);
+  }
   // out = K^-1 (in + sigma |v| y): the combination is formed as the Kronecker solve loads its input
   K.apply32_add(a, in, reinterpret_cast<const float2*>(L.y.ptr), L.nrm.ptr + L.nchan, L.sigma, out, s);
   PP_HIP(hipGetLastError());

@@ -228,6 +228,14 @@ static int fused_blocks(long long n) {
   return (int)b;
 }
 
+// v <- v / sqrt(n2) with the squared norm on the device (a zero vector stays zero)
+__global__ void unit32_dev_kernel(long long n, const double2* __restrict__ n2, float2* __restrict__ v) {
+  const double q = n2->x;
+  const float sc = q > 0 ? (float)(1.0 / sqrt(q)) : 0.f;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    v[i] = make_float2(sc * v[i].x, sc * v[i].y);
+}
+
 __global__ void scale32_kernel(long long n, float s, const float2* __restrict__ x, float2* __restrict__ y) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     y[i] = make_float2(s * x[i].x, s * x[i].y);
@@ -476,6 +484,122 @@ __global__ __launch_bounds__(1024) void overlap_ss32_kernel(const float* __restr
   }
 }
 
+constexpr int kSSLoads = 8;  // input elements per thread staged by overlap_ss32_rows_kernel (x 256)
+
+// The same four passes with a channel spread over ceil(n1 / RB) workgroups: workgroup
+// (channel, block) owns r1 rows [r0, r0 + RB).  U1 couples row i to rows i..i+SD and U1^T
+// to rows i-SD..i, so the r1 passes of the owned rows need input rows r0-SD .. r0+RB+SD-1
+// (recomputed by the neighbours, no exchange); the r2 passes act within a row.  Each output
+// element is summed in the same order as in overlap_ss32_kernel, so the results are equal.
+template <int SD, int RB>
+__global__ __launch_bounds__(256) void overlap_ss32_rows_kernel(const float* __restrict__ gu1, int n1,
+                                                                const float* __restrict__ gu2, int n2,
+                                                                const float2* __restrict__ x, float2* __restrict__ y) {
+  extern __shared__ float2 ssbuf[];
+  constexpr int HR = RB + 2 * SD;  // rows held: owned rows and both halos
+  float2* A = ssbuf;               // input rows a0 .. a1-1 at A[(i - a0) * n2 + c]
+  float2* B = ssbuf + HR * n2;     // first pass, then the r2 passes
+  float* u1 = reinterpret_cast<float*>(ssbuf + 2 * HR * n2);
+  float* u2 = u1 + n1 * (SD + 1);
+  const int c = blockIdx.x, r0 = blockIdx.y * RB, r1 = min(n1, r0 + RB);
+  const int a0 = max(0, r0 - SD), a1 = min(n1, r1 + SD);  // input rows needed
+  const int b0 = a0, b1 = r1;                             // first-pass rows needed
+  const long long off = (long long)c * n1 * n2;
+  // all loads of a thread issued before its LDS stores (the host guarantees that the input
+  // rows fit in kSSLoads x 256 elements and each factor in 4 x 256)
+  const int na = (a1 - a0) * n2;
+  float2 av[kSSLoads];
+#pragma unroll
+  for (int q = 0; q < kSSLoads; ++q) {
+    const int e = threadIdx.x + q * 256;
+    av[q] = e < na ? x[off + (long long)a0 * n2 + e] : make_float2(0.f, 0.f);
+  }
+  float f1[4], f2[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = threadIdx.x + q * 256;
+    f1[q] = e < n1 * (SD + 1) ? gu1[e] : 0.f;
+    f2[q] = e < n2 * (SD + 1) ? gu2[e] : 0.f;
+  }
+#pragma unroll
+  for (int q = 0; q < kSSLoads; ++q) {
+    const int e = threadIdx.x + q * 256;
+    if (e < na) A[e] = av[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = threadIdx.x + q * 256;
+    if (e < n1 * (SD + 1)) u1[e] = f1[q];
+    if (e < n2 * (SD + 1)) u2[e] = f2[q];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < (b1 - b0) * n2; e += blockDim.x) {  // B = U1 A, rows b0 .. b1-1
+    const int i = b0 + e / n2, cc = e % n2;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) {
+      const int j = i + d;
+      if (j < n1) {
+        const float a = u1[j * (SD + 1) + SD - d];
+        const float2 v = A[(j - a0) * n2 + cc];
+        acc.x = fmaf(a, v.x, acc.x);
+        acc.y = fmaf(a, v.y, acc.y);
+      }
+    }
+    B[(i - b0) * n2 + cc] = acc;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < (r1 - r0) * n2; e += blockDim.x) {  // A = U1^T B, owned rows
+    const int i = r0 + e / n2, cc = e % n2;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) {
+      const int j = i - d;
+      if (j >= 0) {
+        const float a = u1[i * (SD + 1) + SD - d];
+        const float2 v = B[(j - b0) * n2 + cc];
+        acc.x = fmaf(a, v.x, acc.x);
+        acc.y = fmaf(a, v.y, acc.y);
+      }
+    }
+    A[(i - r0) * n2 + cc] = acc;  // owned row i now at A[i - r0] (rows read above stay in B)
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < (r1 - r0) * n2; e += blockDim.x) {  // B = U2 A along r2
+    const int i1 = e / n2, i = e - i1 * n2;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) {
+      const int j = i + d;
+      if (j < n2) {
+        const float a = u2[j * (SD + 1) + SD - d];
+        const float2 v = A[i1 * n2 + j];
+        acc.x = fmaf(a, v.x, acc.x);
+        acc.y = fmaf(a, v.y, acc.y);
+      }
+    }
+    B[e] = acc;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < (r1 - r0) * n2; e += blockDim.x) {  // y = U2^T B
+    const int i1 = e / n2, i = e - i1 * n2;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int d = 0; d <= SD; ++d) {
+      const int j = i - d;
+      if (j >= 0) {
+        const float a = u2[i * (SD + 1) + SD - d];
+        const float2 v = B[i1 * n2 + j];
+        acc.x = fmaf(a, v.x, acc.x);
+        acc.y = fmaf(a, v.y, acc.y);
+      }
+    }
+    y[off + (long long)(r0 + i1) * n2 + i] = acc;
+  }
+}
+
+constexpr int kSSRows = 8;  // r1 rows per workgroup of overlap_ss32_rows_kernel
+
 void MixedWork::overlap_multiply32(const Repr& r, const float2* in, float2* out, float2* scratch, hipStream_t s) {
   for (int rank = 0; rank < r.ndim; ++rank) {
     const OverlapFactor* ov = r.ov[rank];
@@ -492,8 +616,27 @@ void MixedWork::overlap_multiply32(const Repr& r, const float2* in, float2* out,
           65536 &&
       in != out) {
     const int n1 = r.extents[1], n2 = r.extents[2];
-    const size_t lds = 2 * (size_t)n1 * n2 * sizeof(float2) + (size_t)(n1 + n2) * (r.ov[1]->sd + 1) * sizeof(float);
-    switch (r.ov[1]->sd) {
+    const int sd = r.ov[1]->sd;
+    const size_t factors = (size_t)(n1 + n2) * (sd + 1) * sizeof(float);
+    const size_t lds_rows = 2 * (size_t)(kSSRows + 2 * sd) * n2 * sizeof(float2) + factors;
+    if (!ss_one_wg && lds_rows <= 65536 && (kSSRows + 2 * sd) * n2 <= kSSLoads * 256 &&
+        (size_t)std::max(n1, n2) * (sd + 1) <= 4 * 256) {  // several workgroups per channel
+      const dim3 grid(r.extents[0], ceil_div(n1, kSSRows));
+      switch (sd) {
+#define PP_SSR(K)                                                                                          \
+  case K:                                                                                                  \
+    hipLaunchKernelGGL((overlap_ss32_rows_kernel<K, kSSRows>), grid, dim3(256), lds_rows, s, u32[1].ptr, n1, \
+                       u32[2].ptr, n2, in, out);                                                           \
+    PP_HIP(hipGetLastError());                                                                             \
+    return;
+        PP_SSR(1) PP_SSR(2) PP_SSR(3) PP_SSR(4) PP_SSR(5) PP_SSR(6) PP_SSR(7) PP_SSR(8)
+#undef PP_SSR
+        default:
+          break;
+      }
+    }
+    const size_t lds = 2 * (size_t)n1 * n2 * sizeof(float2) + factors;
+    switch (sd) {
 #define PP_SS(K)                                                                                       \
   case K:                                                                                              \
     hipLaunchKernelGGL((overlap_ss32_kernel<K>), dim3(r.extents[0]), dim3(1024), lds, s, u32[1].ptr, n1, \
@@ -783,6 +926,203 @@ __global__ __launch_bounds__(1024) void kron_mfma_kernel(const float* __restrict
   }
 }
 
+// The same FP32 solve split over four workgroups per channel in two launches (the
+// one-workgroup form keeps all four stages of a channel on one CU, where the MFMA work of a
+// channel takes longer than the rest of the kernel):
+//   kron_split1_kernel  workgroup (c, t): rows 16t..16t+15 of G = (E1^T B E2) / d, B the
+//                       (scaled, corrected) input, via U = E1[:, strip]^T B; G to g32
+//   kron_split2_kernel  workgroup (c, t): rows 16t..16t+15 of X = E1 G E2^T, via V = E1[strip, :] G
+// Four waves per workgroup, wave w owns the 16-column tile w of each strip product.
+// Strip product of 16 x 64 (LDS, row stride KLD) by 64 x 64 (LDS): out(i, j) = sum_k
+// L(i, k) R(k, j) with L(i, k) = LT ? lm[k][i0 + i] : lm[i0 + i][k] and R(k, j) = RT ? rm[j][k] : rm[k][j]
+template <bool LT, bool RT>
+__device__ inline void kron_strip(const float* lm, int i0, const float* r_re, const float* r_im, bool r_complex,
+                                  const float* l_im, kf32x4& cre, kf32x4& cim) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int j = wave * 16 + lr;
+#pragma unroll 4
+  for (int k0 = 0; k0 < 64; k0 += 4) {
+    const int k = k0 + lk;
+    const float lv = LT ? lm[k * KLD + i0 + lr] : lm[(i0 + lr) * KLD + k];
+    const float rre = RT ? r_re[j * KLD + k] : r_re[k * KLD + j];
+    if (r_complex) {  // real left factor, complex right factor
+      const float rim = RT ? r_im[j * KLD + k] : r_im[k * KLD + j];
+      cre = __builtin_amdgcn_mfma_f32_16x16x4f32(lv, rre, cre, 0, 0, 0);
+      cim = __builtin_amdgcn_mfma_f32_16x16x4f32(lv, rim, cim, 0, 0, 0);
+    } else {  // complex left factor (lm re, l_im im), real right factor
+      const float liv = LT ? l_im[k * KLD + i0 + lr] : l_im[(i0 + lr) * KLD + k];
+      cre = __builtin_amdgcn_mfma_f32_16x16x4f32(lv, rre, cre, 0, 0, 0);
+      cim = __builtin_amdgcn_mfma_f32_16x16x4f32(liv, rre, cim, 0, 0, 0);
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void kron_split1_kernel(const float* __restrict__ E, const float* __restrict__ lam,
+                                                           const int* __restrict__ l1, const int* __restrict__ l2,
+                                                           int n, float2 a, const float2* in, float in_scale,
+                                                           const float2* __restrict__ add,
+                                                           const double* __restrict__ add_norm, float add_sigma,
+                                                           const double2* __restrict__ in_n2, float2* __restrict__ g) {
+  if (in_n2) {
+    const double q = in_n2->x;
+    in_scale = q > 0 ? (float)(1.0 / sqrt(q)) : 0.f;
+  }
+  extern __shared__ float kl[];
+  float* Bre = kl;                // 64 x KLD
+  float* Bim = Bre + 64 * KLD;
+  float* e1 = Bim + 64 * KLD;     // columns i0 .. i0+15 only
+  float* e2 = e1 + 64 * KLD;
+  float* Ure = e2 + 64 * KLD;     // 16 x KLD
+  float* Uim = Ure + 16 * KLD;
+  const int c = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
+  const int i0 = t * 16;
+  const int n2 = n * n;
+  const float* E1 = E + (long long)l1[c] * n2;
+  const float* E2 = E + (long long)l2[c] * n2;
+  const float2* xin = in + (long long)c * n2;
+  // every load of a thread is issued before the LDS stores: 4 of B, 4 of E2, 1 of the E1 strip
+  float2 bv[4];
+  float ev2[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int idx = tid + q * 1024, i = idx >> 6, j = idx & 63;
+    const bool ok = i < n && j < n;
+    bv[q] = ok ? xin[i * n + j] : make_float2(0.f, 0.f);
+    ev2[q] = ok ? E2[i * n + j] : 0.f;
+  }
+  const int sk = tid >> 4, si = i0 + (tid & 15);
+  const float ev1 = (sk < n && si < n) ? E1[sk * n + si] : 0.f;
+  const float as = add ? add_sigma * (float)*add_norm : 0.f;
+  float2 yv[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int idx = tid + q * 1024, i = idx >> 6, j = idx & 63;
+    yv[q] = (add && i < n && j < n) ? add[(long long)c * n2 + i * n + j] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int idx = tid + q * 1024, i = idx >> 6, j = idx & 63;
+    float2 v = make_float2(in_scale * bv[q].x, in_scale * bv[q].y);
+    if (add) {  // in + sigma |v| add (the learned correction)
+      v.x = fmaf(as, yv[q].x, v.x);
+      v.y = fmaf(as, yv[q].y, v.y);
+    }
+    Bre[i * KLD + j] = v.x;
+    Bim[i * KLD + j] = v.y;
+    e2[i * KLD + j] = ev2[q];
+  }
+  e1[sk * KLD + si] = ev1;
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6;
+  kf32x4 cre = {0.f, 0.f, 0.f, 0.f}, cim = {0.f, 0.f, 0.f, 0.f};
+  if (wave < 4) {
+    kron_strip<true, false>(e1, i0, Bre, Bim, true, nullptr, cre, cim);  // U = E1[:, strip]^T B
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = (lane >> 4) * 4 + r, j = wave * 16 + (lane & 15);
+      Ure[i * KLD + j] = cre[r];
+      Uim[i * KLD + j] = cim[r];
+    }
+  }
+  __syncthreads();
+  if (wave >= 4) return;
+  cre = kf32x4{0.f, 0.f, 0.f, 0.f};
+  cim = kf32x4{0.f, 0.f, 0.f, 0.f};
+  kron_strip<false, false>(Ure, 0, e2, nullptr, false, Uim, cre, cim);  // G strip = U E2
+  const float* lam1 = lam + (long long)l1[c] * n;
+  const float* lam2 = lam + (long long)l2[c] * n;
+  float2* gc = g + (long long)c * 64 * 64;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + (lane >> 4) * 4 + r, j = wave * 16 + (lane & 15);
+    float vr = 0.f, vi = 0.f;
+    if (i < n && j < n) {  // divide by d = 1 + a (lam1_i + lam2_j), as kron_mfma_stage
+      const float sl = lam1[i] + lam2[j];
+      const float dr = 1.f + a.x * sl, di = a.y * sl;
+      const float inv = 1.f / (dr * dr + di * di);
+      vr = (cre[r] * dr + cim[r] * di) * inv;
+      vi = (cim[r] * dr - cre[r] * di) * inv;
+    }
+    gc[i * 64 + j] = make_float2(vr, vi);
+  }
+}
+
+__global__ __launch_bounds__(1024) void kron_split2_kernel(const float* __restrict__ E, const int* __restrict__ l1,
+                                                           const int* __restrict__ l2, int n,
+                                                           const float2* __restrict__ g, float2* out, float2* in_wb,
+                                                           float in_scale, const double2* __restrict__ in_n2) {
+  extern __shared__ float kl[];
+  float* Gre = kl;                // 64 x KLD
+  float* Gim = Gre + 64 * KLD;
+  float* e1 = Gim + 64 * KLD;     // rows i0 .. i0+15 only
+  float* e2 = e1 + 64 * KLD;
+  float* Vre = e2 + 64 * KLD;     // 16 x KLD
+  float* Vim = Vre + 16 * KLD;
+  const int c = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
+  const int i0 = t * 16;
+  const int n2 = n * n;
+  const float* E1 = E + (long long)l1[c] * n2;
+  const float* E2 = E + (long long)l2[c] * n2;
+  const float2* gc = g + (long long)c * 64 * 64;
+  float2 gv[4];
+  float ev2[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int idx = tid + q * 1024, i = idx >> 6, j = idx & 63;
+    const bool ok = i < n && j < n;  // rows past the last strip are never written: read as 0
+    gv[q] = ok ? gc[idx] : make_float2(0.f, 0.f);
+    ev2[q] = ok ? E2[i * n + j] : 0.f;
+  }
+  const int si = i0 + (tid >> 6), sk = tid & 63;
+  const float ev1 = (si < n && sk < n) ? E1[si * n + sk] : 0.f;
+  float2* xb = in_wb ? in_wb + (long long)c * n2 : nullptr;
+  float2 wv = make_float2(0.f, 0.f);
+  const int wi = i0 + tid / 64, wj = tid % 64;  // this thread's element of the write-back strip
+  const bool wok = xb && wi < n && wj < n && wi < i0 + 16;
+  if (wok) wv = xb[wi * n + wj];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int idx = tid + q * 1024, i = idx >> 6, j = idx & 63;
+    Gre[i * KLD + j] = gv[q].x;
+    Gim[i * KLD + j] = gv[q].y;
+    e2[i * KLD + j] = ev2[q];
+  }
+  e1[si * KLD + sk] = ev1;
+  if (wok) {  // the scaled input, in place of in (rows of this strip; in does not alias out here)
+    if (in_n2) {
+      const double q = in_n2->x;
+      in_scale = q > 0 ? (float)(1.0 / sqrt(q)) : 0.f;
+    }
+    xb[wi * n + wj] = make_float2(in_scale * wv.x, in_scale * wv.y);
+  }
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6;
+  kf32x4 cre = {0.f, 0.f, 0.f, 0.f}, cim = {0.f, 0.f, 0.f, 0.f};
+  if (wave < 4) {
+    kron_strip<false, false>(e1, i0, Gre, Gim, true, nullptr, cre, cim);  // V = E1[strip, :] G
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = (lane >> 4) * 4 + r, j = wave * 16 + (lane & 15);
+      Vre[i * KLD + j] = cre[r];
+      Vim[i * KLD + j] = cim[r];
+    }
+  }
+  __syncthreads();
+  if (wave >= 4) return;
+  cre = kf32x4{0.f, 0.f, 0.f, 0.f};
+  cim = kf32x4{0.f, 0.f, 0.f, 0.f};
+  kron_strip<false, true>(Vre, 0, e2, nullptr, false, Vim, cre, cim);  // X strip = V E2^T
+  float2* x = out + (long long)c * n2;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + (lane >> 4) * 4 + r, j = wave * 16 + (lane & 15);
+    if (i < n && j < n) x[i * n + j] = make_float2(cre[r], cim[r]);
+  }
+}
+
+static size_t kron_lds_split() { return (4 * 64 + 2 * 16) * (size_t)KLD * sizeof(float); }
+
 static size_t kron_lds_mfma() { return 6 * 64 * (size_t)KLD * sizeof(float); }
 
 static size_t kron_lds64(int n) { return 2 * (size_t)n * n * sizeof(double2); }
@@ -798,6 +1138,10 @@ void Kron::setup_attributes() {
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kron_lds32(n)));
   PP_HIP(hipFuncSetAttribute((const void*)kron_mfma_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)kron_lds_mfma()));
+  PP_HIP(hipFuncSetAttribute((const void*)kron_split1_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)kron_lds_split()));
+  PP_HIP(hipFuncSetAttribute((const void*)kron_split2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)kron_lds_split()));
 }
 
 void kron_solve(const Kron& k, double2 a, double2* psi, hipStream_t s) { k.apply64(a, psi, s); }
@@ -830,6 +1174,17 @@ void Kron::apply32_add(double2 a, const float2* in, const float2* add, const dou
 
 void Kron::launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
                     const float2* add, const double* add_norm, float add_sigma, const double2* in_n2) const {
+  if (n <= 64 && split32 && (!wb || (wb != out && !add))) {
+    if (g32.count < (size_t)nchan * 64 * 64) g32.resize((size_t)nchan * 64 * 64);
+    const dim3 grid(nchan, (n + 15) / 16);
+    const float2 af = make_float2((float)a.x, (float)a.y);
+    hipLaunchKernelGGL(kron_split1_kernel, grid, dim3(1024), kron_lds_split(), s, E32.ptr, lam32.ptr, l1.ptr,
+                       l2.ptr, n, af, in, in_scale, add, add_norm, add_sigma, in_n2, g32.ptr);
+    hipLaunchKernelGGL(kron_split2_kernel, grid, dim3(1024), kron_lds_split(), s, E32.ptr, l1.ptr, l2.ptr, n,
+                       g32.ptr, out, wb, in_scale, in_n2);
+    PP_HIP(hipGetLastError());
+    return;
+  }
   if (n <= 64) {
     hipLaunchKernelGGL(kron_mfma_kernel, dim3(nchan), dim3(1024), kron_lds_mfma(), s, E32.ptr, lam32.ptr, l1.ptr,
                        l2.ptr, n, make_float2((float)a.x, (float)a.y), in, out, in_scale, wb, add, add_norm,
@@ -970,7 +1325,8 @@ void MixedWork::residual(const double2* b64_, double2* r, double2* out, double2*
 
 int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
                         const std::function<void(int, float2*, const double2*, float2*)>& apply_P, const float2* b,
-                        double rtol, int max_it, double* rel_out, hipStream_t s, double beta_in, const double2* base,
+                        double rtol, int max_it, double* rel_out, hipStream_t s, double beta_in,
+                        const double2* base,
                         double2* xout, const std::function<bool(double&)>& first_read, int pre_iters,
                         int* needed) {
   using cd = std::complex<double>;
@@ -1150,9 +1506,15 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   // z = P (in / |in|) with |in|^2 on the device (n2 = NULL: in is already normalised); in
   // is replaced by in / |in| (the normalised basis vector)
   auto apply_P32 = [&](int j, float2* in, const double2* n2, float2* out) {
-    // the learned correction acts on the first direction of the first correction, whose
-    // right-hand side r = -2 i a H psi is the kind of vector it was trained on
-    if (j == 0 && correction == 0 && learned && !n2) {
+    // learned_mode 0: the first direction of the first correction, whose right-hand side
+    // r = -2 i a H psi is the kind of vector the network was trained on; 1: the first
+    // direction of every correction; 2: every direction
+    const bool use = learned && (learned_mode == 2 || (j == 0 && (learned_mode == 1 || correction == 0)));
+    if (use) {
+      if (n2) {  // V_j is normalised here (the Kronecker path does it while loading)
+        hipLaunchKernelGGL(unit32_dev_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, n2, in);
+        PP_HIP(hipGetLastError());
+      }
       learned_apply(*learned, K, ka, in, out, s);
       ++stats.learned_applies;
     } else {
@@ -1402,6 +1764,13 @@ extern "C" int pp_mixed_set_fallback(pp_mixed* m, pp_cayley* plain, int fallback
     PP_REQUIRE(!plain || plain->c.n == m->c.ham->n, "fallback solver size does not match");
     m->c.fallback = plain ? &plain->c : nullptr;
     m->c.fallback_iterations = fallback_iterations;
+  });
+}
+
+extern "C" int pp_mixed_set_learned_mode(pp_mixed* m, int mode) {
+  return guarded([&] {
+    PP_REQUIRE(m && mode >= 0 && mode <= 2, "learned mode must be 0, 1 or 2");
+    m->c.learned_mode = mode;
   });
 }
 

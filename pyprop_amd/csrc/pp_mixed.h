@@ -22,6 +22,8 @@ struct Kron {
   DeviceBuffer<double> F64;          // F_l = S E_l (= E_l^-T), for the forward product
   DeviceBuffer<float> E32, lam32;
   DeviceBuffer<int> l1, l2;          // per channel
+  mutable DeviceBuffer<float2> g32;  // [c][64][64] middle product of the split FP32 solve
+  bool split32 = true;               // FP32 solve (n <= 64) as two launches, four workgroups per channel
   void setup_attributes();
   // psi <- K^-1 psi, a = complex factor of the one-electron part (i dt / 2)
   void apply64(double2 a, double2* psi, hipStream_t s) const;
@@ -80,6 +82,7 @@ struct MixedWork {
   void norm2_32(const float2* w, double2* out, hipStream_t s);  // out[0] = |w|^2
   // r = b - r, out[0] = |r|^2 (FP64; pinned out and device out_dev)
   void residual(const double2* b, double2* r, double2* out, double2* out_dev, hipStream_t s);
+  bool ss_one_wg = false;  // S x S with one workgroup per channel (else a block of r1 rows each)
   // out = S x S in (FP32, out-of-place; scratch has n elements)
   void overlap_multiply32(const Repr& r, const float2* in, float2* out, float2* scratch, hipStream_t s);
   std::vector<DeviceBuffer<float>> u32 = std::vector<DeviceBuffer<float>>(kMaxRank);
@@ -106,7 +109,9 @@ struct MixedCayley {
   Hamiltonian* ham = nullptr;
   const FHam* fham = nullptr;        // factored multiply for both precisions when set
   const Kron* kron = nullptr;        // FP32 base preconditioner (required)
-  Learned* learned = nullptr;        // optional correction on the first direction of each inner solve
+  Learned* learned = nullptr;        // optional correction of the Kronecker preconditioner
+  int learned_mode = 0;              // where it acts: 0 first direction of the first correction,
+                                     // 1 first direction of every correction, 2 every direction
   Cayley* fallback = nullptr;        // optional plain FP64 solver (exact block preconditioner)
   int m = 20;                        // inner FGMRES basis
   double tol_rel = 1e-10;            // outer stop: |b - A x| <= tol_rel |b|

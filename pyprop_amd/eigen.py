@@ -59,10 +59,13 @@ class GMRESShiftInvertSolver:
 
     def __init__(self, psi, base, shift: float, preconditioner=None, krylov_basis_size: int = 20,
                  krylov_tolerance: float = 1e-10, krylov_double_orthogonalization: bool = True, time: float = 0.0,
-                 factored=None):
+                 factored=None, negate: bool = False):
         """preconditioner: a RadialTwoElectronPreconditioner (set up here with Hamiltonian
         scaling 1 and overlap scaling -shift, GMRESShiftInvertSolver.py:44-46) or None.
-        factored: a FactoredHamiltonian to use for H (optional)."""
+        factored: a FactoredHamiltonian to use for H (optional).
+        negate: as a pIRAM operator, apply -(H - shift S)^-1 S (pp_shift_invert_set_negate):
+        with a shift below the spectrum, pIRAM's wanted set is then the lowest energies;
+        E = ShiftInvertEnergies(theta, shift, negate=True)."""
         self.psi = psi
         self.BasePropagator = base
         self.Shift = float(shift)
@@ -82,6 +85,9 @@ class GMRESShiftInvertSolver:
         self.factored = factored
         if factored is not None:
             N.check(N.load().pp_shift_invert_set_factored(self.h, factored.h))
+        self.negate = bool(negate)
+        if negate:
+            N.check(N.load().pp_shift_invert_set_negate(self.h, 1))
 
     def InverseIterations(self, srcPsi, destPsi, t=0.0, dt=0.0):
         """destPsi = (H - shift S)^-1 S srcPsi (GMRESShiftInvertSolver.py:100-126)."""
@@ -182,10 +188,11 @@ class PiramSolver:
             self.h = None
 
 
-def ShiftInvertEnergies(theta, shift: float):
-    """Eigenvalues E of (H, S) from the Ritz values theta of (H - shift S)^-1 S."""
+def ShiftInvertEnergies(theta, shift: float, negate: bool = False):
+    """Eigenvalues E of (H, S) from the Ritz values theta of (H - shift S)^-1 S (of its
+    negative when negate)."""
     theta = np.asarray(theta)
-    return shift + 1.0 / theta
+    return shift - 1.0 / theta if negate else shift + 1.0 / theta
 
 
 def ImaginaryTimePropagation(propagator, psi, timestep: float, steps: int, renormalize: bool = True, t0: float = 0.0):

@@ -320,9 +320,13 @@ class MixedCayleyPropagator:
     def __init__(self, psi, base, kron: KroneckerPreconditioner, learned: Optional[LearnedPreconditioner] = None,
                  fallback=None, fallback_iterations: int = 40, krylov_basis_size: int = 20,
                  tolerance: float = 1e-10, inner_tolerance: float = 1e-5, max_outer: int = 8, factored=None,
-                 device_mode: bool = False):
+                 device_mode: bool = False, learned_mode: int = 0):
         """device_mode: after the first converged step, solve the inner least-squares problems
-        on the device and synchronise once per step (pp_mixed_set_device_mode)."""
+        on the device and synchronise once per step (pp_mixed_set_device_mode).
+        learned_mode: the FGMRES directions the learned correction acts on: 0 the first of
+        the first correction, 1 the first of every correction, 2 all
+        (pp_mixed_set_learned_mode)."""
+        self.learned_mode = learned_mode
         self.psi = psi
         self.device_mode = device_mode
         self.factored = factored
@@ -342,6 +346,9 @@ class MixedCayleyPropagator:
         self.kron.Setup()
         if self.fallback is not None and self.fallback.h is None:
             self.fallback.SetupStep(dt)
+        if self.h is not None:  # a new step size: replace the solver
+            N.check(N.load().pp_mixed_destroy(self.h))
+            self.h = None
         h = C.c_void_p()
         N.check(N.load().pp_mixed_create(self.BasePropagator.ham, self.kron.h, self.basis_size, self.tol,
                                          self.inner_tol, self.max_outer, C.byref(h)))
@@ -353,6 +360,7 @@ class MixedCayleyPropagator:
         if self.factored is not None:
             N.check(N.load().pp_mixed_set_factored(h, self.factored.h))
         N.check(N.load().pp_mixed_set_device_mode(h, int(self.device_mode)))
+        N.check(N.load().pp_mixed_set_learned_mode(h, int(self.learned_mode)))
 
     def SetLearned(self, learned):
         self.learned = learned

@@ -5,9 +5,9 @@ states as examples/tensor/helium_stabilization/example.py:557-604 does for heliu
 Rank 2 (hydrogen-like (l, r)): the bound states come from a dense generalized eigensolve of
 each l block on the host (120 x 120).  Rank 3 (helium (channel, r1, r2)): per total L, the
 field-free Hamiltonian H0 + 1/r12 restricted to that L block is solved on the device by
-pIRAM with GMRES shift-invert at the ionisation threshold (the reference computes its
-bound-state files the same way, pyprop/solver/PiramSolver.py + GMRESShiftInvertSolver.py),
-and the projections are S-metric inner products on the device.
+pIRAM with GMRES shift-invert (the reference computes its bound-state files the same way,
+pyprop/solver/PiramSolver.py + GMRESShiftInvertSolver.py), and the projections are S-metric
+inner products on the device.
 """
 from __future__ import annotations
 
@@ -71,22 +71,30 @@ def helium_bound_states(system, threshold: float = -2.0, count: int = 8, max_cou
                         gmres_tolerance: float = 1e-12, factored: bool = False) -> List[BoundStateBlock]:
     """Every field-free bound state below `threshold` (the He+ 1s threshold, -2.0 a.u., in
     FindIonizationProbability, example.py:557) of each total-L block of a rank-3 helium
-    system.
+    system, by pIRAM on the shift-inverted operator with GMRES and the exact block-diagonal
+    preconditioner (PiramSolver / GMRESShiftInvertSolver as the reference computes its
+    bound-state files).
 
-    Per block: pIRAM on the shift-inverted operator (H - sigma S)^-1 S with sigma =
-    threshold, GMRES with the exact block-diagonal preconditioner.  Its Ritz values theta =
-    1 / (E - sigma) are negative exactly for E < sigma, and pIRAM's wanted values are those
-    with the smallest real part, so the wanted set fills with the states below the threshold
-    first.  `count` states are requested; while every converged wanted value is still
-    negative the request is doubled (up to max_count), so that none below the threshold is
-    missed."""
+    Per block, two stages, both with the shift below the block's spectrum, where H - sigma S
+    and every channel block of the preconditioner are positive definite (the device block
+    LU does not pivot):
+      1. the lowest energy E_min, from one eigenvalue at sigma0 = -Z^2 - 0.5 (below every
+         two-electron energy, each electron being bound by at most Z^2 / 2);
+      2. sigma = E_min - 0.1 and the negated operator -(H - sigma S)^-1 S, whose Ritz values
+         -1 / (E - sigma) put the lowest energies first in pIRAM's wanted set (smallest real
+         part).  `count` states are requested and the request doubled (up to max_count)
+         while every converged one still lies below the threshold.
+    (A shift at the threshold itself would make the states below it the only negative Ritz
+    values, but a request beyond their number then targets the cluster of high-energy
+    values near 0+, which the restarted Arnoldi process cannot resolve.)"""
     from . import device as D
-    from .eigen import GMRESShiftInvertSolver, PiramSolver
+    from .eigen import GMRESShiftInvertSolver, PiramSolver, ShiftInvertEnergies
     from .propagator import BasisPropagator, CombinedRepresentation, RadialTwoElectronPreconditioner, Wavefunction
     from .systems import restrict_channels
     if system.rank != 3 or not system.channels:
         raise ValueError("helium_bound_states needs a rank-3 coupled-channel system")
     torch = D.torch
+    Z = float(system.info.get("Z", 2.0))
     out = []
     for L in sorted(set(ch.L for ch in system.channels)):
         idx = np.array([c for c, ch in enumerate(system.channels) if ch.L == L], dtype=np.int64)
@@ -99,37 +107,41 @@ def helium_bound_states(system, threshold: float = -2.0, count: int = 8, max_cou
         if factored:
             from .factored import FactoredHamiltonian
             fh = FactoredHamiltonian(base)
-        si = GMRESShiftInvertSolver(psi, base, threshold, RadialTwoElectronPreconditioner(psi, sub),
-                                    krylov_basis_size=gmres_basis_size, krylov_tolerance=gmres_tolerance, factored=fh)
-        k = max(1, count)
-        while True:
-            m = min(128, max(2 * k + 10, 20))
-            solver = PiramSolver(psi, si, krylov_basis_size=m, krylov_eigenvalue_count=k,
+
+        def run(shift, k):
+            si = GMRESShiftInvertSolver(psi, base, shift, RadialTwoElectronPreconditioner(psi, sub),
+                                        krylov_basis_size=gmres_basis_size, krylov_tolerance=gmres_tolerance,
+                                        factored=fh, negate=True)
+            solver = PiramSolver(psi, si, krylov_basis_size=min(128, max(2 * k + 10, 20)), krylov_eigenvalue_count=k,
                                  krylov_tolerance=krylov_tolerance, krylov_max_iteration_count=max_restarts)
             solver.Solve()
             theta = solver.GetComplexEigenvalues()
             conv = solver.GetConvergenceEstimates() <= 0
-            below = theta.real < 0
-            if not np.all(below) or 2 * k > max_count:
+            if not np.all(conv):
+                raise RuntimeError("L = %d: pIRAM did not converge %d states at shift %.4f (%s)" %
+                                   (L, k, shift, solver.GetStatistics()))
+            return si, solver, ShiftInvertEnergies(theta, shift, negate=True).real
+
+        _, _, e0 = run(-Z * Z - 0.5, 1)
+        shift = float(e0[0]) - 0.1
+        k = max(1, count)
+        while True:
+            si, solver, energies = run(shift, k)
+            if np.any(energies >= threshold) or 2 * k > max_count:
                 break
             k *= 2
-        if np.all(below):
-            raise RuntimeError("L = %d: all %d wanted Ritz values lie below the threshold; raise max_count" % (L, k))
-        if not np.all(conv[below]):
-            raise RuntimeError("L = %d: pIRAM did not converge every state below the threshold (%s)" %
-                               (L, solver.GetStatistics()))
-        sel = np.nonzero(below)[0]
-        energies = (threshold + 1.0 / theta[sel]).real
-        order = np.argsort(energies)
-        sel, energies = sel[order], energies[order]
+        if np.all(energies < threshold):
+            raise RuntimeError("L = %d: all %d lowest states lie below the threshold; raise max_count" % (L, k))
+        sel = np.nonzero(energies < threshold)[0]
+        sel = sel[np.argsort(energies[sel])]
         vec = torch.empty((len(sel),) + tuple(sub.shape), dtype=torch.complex128, device=psi.data.device)
         tmp = Wavefunction(repr_)
         for j, i in enumerate(sel):
             solver.SetEigenvector(tmp, int(i), normalize=True)
             vec[j].copy_(tmp.data)
         gram = np.array([[repr_.InnerProduct(vec[i], vec[j]) for j in range(len(sel))] for i in range(len(sel))])
-        st = dict(solver.GetStatistics(), **si.GetStatistics(), requested=k)
-        out.append(BoundStateBlock(L, idx, energies, vec, gram, repr_, st))
+        st = dict(solver.GetStatistics(), **si.GetStatistics(), requested=k, shift=shift)
+        out.append(BoundStateBlock(L, idx, energies[sel], vec, gram, repr_, st))
     return out
 
 
@@ -139,9 +151,10 @@ def _helium_yield(psi, system, blocks) -> float:
     sum_i |c_i|^2 for S-orthonormal states (example.py:588-601)."""
     from . import device as D
     torch = D.torch
-    data = getattr(psi, "data", None)
-    if data is None:
-        data = D.to_device(np.ascontiguousarray(psi))
+    if isinstance(psi, np.ndarray):
+        data = D.to_device(np.ascontiguousarray(psi, dtype=np.complex128))
+    else:
+        data = getattr(psi, "data", psi)
     total = getattr(psi, "repr", None)
     if total is None:
         from .propagator import CombinedRepresentation

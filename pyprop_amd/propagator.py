@@ -308,6 +308,9 @@ class CayleyPropagator:
             self.Preconditioner.SetOverlapScaling(1.0)
             self.Preconditioner.Setup(self)
         kron = getattr(self.Preconditioner, "kind", None) == "kron"
+        if self.h is not None:  # a new step size: replace the solver
+            N.check(N.load().pp_cayley_destroy(self.h))
+            self.h = None
         h = C.c_void_p()
         N.check(N.load().pp_cayley_create(self.BasePropagator.ham,
                                           None if self.Preconditioner is None or kron else self.Preconditioner.h,

@@ -421,3 +421,20 @@ def helium_initial_state(system: System) -> np.ndarray:
     idx = [i for i, ch in enumerate(system.channels) if ch.l1 == 0 and ch.l2 == 0 and ch.L == 0]
     psi[idx[0]] = np.multiply.outer(c, c)
     return psi
+
+
+def helium_smooth_start(system: System, seed: int = 17, amplitude: float = 0.05) -> np.ndarray:
+    """The 1s^2 state plus, in every channel (l1, l2, L), a seeded complex multiple
+    (`amplitude` times a standard complex normal) of the product of the lowest l1 state of
+    charge 2 and the lowest l2 state of charge 1: a smooth start that populates all
+    channels (unnormalised)."""
+    rng = np.random.default_rng(seed)
+    basis = system.bases[1]
+    psi = helium_initial_state(system)
+    lmax = max(max(ch.l1, ch.l2) for ch in system.channels)
+    inner = [hydrogenic_1s(basis, 2.0, l)[0] for l in range(lmax + 1)]
+    outer = [hydrogenic_1s(basis, 1.0, l)[0] for l in range(lmax + 1)]
+    for c, ch in enumerate(system.channels):
+        a = amplitude * complex(rng.standard_normal(), rng.standard_normal())
+        psi[c] += a * np.multiply.outer(inner[ch.l1], outer[ch.l2])
+    return psi

@@ -166,9 +166,11 @@ def test_batch_1000_steps_vs_oracle(torch):
     b64 = BatchedAtoms(specs)
     b32 = BatchedAtoms(specs, fp32_matvec=True, packs=b64.packs)
     steps, dt = 1000, 0.01
-    for b in (b64, b32):
-        b.capture(dt, 10)
-        b.replay(steps // 10)
+    stream = torch.cuda.Stream()  # graph capture needs a non-default stream
+    with torch.cuda.stream(stream):
+        for b in (b64, b32):
+            b.capture(dt, 10)
+            b.replay(steps // 10)
     torch.cuda.synchronize()
     s64, t64, _ = b64.get_state()
     s32, _, _ = b32.get_state()

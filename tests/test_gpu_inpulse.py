@@ -182,16 +182,26 @@ def test_helium_in_pulse_learned_fp32_vs_oracle(helium):
 
 
 def test_hydrogen_in_pulse_observables_vs_oracle():
-    """1000 pAMP steps from the ground state at t = T/2 - 10 (field on): S-norm, energy and
-    ionisation yield (>= 1e-4 here) within 1e-9 of the oracle's, the wavefunction within
-    1e-5 (the unconverged-by-design m = 20 expansion amplifies rounding, see
-    test_gpu_observables)."""
+    """The hydrogen workload (8 x 120) from the ground state at t = T/2 - 10, 10 a.u. through
+    the peak of the field, where the ionisation yield grows to ~9e-3: S-norm, energy and
+    yield against the oracle's pAMP on the same steps.
+
+    Krylov 30 and dt = 0.002 (5000 steps): with the configured Krylov 20 and dt = 0.01 the
+    expansion is not converged inside the field (error estimates ~3e-5 per step), and the
+    trajectory is then set by rounding: on the oracle alone a 1e-15 change of the start moves
+    psi by 3.6e-3 and the yield from 1.48e-4 to 1.10e-4 after 1000 steps, while the converged
+    yield of this window is 8.9e-3.  At the converged settings the same perturbation moves psi
+    by ~4e-5 and the observables by ~2e-7; the bars are 10x the oracle's own sensitivity,
+    measured here from that perturbation, for psi and for norm, energy and yield (the latter
+    also at most 1e-5 absolute)."""
     import torch
     assert torch.cuda.is_available()
     from pyprop_amd.observables import bound_states, ionization_yield
     from pyprop_amd.propagator import BasisPropagator, CombinedRepresentation, PamPropagator, Wavefunction
     from pyprop_amd.systems import hydrogen_system
-    say("in-pulse hydrogen: 1000 pAMP steps from T/2 - 10, norm / energy / yield vs oracle")
+    steps, dt, m = 5000, 0.002, 30
+    say("in-pulse hydrogen: %d pAMP steps (m = %d, dt = %g) from T/2 - 10, norm / energy / yield vs oracle" %
+        (steps, m, dt))
     sysm = hydrogen_system()
     bound = bound_states(sysm)
     x0 = np.zeros(sysm.shape, complex)
@@ -199,28 +209,38 @@ def test_hydrogen_in_pulse_observables_vs_oracle():
     repr_ = CombinedRepresentation(sysm.shape, sysm.bases)
     psi = Wavefunction(repr_, x0)
     base = BasisPropagator(psi, sysm.potentials)
-    prop = PamPropagator(psi, base, krylov_basis_size=20)
+    prop = PamPropagator(psi, base, krylov_basis_size=m)
     s = oracle_system(sysm)
-    ref = x0.copy()
-    t = 0.5 * sysm.pulse.pulse_duration - 10.0
-    for k in range(1000):
-        sc = base.scalings(t)
-        ref, _ = s.pamp_step(ref, DT, sc, m=20)
-        prop.AdvanceStep(t, DT)
-        t += DT
+    rng = np.random.default_rng(3)
+    pert = x0 + 1e-15 * np.linalg.norm(x0) * (rng.standard_normal(x0.shape) + 1j * rng.standard_normal(x0.shape)) \
+        / np.sqrt(2 * x0.size)
+    t0 = 0.5 * sysm.pulse.pulse_duration - 10.0
+    refs = []
+    for start in (x0, pert):
+        ref, t = start.copy(), t0
+        for k in range(steps):
+            ref, _ = s.pamp_step(ref, dt, base.scalings(t), m=m)
+            t += dt
+        refs.append(ref)
+    ref, ref2 = refs
+    t = t0
+    for k in range(steps):
+        prop.AdvanceStep(t, dt)
+        t += dt
     got = psi.numpy()
     S = sysm.bases[1].overlap_band()
-    norm_g = float(np.vdot(got, got @ S.T).real)
-    norm_r = float(np.vdot(ref, ref @ S.T).real)
-    e_g = base.GetEnergyExpectationValue(psi, t)
-    e_r = base.GetEnergyExpectationValue(Wavefunction(repr_, ref), t)
-    y_g = ionization_yield(got, sysm, bound)
-    y_r = ionization_yield(ref, sysm, bound)
+
+    def obs(x):
+        w = Wavefunction(repr_, x)
+        return (float(np.vdot(x, x @ S.T).real), base.GetEnergyExpectationValue(w, t), ionization_yield(x, sysm, bound))
+
+    og, o1, o2 = obs(got), obs(ref), obs(ref2)
+    sens = rel(ref2, ref)
     diff = rel(got, ref)
-    say("  norm %.12f / %.12f, energy %.10f / %.10f, yield %.6e / %.6e, |psi diff| %.2e" %
-        (norm_g, norm_r, e_g, e_r, y_g, y_r, diff))
-    assert y_r > 1e-4
-    assert abs(norm_g - norm_r) < 1e-9
-    assert abs(e_g - e_r) < 1e-9
-    assert abs(y_g - y_r) < 1e-9
-    assert diff < 1e-5
+    say("  GPU    norm %.12f energy %.10f yield %.8e" % og)
+    say("  oracle norm %.12f energy %.10f yield %.8e (perturbed start: %.12f %.10f %.8e)" % (o1 + o2))
+    say("  |psi - oracle| %.2e, oracle sensitivity %.2e" % (diff, sens))
+    assert o1[2] > 1e-3
+    for a, b, c in zip(og, o1, o2):
+        assert abs(a - b) < min(1e-5, max(10 * abs(c - b), 1e-9)), (a, b, c)
+    assert diff < max(10 * sens, 1e-10)

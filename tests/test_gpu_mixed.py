@@ -277,17 +277,24 @@ def test_mixed_cayley_fallback_to_plain(torch, rng):
     _check_solution(psi.numpy(), sol, A, b, 1e-11, 1e-8)
 
 
-def test_learned_apply_is_kron_plus_network(torch, rng):
+@pytest.mark.parametrize("size", ["small", "helium"])
+def test_learned_apply_is_kron_plus_network(torch, rng, size):
+    """The preconditioner's inference (the streaming kernels: one 16-column strip per wave
+    over all channel rows) against the training forward pass (tiled GEMM) fed the same
+    BF16 inputs, then the Kronecker solve; small helium (15 channels, hidden 64) and the
+    helium workload's shape (42 channels, 60 x 60 blocks, hidden 512)."""
     from pyprop_amd.mixed import KroneckerPreconditioner, LearnedPreconditioner
-    say("learned apply = K^-1 (v + sigma |v| net(v_c/|v|))")
-    sysm = _small_helium()
+    from pyprop_amd.systems import helium_system
+    say("learned apply = K^-1 (v + sigma |v| net(v_c/|v|)), %s" % size)
+    sysm = _small_helium() if size == "small" else helium_system()
+    hidden = 64 if size == "small" else 512
     repr_, psi, base = _device(sysm, torch)
     kp = KroneckerPreconditioner(psi, sysm)
-    lp = LearnedPreconditioner(kp, hidden=64, seed=3)
+    lp = LearnedPreconditioner(kp, hidden=hidden, seed=3)
     nc, n, _ = sysm.shape
     inp = 2 * n * n
     flat = lp.get_params().astype(np.float64)
-    lay, _ = mlp.layout(inp, 64, nc)
+    lay, _ = mlp.layout(inp, hidden, nc)
     o, shp = lay["W3"]
     flat[o:o + int(np.prod(shp))] = rng.standard_normal(int(np.prod(shp))) * 0.1
     lp.set_params(flat.astype(np.float32))
@@ -307,7 +314,11 @@ def test_learned_apply_is_kron_plus_network(torch, rng):
     kp.SetHamiltonianScaling(0.5j * dt)
     kp.Solve(kv)
     ref = kv.cpu().numpy()
-    assert rel(dst.cpu().numpy(), ref) < 1e-5
+    # the two products sum in different orders (FP32); an h1 or h2 value on a BF16 rounding
+    # boundary can round the other way, which moves y by ~1e-5 of its norm
+    e = rel(dst.cpu().numpy(), ref)
+    say("  |apply - (Kron + forward)| %.2e" % e)
+    assert e < 1e-4
 
 
 def _one_direction_residual(torch, base, kp, lp, r, dt, sc):

@@ -39,6 +39,24 @@ def test_python_binding_covers_header():
 
 
 @pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def dropin_functions():
+    text = (ROOT / "include" / "pyprop_amd_dropin.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*void\s+(tensorpotentialmultiply_[a-z_]+_)\s*\(", text, flags=re.M)))
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_library_exports_the_dropin_symbols():
+    """The generated kernels' names (generator.py:1428-1451) for the storage combinations
+    of the helium and hydrogen examples."""
+    names = dropin_functions()
+    assert names == ["tensorpotentialmultiply_diag_bandnh_", "tensorpotentialmultiply_simp_bandnh_",
+                     "tensorpotentialmultiply_simp_simp_bandnh_"]
+    lib = ctypes.CDLL(str(LIB))
+    assert all(hasattr(lib, n) for n in names)
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
 def test_version_and_error_strings():
     from pyprop_amd import _native
     lib = _native.load()

@@ -519,3 +519,45 @@ def test_threaded_cayley_step_is_bitwise_sequential(rng):
     assert np.array_equal(a[0][0], b[0][0]) and a[0][2] == b[0][2]
     for x, y in zip(a[1:], b[1:]):
         assert np.array_equal(x, y)
+
+
+def test_eig_exp_normal_is_expm(rng):
+    """EigenvectorExponential (pamp.h:567-608) forms V diag(exp(w)) V^H.  For the normal
+    matrices pAMP meets in exact arithmetic (A = -i dt T, T Hermitian tridiagonal, the
+    Hessenberg matrix of a Hermitian operator) that is exp(A)."""
+    for m in (1, 2, 7, 20):
+        d = rng.standard_normal(m)
+        e = rng.standard_normal(m - 1) + 1j * rng.standard_normal(m - 1)
+        T = np.diag(d).astype(complex) + np.diag(e, -1) + np.diag(e.conj(), 1)
+        A = -0.3j * T
+        assert np.allclose(oracle.eig_exp(A), scipy.linalg.expm(A), rtol=0, atol=1e-12)
+
+
+def test_eig_exp_non_normal_matches_the_reference_formula(rng):
+    """On a non-normal Hessenberg matrix the reference's V E V^H is not exp(A); the oracle
+    must reproduce the formula itself: against numpy's eig (LAPACK zgeev, unit-norm
+    eigenvectors; the formula does not depend on their phases)."""
+    m = 12
+    A = np.triu(rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m)), -1) * 0.4
+    w, V = np.linalg.eig(A)
+    ref = V @ np.diag(np.exp(w)) @ V.conj().T
+    got = oracle.eig_exp(A)
+    assert np.allclose(got, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+    assert not np.allclose(got, scipy.linalg.expm(A), atol=1e-3)
+
+
+def test_pamp_eigenvector_method_vs_expm(rng):
+    """pAMP with krylov_exponentiation_method = 1 on a Hermitian operator: the same
+    propagation as the Pade method and as dense expm (the Hessenberg matrix is Hermitian up
+    to rounding, so V E V^H = exp)."""
+    n = 40
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = 0.5 * (X + X.conj().T) / np.sqrt(n)
+    v = crandn(rng, n)
+    dt = 0.05
+    ref = scipy.linalg.expm(-1j * dt * H) @ v
+    out_p, est_p, _, _ = oracle.pamp_propagate(lambda x: H @ x, v, 16, dt, method=0)
+    out_e, est_e, _, _ = oracle.pamp_propagate(lambda x: H @ x, v, 16, dt, method=1)
+    assert np.linalg.norm(out_e - ref) < 1e-10 * np.linalg.norm(ref)
+    assert np.linalg.norm(out_e - out_p) < 1e-11 * np.linalg.norm(ref)
+    assert abs(est_e - est_p) < 1e-10
