@@ -213,6 +213,16 @@ int pp_repr_overlap(const pp_repr* repr, int op, int conj, pp_cplx* psi, pp_stre
  * `work` is a caller-provided device vector of the wavefunction size; result on host. */
 int pp_repr_inner_product(const pp_repr* repr, const pp_cplx* a, const pp_cplx* b,
                           pp_cplx* work, pp_cplx* result, pp_stream stream);
+/* The same inner product with no device allocation (no counterpart in the reference): for
+ * callers that cannot allocate (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Every intermediate lives in a fixed file-scope device array of
+ * 2 PP_NOALLOC_MAX_ELEMENTS + 1025 complex128 values (32 MiB) per device, held until the
+ * process ends; wavefunctions larger than PP_NOALLOC_MAX_ELEMENTS are rejected.  Calls that
+ * overlap in time on one device share that array and race: serialise them.  Same kernels
+ * and result as pp_repr_inner_product, bitwise. */
+#define PP_NOALLOC_MAX_ELEMENTS (1LL << 20)
+int pp_repr_inner_product_noalloc(const pp_repr* repr, const pp_cplx* a, const pp_cplx* b, pp_cplx* result,
+                                  pp_stream stream);
 
 /* ------------------------------------------------------------------------------
  * Hamiltonian: sum of tensor potentials (BasisPropagator,

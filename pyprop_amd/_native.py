@@ -57,6 +57,7 @@ SIGNATURES = [
     ("pp_repr_destroy", I, [P]),
     ("pp_repr_overlap", I, [P, I, I, P, P]),
     ("pp_repr_inner_product", I, [P, P, P, P, P, P]),
+    ("pp_repr_inner_product_noalloc", I, [P, P, P, P, P]),
     ("pp_ham_create", I, [P, I, P, P]),
     ("pp_ham_destroy", I, [P]),
     ("pp_ham_multiply_nooverlap", I, [P, P, P, P, P]),
