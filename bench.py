@@ -201,10 +201,13 @@ def factored_flops(fh, nc, n, precision):
     group, K step); the position variants ("pos<g>") one per (16 band terms, position,
     channel group, K step of the w x w window), channel groups rounded up to a multiple of
     g.  The separable terms run as two 1-D band passes on the vector units and are not
-    counted.  Angular stage: two MFMAs (re, im) per (16 channels, 16 positions, K step of 4),
-    each wave's share of the K steps rounded up to whole batches (12 steps, 4 in the
-    all-rows form).  At helium these are the SQ_INSTS_MFMA counts the hardware reports
-    (band tiles 1,487,808, position form 928,800, angular 259,200 per multiply)."""
+    counted.  Angular stage, GEMM forms: two MFMAs (re, im) per (16 channels, 16 positions,
+    K step of 4), each wave's share of the K steps rounded up to whole batches (12 steps, 4
+    in the all-rows form); the sparse forms run on the vector units (no MFMA).  At helium
+    these are the SQ_INSTS_MFMA counts the hardware reports (band tiles 1,487,808, position
+    form 928,800, GEMM angular 259,200 per multiply).  Useful flops: the radial band products
+    of every term and channel, and 4 real flops per (non-zero of the angular matrices,
+    position) in the angular contraction."""
     T, bw = fh.nterms, fh.bw
     w = 2 * bw + 1
     pairs1 = sum(min(n, i + bw + 1) - max(0, i - bw) for i in range(n))
@@ -217,15 +220,18 @@ def factored_flops(fh, nc, n, precision):
         ks = (16 + 2 * bw + 3) // 4
         rad_mfma = fh.band_terms * pairs1 * ks * math.ceil(n / 16) * groups
     av = fh.ANGULAR_VARIANTS[fh.angular_variant(precision)]
-    waves = int(av.replace("rows", ""))
-    batch = 4 if av.startswith("rows") else 12
-    nks = (T * nc + 3) // 4
-    per = math.ceil(nks / waves)
-    ang_steps = sum(math.ceil(max(0, min(nks, k0 + per) - k0) / batch) * batch
-                    for k0 in range(0, waves * per, per))
-    ang_mfma = math.ceil(n * n / 16) * math.ceil(nc / 16) * ang_steps * 2
+    if av.startswith("sparse"):
+        ang_mfma = 0
+    else:
+        waves = int(av.replace("rows", ""))
+        batch = 4 if av.startswith("rows") else 12
+        nks = (T * nc + 3) // 4
+        per = math.ceil(nks / waves)
+        ang_steps = sum(math.ceil(max(0, min(nks, k0 + per) - k0) / batch) * batch
+                        for k0 in range(0, waves * per, per))
+        ang_mfma = math.ceil(n * n / 16) * math.ceil(nc / 16) * ang_steps * 2
     nnz_r = pairs1 * pairs1
-    useful = T * nnz_r * 4 * nc + 2 * nc * T * nc * 2 * n * n
+    useful = T * nnz_r * 4 * nc + 4 * fh.angular_nnz * n * n
     return (rad_mfma + ang_mfma) * 2048, useful
 
 

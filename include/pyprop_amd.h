@@ -284,6 +284,10 @@ typedef struct pp_pamp pp_pamp;
 int pp_pamp_create(int64_t n, int basis_size, int pade_order, int double_orthogonalization,
                    pp_pamp** out);
 int pp_pamp_destroy(pp_pamp* p);
+/* exponentiation of the Hessenberg matrix (krylov_exponentiation_method, pampwrapper.cpp:56-59):
+   0 = Pade (the default, on the device), 1 = eigenvector exponential (pamp.h:567-608: the
+   eigen-decomposition of the m x m matrix on the host, one synchronisation per step). */
+int pp_pamp_set_exponentiation(pp_pamp* p, int method);
 /* v <- exp(-i dt A) v (pAMP::PropagateVector, pamp.h:613-670).  Must not be given an
  * operator that synchronises the stream if the caller wants to capture the step. */
 int pp_pamp_propagate(pp_pamp* p, pp_operator_fn op, void* ctx, pp_cplx* v, pp_cplx dt,

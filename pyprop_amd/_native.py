@@ -72,6 +72,7 @@ SIGNATURES = [
     ("pp_gmres_error_list", I, [P, P]),
     ("pp_pamp_create", I, [I64, I, I, I, P]),
     ("pp_pamp_destroy", I, [P]),
+    ("pp_pamp_set_exponentiation", I, [P, I]),
     ("pp_pamp_propagate", I, [P, OP_FN, P, P, PPCplx, P]),
     ("pp_pamp_stats", I, [P, P, P, P]),
     ("pp_precond_create", I, [I, I, I, I, P, P, P, I, P, I, P, PPCplx, PPCplx, P]),

@@ -189,6 +189,13 @@ extern "C" int pp_pamp_destroy(pp_pamp* p) {
   return guarded([&] { delete p; });
 }
 
+extern "C" int pp_pamp_set_exponentiation(pp_pamp* p, int method) {
+  return guarded([&] {
+    PP_REQUIRE(p && (method == 0 || method == 1), "exponentiation method must be 0 (Pade) or 1 (eigenvector)");
+    p->p.method = method;
+  });
+}
+
 extern "C" int pp_pamp_propagate(pp_pamp* p, pp_operator_fn op, void* ctx, pp_cplx* v, pp_cplx dt,
                                  pp_stream stream) {
   return guarded([&] {

@@ -421,8 +421,10 @@ constexpr bool kRadPipe[kFhamRadialVariants] = {false, false, false, false, true
 constexpr int kRadPosMG[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 6, 3, 2};
 // angular variants: waves splitting K, and whether one workgroup covers all row blocks of
 // its position tile (reading each Z element once) or one row block
-constexpr int kAngWaveChoices[kFhamAngularVariants] = {16, 8, 4, 8, 16};
-constexpr bool kAngAllRows[kFhamAngularVariants] = {false, false, false, true, true};
+constexpr int kAngWaveChoices[kFhamAngularVariants] = {16, 8, 4, 8, 16, 4, 4};
+constexpr bool kAngAllRows[kFhamAngularVariants] = {false, false, false, true, true, false, false};
+// sparse variants: positions per lane (0: a GEMM form above)
+constexpr int kAngSparsePV[kFhamAngularVariants] = {0, 0, 0, 0, 0, 1, 2};
 constexpr int kAngMaxRowBlocks = 4;  // all-row variants: at most 64 channels
 
 template <class T, int B>
@@ -686,6 +688,122 @@ __global__ __launch_bounds__(64 * W) void fham_angular_rows_kernel(int nc, int K
   }
 }
 
+// Stage 2, sparse form.  The angular matrices are sparse: at helium 1269 of the
+// 18 x 42 x 42 entries are non-zero (the H0 pieces are diagonal in the channel, each
+// multipole and the dipole couple a few channels), so the GEMM above issues ~25 times the
+// useful products.  Here Y[c'][p] = sum over the non-zeros (t, c) of row c' of
+// f_t A_t[c'][c] Z[t][c][p].  One workgroup of kAngSparseWaves waves per (output channel,
+// chunk of 64 PV positions): the row's list -- Z row index t nc + c, coefficient, term --
+// is split into contiguous parts, one per wave, read through scalar loads (uniform in the
+// wave); lane l accumulates positions chunk + 64 r + l (r < PV), so every Z load is one
+// contiguous 64-element read, and a wave issues all loads of its part (at most
+// kAngSparseBatch entries per round) before it waits.  The parts are summed through LDS in
+// wave order (deterministic).  Bandwidth-bound: each Z row is read by the workgroups of the
+// output channels that use it; the workgroups of one position chunk run on one XCD
+// (renumbered grid) and share its L2.
+constexpr int kAngSparseWaves = 4;
+constexpr int kAngSparseBatch = 12;  // list entries whose loads a wave issues together
+template <class T, int PV>
+__global__ __launch_bounds__(64 * kAngSparseWaves) void fham_angular_sparse_kernel(
+    int nc, long long P, int nchunks, const int* __restrict__ rowptr, const int* __restrict__ colk,
+    const T* __restrict__ coef, const int* __restrict__ cterm, const typename MfmaT<T>::C2* __restrict__ Z,
+    FScales<T> sc, const typename MfmaT<T>::C2* addend, typename MfmaT<T>::C2 alpha,
+    const typename MfmaT<T>::C2* __restrict__ zs, typename MfmaT<T>::C2 beta, typename MfmaT<T>::C2* out,
+    typename MfmaT<T>::C2* out2, typename MfmaT<T>::C2 alpha2, typename MfmaT<T>::C2 beta2, int* __restrict__ reset) {
+  typedef typename MfmaT<T>::C2 C2;
+  __shared__ T part_re[kAngSparseWaves - 1][PV][64], part_im[kAngSparseWaves - 1][PV][64];
+  const long long nblocks = (long long)nc * nchunks;
+  const long long per_xcd = (nblocks + 7) / 8;
+  const long long L = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (L >= nblocks) return;  // whole workgroups
+  if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;
+  const int lane = threadIdx.x & 63;
+  // wave-uniform indices (readfirstlane: the row list is then read through scalar loads)
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int chunk = (int)(L / nc), row = (int)(L - (L / nc) * nc);
+  const long long p0 = (long long)chunk * 64 * PV + lane;
+  const int r0 = rowptr[row], r1 = rowptr[row + 1];
+  const int per = (r1 - r0 + kAngSparseWaves - 1) / kAngSparseWaves;
+  const int j0 = min(r1, r0 + wave * per), j1 = min(r1, j0 + per);
+  T re[PV], im[PV];
+  bool ok[PV];
+#pragma unroll
+  for (int r = 0; r < PV; ++r) {
+    re[r] = 0;
+    im[r] = 0;
+    ok[r] = p0 + 64 * r < P;
+  }
+  for (int jb = j0; jb < j1; jb += kAngSparseBatch) {
+    C2 z[kAngSparseBatch][PV];
+    T a[kAngSparseBatch];
+#pragma unroll
+    for (int q = 0; q < kAngSparseBatch; ++q) {
+      const bool have = jb + q < j1;     // uniform
+      const int j = have ? jb + q : jb;  // past the part's end: an in-bounds entry, weight 0
+      const long long base = (long long)colk[j] * P;
+      a[q] = have ? coef[j] * sc.s[cterm[j]] : T(0);
+#pragma unroll
+      for (int r = 0; r < PV; ++r) {
+        if (have && ok[r]) z[q][r] = Z[base + p0 + 64 * r];
+        else {
+          z[q][r].x = 0;
+          z[q][r].y = 0;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kAngSparseBatch; ++q)
+#pragma unroll
+      for (int r = 0; r < PV; ++r) {
+        re[r] = fma(a[q], z[q][r].x, re[r]);
+        im[r] = fma(a[q], z[q][r].y, im[r]);
+      }
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int r = 0; r < PV; ++r) {
+      part_re[wave - 1][r][lane] = re[r];
+      part_im[wave - 1][r][lane] = im[r];
+    }
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int q = 0; q < kAngSparseWaves - 1; ++q)
+#pragma unroll
+    for (int r = 0; r < PV; ++r) {
+      re[r] += part_re[q][r][lane];
+      im[r] += part_im[q][r][lane];
+    }
+#pragma unroll
+  for (int r = 0; r < PV; ++r) {
+    if (!ok[r]) continue;
+    const long long idx = (long long)row * P + p0 + 64 * r;
+    C2 o;
+    if (addend) o = addend[idx];
+    else {
+      o.x = 0;
+      o.y = 0;
+    }
+    const T vr = re[r], vi = im[r];
+    o.x += alpha.x * vr - alpha.y * vi;
+    o.y += alpha.x * vi + alpha.y * vr;
+    C2 v;
+    v.x = 0;
+    v.y = 0;
+    if (zs) v = zs[idx];  // overlap term: its angular part is the identity
+    o.x += beta.x * v.x - beta.y * v.y;
+    o.y += beta.x * v.y + beta.y * v.x;
+    out[idx] = o;
+    if (out2) {
+      C2 o2;
+      o2.x = alpha2.x * vr - alpha2.y * vi + beta2.x * v.x - beta2.y * v.y;
+      o2.y = alpha2.x * vi + alpha2.y * vr + beta2.x * v.y + beta2.y * v.x;
+      out2[idx] = o2;
+    }
+  }
+}
+
 template <class T>
 struct FOut {
   typename MfmaT<T>::C2* dst = nullptr;
@@ -828,6 +946,25 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   const dim3 agrid((unsigned)((ablocks + 7) / 8 * 8));
   const int avar = sizeof(T) == 8 ? F.ang_variant64 : F.ang_variant32;
   const int waves = kAngWaveChoices[avar];
+  if (kAngSparsePV[avar] > 0) {
+    const int PV = kAngSparsePV[avar];
+    const int nchunks = (int)ceil_div(P, 64LL * PV);
+    const long long nb = (long long)nc * nchunks;
+    const dim3 sgrid((unsigned)((nb + 7) / 8 * 8));
+    const T* coef = sizeof(T) == 8 ? reinterpret_cast<const T*>(F.sp_coef64.ptr)
+                                   : reinterpret_cast<const T*>(F.sp_coef32.ptr);
+#define PP_ANGS(V)                                                                                                   \
+  hipLaunchKernelGGL((fham_angular_sparse_kernel<T, V>), sgrid, dim3(64 * kAngSparseWaves), 0, s, nc, P, nchunks,   \
+                     F.sp_rowptr.ptr, F.sp_k.ptr, coef, F.sp_term.ptr, Zc, sc, o1.addend, to_c2<T>(o1.alpha),        \
+                     with_s ? Zc + (size_t)nterms * nc * P : nullptr, to_c2<T>(o1.beta), o1.dst,                     \
+                     o2 ? o2->dst : nullptr, to_c2<T>(o2 ? o2->alpha : make_double2(0, 0)),                          \
+                     to_c2<T>(o2 ? o2->beta : make_double2(0, 0)), differs)
+    if (PV == 1) PP_ANGS(1);
+    else PP_ANGS(2);
+#undef PP_ANGS
+    PP_HIP(hipGetLastError());
+    return;
+  }
   if (kAngAllRows[avar] && nrb <= kAngMaxRowBlocks) {
     typedef typename MfmaT<T>::V4 V4;
     const dim3 rgrid((unsigned)ceil_div(P, 16));
@@ -1052,6 +1189,26 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
         }
     std::vector<int> tk(F.Kp);
     for (int k = 0; k < F.Kp; ++k) tk[k] = std::min(k / nchan, nterms - 1);  // term of column k
+    // the same matrix as row lists of its non-zeros for the sparse angular form: row c' holds
+    // (Z row t nc + c, A_t[c'][c], t) in increasing (t, c), terms in the reordered numbering
+    std::vector<int> sp_rowptr(nchan + 1, 0), sp_k, sp_term;
+    std::vector<double> sp_coef;
+    for (int r = 0; r < nchan; ++r) {
+      for (int t = 0; t < nterms; ++t)
+        for (int c = 0; c < nchan; ++c) {
+          const double v = A[((size_t)order[t] * nchan + r) * nchan + c];
+          if (v == 0.0) continue;
+          sp_k.push_back(t * nchan + c);
+          sp_coef.push_back(v);
+          sp_term.push_back(t);
+        }
+      sp_rowptr[r + 1] = (int)sp_k.size();
+    }
+    if (sp_k.empty()) {  // an all-zero operator: keep the device lists non-empty
+      sp_k.push_back(0);
+      sp_coef.push_back(0.0);
+      sp_term.push_back(0);
+    }
     std::vector<double> bt = band_tiles(F.nband, n, bw, dband.data());
     const size_t nbt = bt.size();
     std::vector<double> rp;
@@ -1060,6 +1217,12 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
       hipStream_t s = nullptr;
       F.A64.upload(a.data(), a.size(), s);
       F.tk.upload(tk.data(), tk.size(), s);
+      F.sp_rowptr.upload(sp_rowptr.data(), sp_rowptr.size(), s);
+      F.sp_k.upload(sp_k.data(), sp_k.size(), s);
+      F.sp_term.upload(sp_term.data(), sp_term.size(), s);
+      F.sp_coef64.upload(sp_coef.data(), sp_coef.size(), s);
+      std::vector<float> spf(sp_coef.begin(), sp_coef.end());
+      F.sp_coef32.upload(spf.data(), spf.size(), s);
       std::vector<float> af(a.begin(), a.end());
       F.A32.upload(af.data(), af.size(), s);
       F.bt64.upload(bt.data(), nbt, s);
@@ -1135,7 +1298,7 @@ extern "C" int pp_fham_set_angular_waves(pp_fham* f, int precision, int waves) {
     PP_REQUIRE(f && (precision == 64 || precision == 32), "precision must be 64 or 32");
     int v = -1;
     for (int i = 0; i < kFhamAngularVariants; ++i)
-      if (!kAngAllRows[i] && kAngWaveChoices[i] == waves) v = i;
+      if (!kAngAllRows[i] && !kAngSparsePV[i] && kAngWaveChoices[i] == waves) v = i;
     PP_REQUIRE(v >= 0, "angular wave count must be 16, 8 or 4");
     (precision == 64 ? f->f.ang_variant64 : f->f.ang_variant32) = v;
   });

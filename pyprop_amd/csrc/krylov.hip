@@ -295,6 +295,115 @@ __global__ __launch_bounds__(256) void pade_exp_kernel(const double2* __restrict
   pade_exp_wg(Hm, m, dt, order, work, out_col, sc + 2, piv_ws);
 }
 
+// EigenvectorExponential (pamp.h:567-608): exp(A) ~ V diag(exp(w)) V^H with the unit-norm
+// eigenvectors V of A = -i dt H_m, the reference's zgeev on the m x m upper Hessenberg matrix.
+// Here: complex Schur form by the single-shift QR algorithm (Givens bulge chase, Wilkinson
+// shift, an exceptional shift every 11th iteration), eigenvectors of the triangular factor by
+// back substitution (divisors below eps |T| raised to it, as ztrevc does), v = Q y normalised.
+// h: H_m column-major (H(i, j) at [j * m + i]); returns column 0 of the exponential.
+static std::vector<std::complex<double>> eigenvector_exp_col0(int m, const std::vector<double2>& h, double2 dt) {
+  using cd = std::complex<double>;
+  const cd f = -cd(0, 1) * cd(dt.x, dt.y);
+  std::vector<cd> a((size_t)m * m), q((size_t)m * m, 0.0);  // row-major A and Q
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) a[(size_t)i * m + j] = f * cd(h[(size_t)j * m + i].x, h[(size_t)j * m + i].y);
+  for (int i = 0; i < m; ++i) q[(size_t)i * m + i] = 1.0;
+  auto A = [&](int i, int j) -> cd& { return a[(size_t)i * m + j]; };
+  const double eps = std::numeric_limits<double>::epsilon();
+  double amax = 0;
+  for (const cd& z : a) amax = std::max(amax, std::abs(z));
+  // apply G = [c s; -conj(s) c] to rows k, k+1 (columns >= c0), G^H to columns k, k+1 (rows <= r1) and to Q
+  auto rot = [&](int k, double c, cd sn, int c0, int r1) {
+    for (int j = c0; j < m; ++j) {
+      const cd x = A(k, j), y = A(k + 1, j);
+      A(k, j) = c * x + sn * y;
+      A(k + 1, j) = c * y - std::conj(sn) * x;
+    }
+    for (int i = 0; i <= r1; ++i) {
+      const cd x = A(i, k), y = A(i, k + 1);
+      A(i, k) = c * x + std::conj(sn) * y;
+      A(i, k + 1) = c * y - sn * x;
+    }
+    for (int i = 0; i < m; ++i) {
+      const cd x = q[(size_t)i * m + k], y = q[(size_t)i * m + k + 1];
+      q[(size_t)i * m + k] = c * x + std::conj(sn) * y;
+      q[(size_t)i * m + k + 1] = c * y - sn * x;
+    }
+  };
+  auto givens = [](cd x, cd y, double& c, cd& sn) {
+    const double ax = std::abs(x), ay = std::abs(y), r = std::hypot(ax, ay);
+    if (r == 0) { c = 1; sn = 0; return; }
+    if (ax == 0) { c = 0; sn = std::conj(y) / ay; return; }
+    c = ax / r;
+    sn = (x / ax) * std::conj(y) / r;
+  };
+  int hi = m - 1, its = 0, total = 0;
+  while (hi > 0) {
+    int lo = hi;
+    for (; lo > 0; --lo) {
+      double sc = std::abs(A(lo - 1, lo - 1)) + std::abs(A(lo, lo));
+      if (sc == 0) sc = amax;
+      if (std::abs(A(lo, lo - 1)) <= eps * sc) {
+        A(lo, lo - 1) = 0;
+        break;
+      }
+    }
+    if (lo == hi) {
+      --hi;
+      its = 0;
+      continue;
+    }
+    PP_REQUIRE(++total <= 60 * m, "eigenvector exponential: the QR iteration did not converge");
+    ++its;
+    cd mu;
+    if (its % 11 == 0) {
+      mu = A(hi, hi) + 0.75 * std::abs(A(hi, hi - 1));
+    } else {
+      const cd half = 0.5 * (A(hi - 1, hi - 1) - A(hi, hi));
+      const cd disc = std::sqrt(half * half + A(hi - 1, hi) * A(hi, hi - 1));
+      const cd m1 = A(hi, hi) + half + disc, m2 = A(hi, hi) + half - disc;
+      mu = std::abs(m1 - A(hi, hi)) < std::abs(m2 - A(hi, hi)) ? m1 : m2;
+    }
+    for (int k = lo; k < hi; ++k) {
+      double c;
+      cd sn;
+      if (k == lo) givens(A(k, k) - mu, A(k + 1, k), c, sn);
+      else givens(A(k, k - 1), A(k + 1, k - 1), c, sn);
+      rot(k, c, sn, k == lo ? lo : k - 1, std::min(hi, k + 2));
+      if (k > lo) A(k + 1, k - 1) = 0;
+    }
+  }
+  double tmax = 0;
+  for (int i = 0; i < m; ++i)
+    for (int j = i; j < m; ++j) tmax = std::max(tmax, std::abs(A(i, j)));
+  const double smin = std::max(eps * tmax, std::numeric_limits<double>::min());
+  std::vector<cd> out(m, 0.0), y(m), v(m);
+  for (int k = 0; k < m; ++k) {
+    const cd wk = A(k, k);
+    std::fill(y.begin(), y.end(), cd(0));
+    y[k] = 1;
+    for (int j = k - 1; j >= 0; --j) {
+      cd acc = 0;
+      for (int i = j + 1; i <= k; ++i) acc += A(j, i) * y[i];
+      cd d = A(j, j) - wk;
+      if (std::abs(d) < smin) d = smin;
+      y[j] = -acc / d;
+    }
+    double nrm = 0;
+    for (int i = 0; i < m; ++i) {
+      cd t = 0;
+      for (int j = 0; j <= k; ++j) t += q[(size_t)i * m + j] * y[j];
+      v[i] = t;
+      nrm += std::norm(t);
+    }
+    nrm = std::sqrt(nrm);
+    // column 0 of V diag(exp(w)) V^H: sum_k v_k exp(w_k) conj(v_k[0])
+    const cd coef = std::exp(wk) * std::conj(v[0]) / (nrm * nrm);
+    for (int i = 0; i < m; ++i) out[i] += v[i] * coef;
+  }
+  return out;
+}
+
 Pamp::Pamp(long long n_, int m_, int order_, bool dorth_) : n(n_), m(m_), order(order_), dorth(dorth_) {
   PP_REQUIRE(m >= 2 && m <= 127, "pAMP basis size must be 2..127");
   PP_REQUIRE(order >= 1 && order < 30, "Pade order must be 1..29");
@@ -354,8 +463,20 @@ void Pamp::propagate(pp_operator_fn op, void* ctx, double2* v, double2 dt, hipSt
     }
     PP_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(pade_exp_kernel, dim3(1), dim3(256), 0, s, H.ptr, m, dt, order, work.ptr, expcol.ptr, sc.ptr,
-                     piv.ptr);
+  if (method == 1) {  // ExponentiationEigenvector (pamp.h:640-643): on the host, one synchronisation
+    std::vector<double2> hh((size_t)m * m);
+    PP_HIP(hipMemcpyAsync(hh.data(), H.ptr, hh.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+    PP_HIP(hipStreamSynchronize(s));
+    const std::vector<std::complex<double>> col = eigenvector_exp_col0(m, hh, dt);
+    std::vector<double2> c(m);
+    for (int i = 0; i < m; ++i) c[i] = make_double2(col[i].real(), col[i].imag());
+    PP_HIP(hipMemcpyAsync(expcol.ptr, c.data(), m * sizeof(double2), hipMemcpyHostToDevice, s));
+    PP_HIP(hipMemcpyAsync(sc.ptr + 2, &c[m - 1], sizeof(double2), hipMemcpyHostToDevice, s));
+    PP_HIP(hipStreamSynchronize(s));  // the host vector is released on return
+  } else {
+    hipLaunchKernelGGL(pade_exp_kernel, dim3(1), dim3(256), 0, s, H.ptr, m, dt, order, work.ptr, expcol.ptr, sc.ptr,
+                       piv.ptr);
+  }
   PP_HIP(hipGetLastError());
   // input = norm * V exp(H)[:, 0]
   vec_lincomb(n, m, V.ptr, n, expcol.ptr, sc.ptr, v, s);

@@ -11,7 +11,7 @@ namespace pp {
 
 constexpr int kMaxFTerms = 32;
 constexpr int kFhamRadialVariants = 10;  // radial-stage kernel shapes (band-tile and position forms)
-constexpr int kFhamAngularVariants = 5;  // angular-stage forms (waves splitting K, row blocks per workgroup)
+constexpr int kFhamAngularVariants = 7;  // angular-stage forms (GEMM: waves splitting K, row blocks per workgroup; sparse)
 
 struct FHamOut {
   double2* dst = nullptr;
@@ -25,6 +25,11 @@ struct FHam {
   DeviceBuffer<int> tk;               // term index of each angular GEMM column k (k / nc)
   DeviceBuffer<double> A64, bt64;     // A: MFMA operand order [ceil16(nc)/16][Kp/4][64] of A(c, t*nc + c');
   DeviceBuffer<float> A32, bt32;      // bt: band tiles in MFMA lane order [T][n][w][tiles][KS][64]
+  // the angular matrix as row lists of its non-zeros (sparse angular form): row c' holds
+  // entries [sp_rowptr[c'], sp_rowptr[c' + 1]) of (Z row t nc + c, A_t[c'][c], term t)
+  DeviceBuffer<int> sp_rowptr, sp_k, sp_term;
+  DeviceBuffer<double> sp_coef64;
+  DeviceBuffer<float> sp_coef32;
   mutable DeviceBuffer<double> W64;   // radial products Z [T][nc][n][n] complex
   mutable DeviceBuffer<float> W32;
   mutable DeviceBuffer<double> X64;   // channel-minor copy of the input

@@ -77,6 +77,7 @@ struct Pamp {
   DeviceBuffer<int> flags, piv;
   Reducer red;
   int op_count = 0;
+  int method = 0;  // 0: Pade exponential on the device, 1: eigenvector exponential on the host
   Pamp(long long n, int m, int order, bool dorth);
   void propagate(pp_operator_fn op, void* ctx, double2* v, double2 dt, hipStream_t s);
 };

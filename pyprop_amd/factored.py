@@ -78,6 +78,7 @@ class FactoredHamiltonian:
         nchan, n, _ = shape
         A, tpot, dia, bw = factor_potentials(base.PotentialList, nchan, n)
         self.nterms = A.shape[0]
+        self.angular_nnz = int(np.count_nonzero(A))  # non-zeros of all A_t (the sparse angular form's list)
         self.bw = bw
         self.shape = shape
         self.base = base
@@ -145,9 +146,11 @@ class FactoredHamiltonian:
         return best.value, dict(zip(self.RADIAL_SHAPES, tim[:nr].tolist()))
 
     ANGULAR_WAVES = (16, 8, 4)  # waves splitting K in the angular stage, one row block per workgroup
-    # every angular variant: "<waves>" (one row block of 16 channels per workgroup) or
+    # every angular variant: "<waves>" (one row block of 16 channels per workgroup),
     # "rows<waves>" (all row blocks of a position tile per workgroup, each Z element read once)
-    ANGULAR_VARIANTS = ("16", "8", "4", "rows8", "rows16")
+    # or "sparse<PV>" (only the non-zeros of the angular matrices, one wave per output channel
+    # and 64 PV positions)
+    ANGULAR_VARIANTS = ("16", "8", "4", "rows8", "rows16", "sparse1", "sparse2")
 
     def set_angular_variant(self, precision: int, variant: int):
         N.check(N.load().pp_fham_set_angular_variant(self.h, precision, variant))

@@ -357,7 +357,10 @@ class PamPropagator:
     """pAMP step in the balanced-overlap form (PamPropagator.py:1-83)."""
 
     def __init__(self, psi: Wavefunction, base: BasisPropagator, krylov_basis_size: int = 20,
-                 pade_order: int = 6, krylov_double_orthogonalization: bool = False):
+                 pade_order: int = 6, krylov_double_orthogonalization: bool = False,
+                 krylov_exponentiation_method: int = 0):
+        """krylov_exponentiation_method: 0 = Pade (ExponentiationPade), 1 = eigenvector
+        exponential (ExponentiationEigenvector), pamp.h:43-47 / pampwrapper.cpp:56-60."""
         self.psi = psi
         self.BasePropagator = base
         self.basis_size = krylov_basis_size
@@ -365,6 +368,8 @@ class PamPropagator:
         N.check(N.load().pp_pamp_create(int(np.prod(psi.repr.shape)), krylov_basis_size, pade_order,
                                         int(krylov_double_orthogonalization), C.byref(h)))
         self.h = h
+        if krylov_exponentiation_method:
+            N.check(N.load().pp_pamp_set_exponentiation(self.h, int(krylov_exponentiation_method)))
 
     def SetupStep(self, dt):
         self.BasePropagator.SetupStep(dt)
@@ -448,12 +453,14 @@ class PampWrapper:
     (krylov_PampWrapper, pampwrapper.cpp:18-108)."""
 
     def __init__(self, n: int, krylov_basis_size: int = 20, pade_order: int = 6,
-                 krylov_double_orthogonalization: bool = False):
+                 krylov_double_orthogonalization: bool = False, krylov_exponentiation_method: int = 0):
         h = C.c_void_p()
         N.check(N.load().pp_pamp_create(n, krylov_basis_size, pade_order, int(krylov_double_orthogonalization),
                                         C.byref(h)))
         self.h = h
         self.n = n
+        if krylov_exponentiation_method:
+            N.check(N.load().pp_pamp_set_exponentiation(self.h, int(krylov_exponentiation_method)))
 
     def AdvanceStep(self, callback, psi, tempPsi, dt, t: float, usePypropIntegration: bool = False):
         """psi <- exp(-i dt A) psi, A applied by callback(srcPsi, destPsi, dt, t)

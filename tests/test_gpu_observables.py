@@ -181,6 +181,49 @@ def test_pamp_wrapper_callback_vs_oracle():
         wrapper.AdvanceStep(callback, psi, tmp, dt, t, True)
 
 
+def test_pamp_eigenvector_exponential_vs_oracle():
+    """krylov_exponentiation_method = 1 (ExponentiationEigenvector, pamp.h:567-608,640-643):
+    the library's pAMP step with the host eigen-decomposition of the Hessenberg matrix against
+    the oracle's step with the same method, 3 hydrogen steps in the field at m = 30,
+    dt = 0.002 (a converged expansion, so the Pade step must agree with it too), and the
+    error estimate (the last entry of the exponential's first column) against the oracle's.
+    The eigenvector form is accurate to about m eps |dt H_m| in absolute terms: the radial
+    spectrum reaches ~6e6 a.u., so |dt H_m| ~ 1.2e4 and the bound is ~8e-11.  The estimate
+    here is at that rounding level (~1e-11), so it is compared with an absolute bar of 1e-10
+    (the Pade step's estimate, a different computation, agrees with the oracle's to 3%)."""
+    from pyprop_amd.propagator import BasisPropagator, CombinedRepresentation, PamPropagator, Wavefunction
+    from pyprop_amd.systems import hydrogen_system, hydrogenic_1s
+    say("pAMP eigenvector exponential vs oracle (hydrogen, m = 30)")
+    sysm = hydrogen_system()
+    x0 = np.zeros(sysm.shape, complex)
+    for l, amp in ((0, 1.0), (1, 0.4j), (3, 0.2)):
+        x0[l] = amp * hydrogenic_1s(sysm.bases[1], 1.0, l)[0]
+    repr_ = CombinedRepresentation(sysm.shape, sysm.bases)
+    psi_e = Wavefunction(repr_, x0)
+    psi_p = Wavefunction(repr_, x0)
+    base = BasisPropagator(psi_e, sysm.potentials)
+    eig = PamPropagator(psi_e, base, krylov_basis_size=30, krylov_exponentiation_method=1)
+    pade = PamPropagator(psi_p, base, krylov_basis_size=30)
+    s = oracle_system(sysm)
+    ref = x0.copy()
+    dt, t = 0.002, 0.5 * sysm.pulse.pulse_duration
+    for k in range(3):
+        ref, est = s.pamp_step(ref, dt, base.scalings(t), m=30, method=1)
+        eig.AdvanceStep(t, dt)
+        pade.AdvanceStep(t, dt)
+        t += dt
+    got = psi_e.numpy()
+    e_ref = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    e_pade = np.linalg.norm(got - psi_p.numpy()) / np.linalg.norm(ref)
+    g_est = eig.GetPropagationErrorEstimate()
+    say("  |eig - oracle| %.2e, |eig - pade| %.2e, estimate %s (oracle %s)" % (e_ref, e_pade, g_est, est))
+    assert e_ref < 1e-10
+    assert e_pade < 1e-9
+    assert abs(g_est - est) < 1e-10
+    with pytest.raises(RuntimeError):
+        PamPropagator(psi_e, base, krylov_basis_size=30, krylov_exponentiation_method=2)
+
+
 def test_problem_driver_on_device():
     """Problem.Advance over the device pAMP propagator equals the explicit step loop (same
     kernels, same times: bitwise), with the reference's time bookkeeping; imaginary-time
