@@ -554,6 +554,12 @@ int pp_mixed_set_factored(pp_mixed* m, const pp_fham* f);
  * the inner iteration count adapted at its synchronisations.  The results agree; on one
  * MI355X with helium both modes take 1.41 ms per step (the step is kernel-bound). */
 int pp_mixed_set_device_mode(pp_mixed* m, int device_mode);
+/* Classical Gram-Schmidt passes per inner FGMRES iteration (no reference counterpart: the
+ * reference's GMRES orthogonalises in FP64, gmres.h:433-505).  1 (the default): one pass,
+ * the norm taken from the updated vector (3 launches); the FP32 basis is then orthogonal
+ * only to ~eps32 / h_{j+1,j}, which the FP64 refinement absorbs (at helium: the same 3 inner
+ * iterations per correction and 1e-11 error after 20 steps).  2: two passes (4 launches). */
+int pp_mixed_set_gram_schmidt(pp_mixed* m, int passes);
 int pp_mixed_step(pp_mixed* m, pp_cplx* psi, double dt, const double* scalings, pp_stream stream);
 int pp_mixed_stats(const pp_mixed* m, int* counts, double* residuals);
 

@@ -202,6 +202,23 @@ __global__ __launch_bounds__(256) void cgs32_kernel(long long n, int m, const fl
   }
 }
 
+// One-pass Gram-Schmidt finish: out = [h_0 .. h_{m-1}, |w|^2 before the update | m + 1
+// zeros | |w|^2 after the update] (the layout of the two-pass results, the second dots zero)
+__global__ __launch_bounds__(256) void finish_cgs1_kernel(int blocks, const double2* __restrict__ p1, int m,
+                                                          const double2* __restrict__ p3, double2* __restrict__ out,
+                                                          double2* __restrict__ out2) {
+  __shared__ double2 res[130];
+  sum_partials(p1, blocks, m + 1, m + 1, res);
+  sum_partials(p3, blocks, 1, 1, res + m + 1);
+  __syncthreads();
+  const int width = m + 1;
+  for (int j = threadIdx.x; j < 2 * width + 1; j += blockDim.x) {
+    const double2 v = j < width ? res[j] : (j == 2 * width ? res[width] : make_double2(0.0, 0.0));
+    out[j] = v;
+    if (out2) out2[j] = v;
+  }
+}
+
 // r = b - r (r holds A x on entry); partial[block] = block's |r|^2
 __global__ __launch_bounds__(256) void residual_partial_kernel(long long n, const double2* __restrict__ b,
                                                                double2* __restrict__ r,
@@ -1303,6 +1320,28 @@ void MixedWork::cgs2_32(int k, float2* w_, double2* out, double2* out_dev, hipSt
   PP_HIP(hipGetLastError());
 }
 
+// One-pass classical Gram-Schmidt against V_0..V_{k-1} in three launches: the dots (+|w|^2),
+// the update with the norm of the updated vector, and a one-block finish (results in the
+// two-pass layout, see finish_cgs1_kernel)
+void MixedWork::cgs1_32(int k, float2* w_, double2* out, double2* out_dev, hipStream_t s) {
+  const int blocks = fused_blocks(n);
+  const int width = k + 1;
+  PP_REQUIRE(2 * width + 1 <= 128, "Gram-Schmidt basis too large");
+  red.ensure(blocks, width + 1);
+  double2* p1 = red.partial.ptr;
+  double2* p3 = p1 + (size_t)blocks * width;
+  const dim3 g(blocks), b(256);
+  hipLaunchKernelGGL((cgs32_kernel<false, true>), g, b, 0, s, n, k, V.ptr, n, nullptr, 0, w_, p1);
+  hipLaunchKernelGGL((cgs32_kernel<true, false>), g, b, 0, s, n, k, V.ptr, n, p1, width, w_, p3);
+  hipLaunchKernelGGL(finish_cgs1_kernel, dim3(1), b, 0, s, blocks, p1, k, p3, out, out_dev);
+  PP_HIP(hipGetLastError());
+}
+
+void MixedWork::gram_schmidt(int k, float2* w_, double2* out, double2* out_dev, hipStream_t s) {
+  if (gs_passes == 1) cgs1_32(k, w_, out, out_dev, s);
+  else cgs2_32(k, w_, out, out_dev, s);
+}
+
 // out[0] = |w|^2 (w is only read)
 void MixedWork::norm2_32(const float2* w_, double2* out, hipStream_t s) {
   const int blocks = fused_blocks(n);
@@ -1367,8 +1406,8 @@ int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply
     const double2* n2 = j == 0 ? nullptr : hdev.ptr + (size_t)(j - 1) * stride + 2 * (j + 1);
     apply_P(j, vj, n2, zj);
     apply_A(zj, vn);
-    // classical Gram-Schmidt, two passes: [first dots, |w|^2 | second dots, |w|^2 | final |w|^2]
-    cgs2_32(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s);
+    // classical Gram-Schmidt: [first dots, |w|^2 | second dots, |w|^2 | final |w|^2]
+    gram_schmidt(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s);
     if (j + 1 < pre) continue;
     PP_HIP(hipStreamSynchronize(s));  // the finishing kernels wrote straight into small
     for (int jj = processed; jj <= j && !done; ++jj) {
@@ -1436,7 +1475,7 @@ void MixedWork::fgmres32_device(const std::function<void(const float2*, float2*)
     const double2* n2 = j == 0 ? nullptr : hdev.ptr + (size_t)(j - 1) * stride + 2 * (j + 1);
     apply_P(j, vj, n2, zj);
     apply_A(zj, vn);
-    cgs2_32(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s);
+    gram_schmidt(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s);
   }
   hipLaunchKernelGGL(hessenberg_lsq_kernel, dim3(1), dim3(64), 0, s, k, stride, hdev.ptr, beta2_dev, ycoef.ptr,
                      small_dev + est);
@@ -1771,6 +1810,13 @@ extern "C" int pp_mixed_set_learned_mode(pp_mixed* m, int mode) {
   return guarded([&] {
     PP_REQUIRE(m && mode >= 0 && mode <= 2, "learned mode must be 0, 1 or 2");
     m->c.learned_mode = mode;
+  });
+}
+
+extern "C" int pp_mixed_set_gram_schmidt(pp_mixed* m, int passes) {
+  return guarded([&] {
+    PP_REQUIRE(m && (passes == 1 || passes == 2), "Gram-Schmidt passes must be 1 or 2");
+    m->c.work.gs_passes = passes;
   });
 }
 

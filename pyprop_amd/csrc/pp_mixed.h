@@ -79,6 +79,10 @@ struct MixedWork {
   // Two-pass classical Gram-Schmidt of w against V_0..V_{k-1} (FP64 sums): out[0..k] = first
   // dots and |w|^2, out[k+1..2k+1] = second dots and |w|^2, out[2k+2] = final |w|^2
   void cgs2_32(int k, float2* w, double2* out, double2* out_dev, hipStream_t s);
+  // one pass: dots, then the update with the norm of the updated vector
+  void cgs1_32(int k, float2* w, double2* out, double2* out_dev, hipStream_t s);
+  void gram_schmidt(int k, float2* w, double2* out, double2* out_dev, hipStream_t s);
+  int gs_passes = 1;  // classical Gram-Schmidt passes per inner iteration (1 or 2)
   void norm2_32(const float2* w, double2* out, hipStream_t s);  // out[0] = |w|^2
   // r = b - r, out[0] = |r|^2 (FP64; pinned out and device out_dev)
   void residual(const double2* b, double2* r, double2* out, double2* out_dev, hipStream_t s);

@@ -320,13 +320,16 @@ class MixedCayleyPropagator:
     def __init__(self, psi, base, kron: KroneckerPreconditioner, learned: Optional[LearnedPreconditioner] = None,
                  fallback=None, fallback_iterations: int = 40, krylov_basis_size: int = 20,
                  tolerance: float = 1e-10, inner_tolerance: float = 1e-5, max_outer: int = 8, factored=None,
-                 device_mode: bool = False, learned_mode: int = 0):
+                 device_mode: bool = False, learned_mode: int = 0, gram_schmidt_passes: int = 1):
         """device_mode: after the first converged step, solve the inner least-squares problems
         on the device and synchronise once per step (pp_mixed_set_device_mode).
         learned_mode: the FGMRES directions the learned correction acts on: 0 the first of
         the first correction, 1 the first of every correction, 2 all
-        (pp_mixed_set_learned_mode)."""
+        (pp_mixed_set_learned_mode).
+        gram_schmidt_passes: classical Gram-Schmidt passes per inner iteration (1 saves one
+        launch per iteration; pp_mixed_set_gram_schmidt)."""
         self.learned_mode = learned_mode
+        self.gram_schmidt_passes = gram_schmidt_passes
         self.psi = psi
         self.device_mode = device_mode
         self.factored = factored
@@ -361,6 +364,7 @@ class MixedCayleyPropagator:
             N.check(N.load().pp_mixed_set_factored(h, self.factored.h))
         N.check(N.load().pp_mixed_set_device_mode(h, int(self.device_mode)))
         N.check(N.load().pp_mixed_set_learned_mode(h, int(self.learned_mode)))
+        N.check(N.load().pp_mixed_set_gram_schmidt(h, int(self.gram_schmidt_passes)))
 
     def SetLearned(self, learned):
         self.learned = learned

@@ -635,18 +635,21 @@ def test_factored_without_separable_split(torch, rng):
     assert all(ms > 0 for k, ms in tim.items() if not k.startswith("pos"))
 
 
-def test_mixed_cayley_factored_full_size_vs_plain(torch):
+@pytest.mark.parametrize("passes", [2, 1])
+def test_mixed_cayley_factored_full_size_vs_plain(torch, passes):
     """Full helium, 20 steps: mixed FP32 + factored multiply (with its FP64 overlap term)
-    against the plain FP64 path converged to 1e-14.  The bar, 2e-10 after 20 steps, is about
-    ten times the measured 2e-11: a solver whose refinement stalls in the small-eigenvalue
-    directions of S (x) S (an FP32 operator that applies S (x) S entry by entry does) passes a
-    residual test but drifts by ~1e-10 per step, and fails this."""
+    against the plain FP64 path converged to 1e-14, with two- and one-pass Gram-Schmidt in
+    the inner solve.  The bar, 2e-10 after 20 steps, is about ten times the measured 2e-11: a
+    solver whose refinement stalls in the small-eigenvalue directions of S (x) S (an FP32
+    operator that applies S (x) S entry by entry does) passes a residual test but drifts by
+    ~1e-10 per step, and fails this.  The one-pass form may not need more inner iterations
+    than the two-pass one's 3 per correction by more than one per step."""
     from pyprop_amd.factored import FactoredHamiltonian
     from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
     from pyprop_amd.propagator import (BasisPropagator, CayleyPropagator, CombinedRepresentation,
                                        RadialTwoElectronPreconditioner, Wavefunction)
     from pyprop_amd.systems import helium_initial_state, helium_system
-    say("full-size helium: mixed+factored vs plain FP64 (tol 1e-14), 20 steps")
+    say("full-size helium: mixed+factored vs plain FP64 (tol 1e-14), 20 steps, %d-pass Gram-Schmidt" % passes)
     sysm = helium_system()
     repr_ = CombinedRepresentation(sysm.shape, sysm.bases)
     psi = Wavefunction(repr_, helium_initial_state(sysm))
@@ -659,16 +662,20 @@ def test_mixed_cayley_factored_full_size_vs_plain(torch):
     plain.SetupStep(dt)
     fh = FactoredHamiltonian(base)
     assert fh.has_overlap
-    mixed = MixedCayleyPropagator(psi2, base, KroneckerPreconditioner(psi2, sysm), tolerance=1e-10, factored=fh)
+    mixed = MixedCayleyPropagator(psi2, base, KroneckerPreconditioner(psi2, sysm), tolerance=1e-10, factored=fh,
+                                  gram_schmidt_passes=passes)
     t = 100.0
+    inner = []
     for k in range(20):
         plain.AdvanceStep(t, dt)
         mixed.AdvanceStep(t, dt)
+        inner.append(mixed.GetStats()["inner"])
         t += dt
-    say("  last step stats %s" % mixed.GetStats())
+    say("  last step stats %s, inner iterations per step %s" % (mixed.GetStats(), inner))
     err = rel(psi2.numpy(), psi.numpy())
     say("  wavefunction difference %.2e" % err)
     assert err < 2e-10
+    assert np.mean(inner[1:]) <= 7.0
 
 
 def test_mixed_factored_reuse_detects_modified_psi(torch, rng):
