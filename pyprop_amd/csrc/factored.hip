@@ -820,69 +820,140 @@ static typename MfmaT<T>::C2 to_c2(double2 v) {
 }
 
 // Radial products of the separable terms R = a (x) b: Z_c = a X_c b^T for one (channel,
-// term) per workgroup, with X_c and the r2 pass held in LDS.  a, b: band rows [n][2 bw + 1]
-// (a(i, i - bw + jj)).  Term j writes product slot first + j.  Skipped, like the band
+// term) per workgroup, X_c and the r2 pass held in LDS.  a, b: band rows [n][2 BW + 1]
+// (a(i, i - BW + jj)).  Term j writes product slot first + j.  Skipped, like the band
 // kernels, when the input equals the one whose products are held.
-template <class T>
-__global__ __launch_bounds__(1024) void fham_separable_kernel(int nc, int n, int bw, int first,
+// Two optional jobs of the FP32 inner operator ride along:
+//  * with cu1 (the upper band factors U1, U2 of the two radial overlaps, [j][BW + i - j],
+//    band BW), workgroup y = 0 forms (S1 (x) S2) X_c in Cholesky form, U1^T U1 X_c U2^T U2,
+//    by four triangular passes into slot cslot (the terms then take y = 1 ..): in FP32,
+//    S (x) S applied entry by entry loses its small eigenvalues, this form keeps them;
+//  * with Xt, the workgroups of channel c write its block, a slice each, into the
+//    channel-minor copy Xt[j1][j2][c][part] that the radial kernels read (the transpose
+//    launch is then skipped).
+// A pass out[o] = sum_t coef(o, t) in[o + off + t] runs along one radial axis: wave k
+// computes outputs 4k .. 4k + 3 (so the coefficients are wave-uniform: scalar loads), lane p
+// the index across the axis, holding the 3 + taps inputs it needs in registers (4 + taps - 1
+// LDS reads for 4 outputs).  LDS rows have stride kSepLD, so that lanes reading down a column
+// hit distinct banks.
+constexpr int kSepLD = 65;
+constexpr int kSepR = 4;  // outputs per lane along a pass (16 waves x 4 >= 64)
+template <class T, int TAPS, bool ALONG_R1, class CoefFn>
+__device__ inline void sep_pass(const typename MfmaT<T>::C2* src, typename MfmaT<T>::C2* dst, int dst_ld, int n,
+                                int off, int o0, int p, CoefFn coef) {
+  typedef typename MfmaT<T>::C2 C2;
+  C2 win[kSepR + TAPS - 1];
+#pragma unroll
+  for (int k = 0; k < kSepR + TAPS - 1; ++k) {
+    const int i = o0 + off + k;
+    if (i >= 0 && i < n && p < n) win[k] = ALONG_R1 ? src[i * kSepLD + p] : src[p * kSepLD + i];
+    else {
+      win[k].x = 0;
+      win[k].y = 0;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kSepR; ++r) {
+    const int o = o0 + r;
+    const int oc = min(o, n - 1);  // coefficient row (outputs past n are not stored)
+    T re = 0, im = 0;
+#pragma unroll
+    for (int t = 0; t < TAPS; ++t) {
+      const T a = coef(oc, t);
+      re = fma(a, win[r + t].x, re);
+      im = fma(a, win[r + t].y, im);
+    }
+    if (o < n && p < n) {
+      C2 v;
+      v.x = re;
+      v.y = im;
+      if (ALONG_R1) dst[o * dst_ld + p] = v;
+      else dst[p * dst_ld + o] = v;
+    }
+  }
+}
+
+template <class T, int BW>
+__global__ __launch_bounds__(1024) void fham_separable_kernel(int nc, int n, int first,
                                                               const T* __restrict__ SA, const T* __restrict__ SB,
                                                               const typename MfmaT<T>::C2* __restrict__ X,
                                                               typename MfmaT<T>::C2* __restrict__ Z,
-                                                              const int* __restrict__ differs) {
+                                                              const int* __restrict__ differs,
+                                                              const T* __restrict__ cu1, const T* __restrict__ cu2,
+                                                              int cslot, T* __restrict__ Xt, int ncp) {
   typedef typename MfmaT<T>::C2 C2;
+  constexpr int W = 2 * BW + 1;
   if (differs && *differs == 0) return;
   extern __shared__ char sep_raw[];
-  const int w = 2 * bw + 1, m = n * n;
-  C2* Xs = reinterpret_cast<C2*>(sep_raw);
-  C2* Ts = Xs + m;
-  T* as = reinterpret_cast<T*>(Ts + m);
-  T* bs = as + n * w;
-  const int c = blockIdx.x, j = blockIdx.y;
+  const int m = n * n;
+  C2* Xs = reinterpret_cast<C2*>(sep_raw);  // n rows of kSepLD
+  C2* Ts = Xs + n * kSepLD;
+  const int c = blockIdx.x;
+  // with cu1, y = 0 is the Cholesky-form overlap job (dispatched first: its four dependent
+  // passes are the longest chain) and y - 1 the separable term; else y is the term
+  const bool chol = cu1 && blockIdx.y == 0;
+  const int j = cu1 ? (int)blockIdx.y - 1 : (int)blockIdx.y;
   const C2* x = X + (size_t)c * m;
-  for (int e = threadIdx.x; e < m; e += blockDim.x) Xs[e] = x[e];
-  for (int e = threadIdx.x; e < n * w; e += blockDim.x) {
-    as[e] = SA[(size_t)j * n * w + e];
-    bs[e] = SB[(size_t)j * n * w + e];
+  // staging: every load of a thread is issued before the first LDS store
+  constexpr int kStageX = 4;  // m <= 4096 (separable_fits)
+  C2 xv[kStageX];
+#pragma unroll
+  for (int q = 0; q < kStageX; ++q) {  // past m: element 0 again (not stored)
+    const int e = threadIdx.x + q * 1024;
+    xv[q] = x[e < m ? e : 0];
   }
-  __syncthreads();
-  // thread -> (row i1 = threadIdx.x / 64 + 16 k, column i2 = threadIdx.x % 64): no division
-  const int col = threadIdx.x & 63, row0 = threadIdx.x >> 6, rstep = blockDim.x >> 6;
-  if (col < n) {
-    for (int i1 = row0; i1 < n; i1 += rstep) {  // T = X b^T (along r2)
-      T re = 0, im = 0;
-      for (int d = 0; d < w; ++d) {
-        const int j2 = col - bw + d;
-        if (j2 < 0 || j2 >= n) continue;
-        const T f = bs[col * w + d];
-        const C2 v = Xs[i1 * n + j2];
-        re = fma(f, v.x, re);
-        im = fma(f, v.y, im);
-      }
-      C2 o;
-      o.x = re;
-      o.y = im;
-      Ts[i1 * n + col] = o;
+#pragma unroll
+  for (int q = 0; q < kStageX; ++q) {
+    const int e = threadIdx.x + q * 1024;
+    if (e < m) {
+      const int i = e / n;
+      Xs[i * kSepLD + (e - i * n)] = xv[q];
     }
   }
   __syncthreads();
+  if (Xt) {  // the channel's workgroups share the channel-minor copy, a slice each
+    const int jobs = gridDim.y, per = (m + jobs - 1) / jobs;
+    const int e0 = (int)blockIdx.y * per, e1 = min(m, e0 + per);
+    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+      const int i = e / n;
+      reinterpret_cast<C2*>(Xt)[(size_t)e * ncp + c] = Xs[i * kSepLD + (e - i * n)];
+    }
+  }
+  const int p = threadIdx.x & 63;
+  const int o0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * kSepR;
+  if (chol) {
+    // U along an axis: out[o] = sum_t U(o, o + t) in[o + t] (U(o, o + t) past the last row
+    // multiplies a zero input: its index is clamped into the array); U^T: out[o] =
+    // sum_t U(o - BW + t, o) in[o - BW + t]
+    auto Uc = [&](const T* u) {
+      const int nn = n;
+      return [u, nn](int o, int t) { return u[min(o + t, nn - 1) * (BW + 1) + BW - t]; };
+    };
+    auto UT = [&](const T* u) { return [u](int o, int t) { return u[o * (BW + 1) + t]; }; };
+    sep_pass<T, BW + 1, false>(Xs, Ts, kSepLD, n, 0, o0, p, Uc(cu2));    // U2 along r2
+    __syncthreads();
+    sep_pass<T, BW + 1, false>(Ts, Xs, kSepLD, n, -BW, o0, p, UT(cu2));  // U2^T along r2
+    __syncthreads();
+    sep_pass<T, BW + 1, true>(Xs, Ts, kSepLD, n, 0, o0, p, Uc(cu1));     // U1 along r1
+    __syncthreads();
+    C2* z = Z + ((size_t)cslot * nc + c) * m;
+    sep_pass<T, BW + 1, true>(Ts, z, n, n, -BW, o0, p, UT(cu1));         // U1^T along r1, to the slot
+    return;
+  }
+  const T* fa = SA + (size_t)j * n * W;
+  const T* fb = SB + (size_t)j * n * W;
+  // T = X b^T along r2: T[i1][o] = sum_d b[o][d] X[i1][o - BW + d]
+  sep_pass<T, W, false>(Xs, Ts, kSepLD, n, -BW, o0, p, [fb](int o, int t) { return fb[o * W + t]; });
+  __syncthreads();
+  // Z = a T along r1: Z[o][i2] = sum_jj a[o][jj] T[o - BW + jj][i2]
   C2* z = Z + ((size_t)(first + j) * nc + c) * m;
-  if (col < n) {
-    for (int i1 = row0; i1 < n; i1 += rstep) {  // Z = a T (along r1)
-      T re = 0, im = 0;
-      for (int jj = 0; jj < w; ++jj) {
-        const int j1 = i1 - bw + jj;
-        if (j1 < 0 || j1 >= n) continue;
-        const T f = as[i1 * w + jj];
-        const C2 v = Ts[j1 * n + col];
-        re = fma(f, v.x, re);
-        im = fma(f, v.y, im);
-      }
-      C2 o;
-      o.x = re;
-      o.y = im;
-      z[i1 * n + col] = o;
-    }
-  }
+  sep_pass<T, W, true>(Ts, z, n, n, -BW, o0, p, [fa](int o, int t) { return fa[o * W + t]; });
+}
+
+template <class T>
+__global__ void fham_fill_zero_kernel(long long count, T* __restrict__ p) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count; i += (long long)gridDim.x * blockDim.x)
+    p[i] = T(0);
 }
 
 // o1.dst = o1.addend + o1.alpha Y + o1.beta Zs, o2 likewise without addend (may be NULL);
@@ -897,16 +968,54 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   const long long P = (long long)n * n;
   auto nz = [](double2 v) { return v.x != 0.0 || v.y != 0.0; };
   const bool with_s = nz(o1.beta) || (o2 && nz(o2->beta));
-  PP_REQUIRE(!with_s || F.has_overlap, "overlap coefficient given but the operator has no overlap term");
+  // FP32 with the Cholesky factors: the overlap product by the four triangular passes
+  const bool chol = sizeof(T) == 4 && with_s && F.csd == F.bw && F.sep_enabled;
+  PP_REQUIRE(!with_s || F.has_overlap || chol, "overlap coefficient given but the operator has no overlap term");
   PP_REQUIRE(!(o2 && o2->addend), "the second output takes no addend");
   const int nrad = nterms + (with_s ? 1 : 0);  // radial terms formed (the overlap term is last)
-  const size_t zneed = (size_t)(nterms + (F.has_overlap ? 1 : 0)) * nc * P * 2;
+  const size_t zneed = (size_t)(nterms + ((F.has_overlap || chol) ? 1 : 0)) * nc * P * 2;
   if (Zb.count < zneed) Zb.resize(zneed);
   const int ngroups = (nc + 7) / 8;
   const int ncp = ngroups * 8;
-  if (Xb.count < (size_t)P * ncp * 2) Xb.resize((size_t)P * ncp * 2);
-  hipLaunchKernelGGL(fham_transpose_kernel<T>, dim3((unsigned)std::min<long long>(4096, ceil_div(P * ncp * 2, 256))),
-                     dim3(256), 0, s, nc, ncp, n, reinterpret_cast<const T*>(src), Xb.ptr, ref, differs);
+  const size_t xneed = (size_t)P * ncp * 2;
+  if (Xb.count < xneed) {
+    Xb.resize(xneed);
+    // padding channels stay zero (the separable kernel writes only the nc real ones)
+    hipLaunchKernelGGL(fham_fill_zero_kernel<T>, dim3((unsigned)std::min<long long>(4096, ceil_div((long long)xneed, 256LL))),
+                       dim3(256), 0, s, (long long)xneed, Xb.ptr);
+  }
+  // separable terms (and, in FP32, the Cholesky-form overlap) run first when their kernel
+  // can also write the channel-minor copy: no transpose launch (FP32 only: the FP64 path
+  // compares the input with the held one in the transpose kernel)
+  const int nsep_terms = F.sep_enabled ? nterms - F.nband : 0;
+  const int nsep_run = nsep_terms + (with_s && !chol && F.sep_enabled ? 1 : 0);
+  const int sep_jobs = nsep_run + (chol ? 1 : 0);
+  const bool fused_xt = sizeof(T) == 4 && F.sep_enabled && sep_jobs > 0 && !ref;
+  auto launch_separable = [&](T* xt) {
+    if (sep_jobs <= 0) return;
+    typedef typename MfmaT<T>::C2 C2s;
+    const size_t lds = 2 * (size_t)n * kSepLD * sizeof(C2s);
+    const T* cu1 = chol ? reinterpret_cast<const T*>(F.cu1_32.ptr) : nullptr;
+    const T* cu2 = chol ? reinterpret_cast<const T*>(F.cu2_32.ptr) : nullptr;
+    switch (F.bw) {
+#define PP_SEP(B)                                                                                                 \
+  case B:                                                                                                         \
+    PP_HIP(hipFuncSetAttribute((const void*)fham_separable_kernel<T, B>,                                          \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                            \
+    hipLaunchKernelGGL((fham_separable_kernel<T, B>), dim3(nc, sep_jobs), dim3(1024), lds, s, nc, n, F.nband,      \
+                       SAb.ptr, SBb.ptr, src, reinterpret_cast<C2s*>(Zb.ptr), differs, cu1, cu2, nterms, xt, ncp);  \
+    break;
+      PP_SEP(1) PP_SEP(2) PP_SEP(3) PP_SEP(4) PP_SEP(5) PP_SEP(6) PP_SEP(7) PP_SEP(8)
+#undef PP_SEP
+      default:
+        throw Error(PP_ERR_INVALID, "factored multiply supports band half-widths 1..8");
+    }
+    PP_HIP(hipGetLastError());
+  };
+  if (fused_xt) launch_separable(Xb.ptr);
+  else
+    hipLaunchKernelGGL(fham_transpose_kernel<T>, dim3((unsigned)std::min<long long>(4096, ceil_div(P * ncp * 2, 256))),
+                       dim3(256), 0, s, nc, ncp, n, reinterpret_cast<const T*>(src), Xb.ptr, ref, differs);
   const int tiles = (n + 15) / 16;
   const int variant = sizeof(T) == 8 ? F.rad_variant64 : F.rad_variant32;
   // band-tile terms first; with separable terms those (and the overlap term) follow
@@ -925,18 +1034,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
         throw Error(PP_ERR_INVALID, "factored multiply supports band half-widths 1..8");
     }
   }
-  if (F.sep_enabled) {
-    const int nsep_run = (nterms - F.nband) + (with_s ? 1 : 0);
-    if (nsep_run > 0) {
-      typedef typename MfmaT<T>::C2 C2s;
-      const size_t lds = 2 * (size_t)P * sizeof(C2s) + 2 * (size_t)n * (2 * F.bw + 1) * sizeof(T);
-      PP_HIP(hipFuncSetAttribute((const void*)fham_separable_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds));
-      hipLaunchKernelGGL(fham_separable_kernel<T>, dim3(nc, nsep_run), dim3(1024), lds, s, nc, n, F.bw, F.nband,
-                         SAb.ptr, SBb.ptr, src, reinterpret_cast<C2s*>(Zb.ptr), differs);
-      PP_HIP(hipGetLastError());
-    }
-  }
+  if (!fused_xt && (F.sep_enabled || chol)) launch_separable(nullptr);
   FScales<T> sc;
   for (int t = 0; t < kMaxFTerms; ++t) sc.s[t] = t < nterms ? (T)scalings[F.term_pot[t]] : T(0);
   const int nrb = (nc + 15) / 16;
@@ -996,6 +1094,16 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   else PP_ANG(16);
 #undef PP_ANG
   PP_HIP(hipGetLastError());
+}
+
+void FHam::set_cholesky32(const std::vector<double>& u1, const std::vector<double>& u2, int sd) const {
+  PP_REQUIRE(sd >= 0 && sd <= 2 * bw, "overlap band wider than the operator's");
+  PP_REQUIRE(u1.size() == (size_t)n * (sd + 1) && u2.size() == (size_t)n * (sd + 1), "overlap factor size mismatch");
+  std::vector<float> f1(u1.begin(), u1.end()), f2(u2.begin(), u2.end());
+  cu1_32.upload(f1.data(), f1.size(), nullptr);
+  cu2_32.upload(f2.data(), f2.size(), nullptr);
+  PP_HIP(hipStreamSynchronize(nullptr));
+  csd = sd;
 }
 
 void fham_apply64(const FHam& f, const double* scalings, const double2* src, double2* dst, const double2* addend,
@@ -1131,7 +1239,7 @@ static bool separable_factors(const double* D, int n, int bw, std::vector<double
 // the separable kernel holds X_c, one pass and both factors in LDS, and maps one thread to
 // each of at most 64 columns
 static bool separable_fits(int n) {
-  return n <= 64 && 2 * (size_t)n * n * sizeof(double2) + 2 * (size_t)n * 17 * sizeof(double) <= 160 * 1024;
+  return n <= 64 && 2 * (size_t)n * kSepLD * sizeof(double2) <= 160 * 1024;
 }
 
 extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double* A, const int* term_potential,

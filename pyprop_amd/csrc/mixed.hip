@@ -1536,9 +1536,13 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     // operator holds S (x) S: in FP32, S (x) S applied entry by entry loses its small
     // eigenvalues (relative error ~ eps32 cond(S (x) S)), and the refinement then cannot
     // reduce the error in those directions, where the residual barely sees it.
-    work.overlap_multiply32(R, in, work.tmp.ptr, work.w.ptr, s);
-    if (fham) fham->apply32(scal.data(), in, out, work.tmp.ptr, make_double2(0.0, a), s);
-    else H.apply_f32(scal.data(), in, out, work.tmp.ptr, make_float2(0.f, (float)a), s);
+    if (fham && fham->csd == fham->bw && fham->sep_enabled) {  // S in Cholesky form inside the multiply
+      fham->apply32(scal.data(), in, out, nullptr, make_double2(0.0, a), s, make_double2(1.0, 0.0));
+    } else {
+      work.overlap_multiply32(R, in, work.tmp.ptr, work.w.ptr, s);
+      if (fham) fham->apply32(scal.data(), in, out, work.tmp.ptr, make_double2(0.0, a), s);
+      else H.apply_f32(scal.data(), in, out, work.tmp.ptr, make_float2(0.f, (float)a), s);
+    }
     ++stats.inner_ops;
   };
   int correction = 0;
@@ -1833,6 +1837,12 @@ extern "C" int pp_mixed_set_factored(pp_mixed* m, const pp_fham* f) {
     PP_REQUIRE(!f || (long long)f->f.nc * f->f.n * f->f.n == m->c.ham->n, "factored operator shape does not match");
     PP_REQUIRE(!f || f->f.npot == (int)m->c.ham->terms.size(), "factored operator has a different potential count");
     m->c.fham = f ? &f->f : nullptr;
+    // the FP32 inner operator forms S (x) S in Cholesky form inside the factored multiply
+    // when the representation's two radial overlap factors match it
+    const Repr& R = *m->c.ham->repr;
+    if (f && f->f.csd < 0 && f->f.sep_enabled && R.ndim == 3 && !R.ov[0] && R.ov[1] && R.ov[2] &&
+        R.ov[1]->sd == R.ov[2]->sd && R.ov[1]->n == f->f.n && R.ov[2]->n == f->f.n && R.ov[1]->sd <= 2 * f->f.bw)
+      f->f.set_cholesky32(R.ov[1]->u_host, R.ov[2]->u_host, R.ov[1]->sd);
   });
 }
 

@@ -49,6 +49,13 @@ struct FHam {
   // jj w + d, t = 16 tile + l % 16
   DeviceBuffer<double> rp64;
   DeviceBuffer<float> rp32;
+  // FP32 overlap in Cholesky form (set by the mixed solver from its representation): the
+  // upper band factors U1, U2 of the two radial overlaps ([j][csd + i - j]); with them an
+  // FP32 multiply with a beta forms (S1 (x) S2) x as U1^T U1 x U2^T U2 in the separable kernel
+  // (a cache of the representation's factors, attached by a solver: mutable)
+  mutable int csd = -1;
+  mutable DeviceBuffer<float> cu1_32, cu2_32;
+  void set_cholesky32(const std::vector<double>& u1, const std::vector<double>& u2, int sd) const;
   int rad_variant64 = 0, rad_variant32 = 0;  // radial kernel shape per precision
   int ang_variant64 = 0, ang_variant32 = 0;  // angular stage wave count per precision
   mutable unsigned long long gen64 = 0;      // FP64 multiplies issued

@@ -486,6 +486,61 @@ def test_factored_overlap_term_vs_dense(torch, rng):
         fh2.MultiplyOverlap(src, torch.empty_like(src), t, 1.0, 1.0)
 
 
+def test_factored_fp32_overlap_cholesky_form(torch, rng):
+    """The FP32 inner operator's S (x) S inside the factored multiply: once a mixed solver
+    attaches the representation's Cholesky factors, an FP32 multiply with a beta forms
+    U1^T U1 x U2^T U2 in the separable kernel (and that kernel also writes the channel-minor
+    copy, with no transpose launch).  On the eigenvector of S1 (x) S2 with the smallest
+    eigenvalue (cond(S1 (x) S2) = 8.5e3 here) the Cholesky form must be accurate to
+    10 eps32 sqrt(cond) (measured 1.2e-5), and differ from the separable two-pass form used
+    before the factors are attached (measured 1.05e-5: at this size both are accurate, since a
+    1-D pass loses only eps32 cond(S) per rank); with the Hamiltonian added, FP32 against
+    FP64 to 2e-6."""
+    import scipy.linalg
+    from pyprop_amd.factored import FactoredHamiltonian
+    from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
+    say("factored FP32 overlap in Cholesky form (small helium)")
+    sysm = _small_helium()
+    repr_, psi, base = _device(sysm, torch)
+    base.SetupStep()
+    fh = FactoredHamiltonian(base)
+    nc, n, _ = sysm.shape
+    S1 = sysm.bases[1].overlap_band()
+    S2 = sysm.bases[2].overlap_band()
+    w1, v1 = scipy.linalg.eigh(S1)
+    w2, v2 = scipy.linalg.eigh(S2)
+    lam = w1[0] * w2[0]
+    x = np.zeros(sysm.shape, complex)
+    for c in range(nc):
+        x[c] = (1.0 + 0.5j * c) * np.outer(v1[:, 0], v2[:, 0])
+    say("  cond(S1 x S2) %.2e" % (w1[-1] * w2[-1] / lam))
+    t = 55.0
+    x32 = torch.from_numpy(x).to("cuda").to(torch.complex64)
+
+    def overlap32():
+        d = torch.empty_like(x32)
+        fh.MultiplyOverlap(x32, d, t, alpha=0.0, beta=1.0)
+        return d.cpu().numpy().astype(complex)
+
+    e_entry = rel(overlap32(), lam * x)
+    prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-10, factored=fh)
+    prop.SetupStep(0.01)  # attaches the Cholesky factors to fh
+    e_chol = rel(overlap32(), lam * x)
+    say("  smallest-eigenvalue direction: entry by entry %.2e, Cholesky form %.2e" % (e_entry, e_chol))
+    cond = w1[-1] * w2[-1] / lam
+    assert e_chol < 10 * np.finfo(np.float32).eps * np.sqrt(cond)
+    assert e_chol != e_entry  # the Cholesky path ran
+    xr = crandn(rng, sysm.shape)
+    src = torch.from_numpy(xr).to("cuda")
+    d64 = torch.empty_like(src)
+    fh.MultiplyOverlap(src, d64, t, alpha=0.005j, beta=1.0)
+    d32 = torch.empty_like(src, dtype=torch.complex64)
+    fh.MultiplyOverlap(src.to(torch.complex64), d32, t, alpha=0.005j, beta=1.0)
+    e = rel(d32.cpu().numpy(), d64.cpu().numpy())
+    say("  S + 0.005i H, FP32 vs FP64: %.2e" % e)
+    assert e < 2e-6
+
+
 def test_factored_radial_variants_vs_dense(torch, rng):
     """Every radial kernel shape (terms x channel groups per wave) against the dense operator.
     The small system has 13 channels (2 groups of 8, the second partly empty; 3 groups per
