@@ -493,6 +493,9 @@ def main():
     ap.add_argument("--profile-learned", type=int, default=0,
                     help="only train the network on a few trajectories, then run this many mixed steps with the "
                          "learned preconditioner (for rocprof)")
+    ap.add_argument("--device-mode", type=int, default=0, choices=(0, 1),
+                    help="mixed solver: 1 = solve the inner least-squares problems on the device and "
+                         "synchronise once per step (pp_mixed_set_device_mode)")
     ap.add_argument("--gram-schmidt", type=int, default=1, choices=(1, 2),
                     help="classical Gram-Schmidt passes per inner FGMRES iteration of the mixed solver")
     ap.add_argument("--profile-mixed", type=int, default=0,
@@ -560,7 +563,8 @@ def main():
         learned.fit(x, ch, tt, epochs=2, batch=256, lr=1e-3, seed=1)
         psi.data.copy_(psi0)
         prop = MixedCayleyPropagator(psi, base, kron, learned=learned, tolerance=1e-10,
-                                     factored=tuned_factored(base)[0], gram_schmidt_passes=a.gram_schmidt)
+                                     factored=tuned_factored(base)[0], gram_schmidt_passes=a.gram_schmidt,
+                                     device_mode=bool(a.device_mode))
         prop.SetupStep(dt)
         sec, _ = run_steps(torch, prop, psi, t0, dt, 2, a.profile_learned)
         log("%d learned mixed steps %.2f ms/step, last %s" % (a.profile_learned, 1e3 * sec / a.profile_learned,
@@ -568,7 +572,8 @@ def main():
         return
     if a.profile_mixed:
         prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-10,
-                                     factored=tuned_factored(base)[0], gram_schmidt_passes=a.gram_schmidt)
+                                     factored=tuned_factored(base)[0], gram_schmidt_passes=a.gram_schmidt,
+                                     device_mode=bool(a.device_mode))
         prop.SetupStep(dt)
         sec, _ = run_steps(torch, prop, psi, t0, dt, 2, a.profile_mixed)
         log("%d mixed steps %.2f ms/step, last %s" % (a.profile_mixed, 1e3 * sec / a.profile_mixed, prop.GetStats()))
@@ -725,7 +730,7 @@ def main():
     # 4. mixed precision: FP32 inner FGMRES, FP64 refinement; Kronecker alone and with the network
     for name, lrn in (("fp32_kron", None), ("fp32_learned", learned)):
         prop = MixedCayleyPropagator(psi, base, kron, lrn, fallback=plain, tolerance=1e-10, factored=fh,
-                                     gram_schmidt_passes=a.gram_schmidt)
+                                     gram_schmidt_passes=a.gram_schmidt, device_mode=bool(a.device_mode))
         sps, sec, t_e, end, st, chk = timed(prop, name)
         runs[name] = dict(steps_per_s=sps, ms_per_step=1e3 * sec / a.steps, error_vs_plain=err_vs(end),
                           stats=agg(st), check=chk,
@@ -786,6 +791,7 @@ def main():
                        "fp32_learned_kron_factored": lrn["steps_per_s"]},
                    "runs": runs,
                    "gram_schmidt_passes": a.gram_schmidt,
+                   "device_mode": a.device_mode,
                    "wavefunction_error_vs_oracle": oracle_err,
                    "oracle_check_steps": a.cpu_steps if not a.no_cpu_baseline else 0,
                    "wavefunction_error_vs_plain_fp64": lrn["error_vs_plain"], "norm_error": lrn["norm_error"],
