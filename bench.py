@@ -496,6 +496,8 @@ def main():
     ap.add_argument("--device-mode", type=int, default=0, choices=(0, 1),
                     help="mixed solver: 1 = solve the inner least-squares problems on the device and "
                          "synchronise once per step (pp_mixed_set_device_mode)")
+    ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
+                    help="mixed solver in device mode: 1 = replay each step from a captured hipGraph")
     ap.add_argument("--gram-schmidt", type=int, default=1, choices=(1, 2),
                     help="classical Gram-Schmidt passes per inner FGMRES iteration of the mixed solver")
     ap.add_argument("--profile-mixed", type=int, default=0,
@@ -564,7 +566,7 @@ def main():
         psi.data.copy_(psi0)
         prop = MixedCayleyPropagator(psi, base, kron, learned=learned, tolerance=1e-10,
                                      factored=tuned_factored(base)[0], gram_schmidt_passes=a.gram_schmidt,
-                                     device_mode=bool(a.device_mode))
+                                     device_mode=bool(a.device_mode or a.graph), graph=bool(a.graph))
         prop.SetupStep(dt)
         sec, _ = run_steps(torch, prop, psi, t0, dt, 2, a.profile_learned)
         log("%d learned mixed steps %.2f ms/step, last %s" % (a.profile_learned, 1e3 * sec / a.profile_learned,
@@ -573,7 +575,7 @@ def main():
     if a.profile_mixed:
         prop = MixedCayleyPropagator(psi, base, KroneckerPreconditioner(psi, sysm), tolerance=1e-10,
                                      factored=tuned_factored(base)[0], gram_schmidt_passes=a.gram_schmidt,
-                                     device_mode=bool(a.device_mode))
+                                     device_mode=bool(a.device_mode or a.graph), graph=bool(a.graph))
         prop.SetupStep(dt)
         sec, _ = run_steps(torch, prop, psi, t0, dt, 2, a.profile_mixed)
         log("%d mixed steps %.2f ms/step, last %s" % (a.profile_mixed, 1e3 * sec / a.profile_mixed, prop.GetStats()))
@@ -730,7 +732,8 @@ def main():
     # 4. mixed precision: FP32 inner FGMRES, FP64 refinement; Kronecker alone and with the network
     for name, lrn in (("fp32_kron", None), ("fp32_learned", learned)):
         prop = MixedCayleyPropagator(psi, base, kron, lrn, fallback=plain, tolerance=1e-10, factored=fh,
-                                     gram_schmidt_passes=a.gram_schmidt, device_mode=bool(a.device_mode))
+                                     gram_schmidt_passes=a.gram_schmidt, device_mode=bool(a.device_mode or a.graph),
+                                     graph=bool(a.graph))
         sps, sec, t_e, end, st, chk = timed(prop, name)
         runs[name] = dict(steps_per_s=sps, ms_per_step=1e3 * sec / a.steps, error_vs_plain=err_vs(end),
                           stats=agg(st), check=chk,
@@ -791,7 +794,8 @@ def main():
                        "fp32_learned_kron_factored": lrn["steps_per_s"]},
                    "runs": runs,
                    "gram_schmidt_passes": a.gram_schmidt,
-                   "device_mode": a.device_mode,
+                   "device_mode": int(a.device_mode or a.graph),
+                   "graph": a.graph,
                    "wavefunction_error_vs_oracle": oracle_err,
                    "oracle_check_steps": a.cpu_steps if not a.no_cpu_baseline else 0,
                    "wavefunction_error_vs_plain_fp64": lrn["error_vs_plain"], "norm_error": lrn["norm_error"],

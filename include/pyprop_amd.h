@@ -560,6 +560,15 @@ int pp_mixed_set_device_mode(pp_mixed* m, int device_mode);
  * only to ~eps32 / h_{j+1,j}, which the FP64 refinement absorbs (at helium: the same 3 inner
  * iterations per correction and 1e-11 error after 20 steps).  2: two passes (4 launches). */
 int pp_mixed_set_gram_schmidt(pp_mixed* m, int passes);
+/* Graph mode (no reference counterpart; with device mode 1): a device-mode step is one fixed
+ * sequence of launches for given correction and inner-iteration counts; it is captured into
+ * a hipGraph once, after the same sequence has run eagerly, and replayed on the caller's
+ * stream for every later step with the same counts (recaptured when they change).  The time
+ * scalings are staged in device memory each step.  Needs the factored operator with a sparse
+ * angular form in both precisions (the tuned form at helium); otherwise the step runs
+ * eagerly.  Results are bitwise those of eager device mode. */
+int pp_mixed_set_graph(pp_mixed* m, int graph_mode);
+int pp_mixed_graph_stats(const pp_mixed* m, int* captures, int* launches);
 int pp_mixed_step(pp_mixed* m, pp_cplx* psi, double dt, const double* scalings, pp_stream stream);
 int pp_mixed_stats(const pp_mixed* m, int* counts, double* residuals);
 

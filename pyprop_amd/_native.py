@@ -112,6 +112,8 @@ SIGNATURES = [
     ("pp_mixed_set_factored", I, [P, P]),
     ("pp_mixed_set_device_mode", I, [P, I]),
     ("pp_mixed_set_gram_schmidt", I, [P, I]),
+    ("pp_mixed_set_graph", I, [P, I]),
+    ("pp_mixed_graph_stats", I, [P, P, P]),
     ("pp_cayley_set_factored", I, [P, P]),
     ("pp_fham_create", I, [I, I, I, I, P, P, P, I, P]),
     ("pp_fham_destroy", I, [P]),

@@ -692,7 +692,9 @@ __global__ __launch_bounds__(64 * W) void fham_angular_rows_kernel(int nc, int K
 // 18 x 42 x 42 entries are non-zero (the H0 pieces are diagonal in the channel, each
 // multipole and the dipole couple a few channels), so the GEMM above issues ~25 times the
 // useful products.  Here Y[c'][p] = sum over the non-zeros (t, c) of row c' of
-// f_t A_t[c'][c] Z[t][c][p].  One workgroup of kAngSparseWaves waves per (output channel,
+// f_t A_t[c'][c] Z[t][c][p] (f_t by value, or with scd from a per-potential array in device
+// memory, so that a captured graph reads the scalings of each replay).  One workgroup of
+// kAngSparseWaves waves per (output channel,
 // chunk of 64 PV positions): the row's list -- Z row index t nc + c, coefficient, term --
 // is split into contiguous parts, one per wave, read through scalar loads (uniform in the
 // wave); lane l accumulates positions chunk + 64 r + l (r < PV), so every Z load is one
@@ -707,7 +709,8 @@ template <class T, int PV>
 __global__ __launch_bounds__(64 * kAngSparseWaves) void fham_angular_sparse_kernel(
     int nc, long long P, int nchunks, const int* __restrict__ rowptr, const int* __restrict__ colk,
     const T* __restrict__ coef, const int* __restrict__ cterm, const typename MfmaT<T>::C2* __restrict__ Z,
-    FScales<T> sc, const typename MfmaT<T>::C2* addend, typename MfmaT<T>::C2 alpha,
+    FScales<T> sc, const double* __restrict__ scd, const int* __restrict__ cpot,
+    const typename MfmaT<T>::C2* addend, typename MfmaT<T>::C2 alpha,
     const typename MfmaT<T>::C2* __restrict__ zs, typename MfmaT<T>::C2 beta, typename MfmaT<T>::C2* out,
     typename MfmaT<T>::C2* out2, typename MfmaT<T>::C2 alpha2, typename MfmaT<T>::C2 beta2, int* __restrict__ reset) {
   typedef typename MfmaT<T>::C2 C2;
@@ -741,7 +744,9 @@ __global__ __launch_bounds__(64 * kAngSparseWaves) void fham_angular_sparse_kern
       const bool have = jb + q < j1;     // uniform
       const int j = have ? jb + q : jb;  // past the part's end: an in-bounds entry, weight 0
       const long long base = (long long)colk[j] * P;
-      a[q] = have ? coef[j] * sc.s[cterm[j]] : T(0);
+      // the time scaling: by value per term, or (scd) per potential from device memory
+      const T f = scd ? (T)scd[cpot[j]] : sc.s[cterm[j]];
+      a[q] = have ? coef[j] * f : T(0);
 #pragma unroll
       for (int r = 0; r < PV; ++r) {
         if (have && ok[r]) z[q][r] = Z[base + p0 + 64 * r];
@@ -963,7 +968,7 @@ template <class T>
 static void fham_run(const FHam& F, const double* scalings, const typename MfmaT<T>::C2* src, const FOut<T>& o1,
                      const FOut<T>* o2, const DeviceBuffer<T>& Ab, const DeviceBuffer<T>& Db,
                      const DeviceBuffer<T>& RPb, const DeviceBuffer<T>& SAb, const DeviceBuffer<T>& SBb, DeviceBuffer<T>& Zb,
-                     DeviceBuffer<T>& Xb, const T* ref, int* differs, hipStream_t s) {
+                     DeviceBuffer<T>& Xb, const T* ref, int* differs, hipStream_t s, const double* sc_dev) {
   const int nc = F.nc, n = F.n, nterms = F.nterms;
   const long long P = (long long)n * n;
   auto nz = [](double2 v) { return v.x != 0.0 || v.y != 0.0; };
@@ -1053,7 +1058,8 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
                                    : reinterpret_cast<const T*>(F.sp_coef32.ptr);
 #define PP_ANGS(V)                                                                                                   \
   hipLaunchKernelGGL((fham_angular_sparse_kernel<T, V>), sgrid, dim3(64 * kAngSparseWaves), 0, s, nc, P, nchunks,   \
-                     F.sp_rowptr.ptr, F.sp_k.ptr, coef, F.sp_term.ptr, Zc, sc, o1.addend, to_c2<T>(o1.alpha),        \
+                     F.sp_rowptr.ptr, F.sp_k.ptr, coef, F.sp_term.ptr, Zc, sc, sc_dev, F.sp_pot.ptr, o1.addend,     \
+                     to_c2<T>(o1.alpha),                                                                             \
                      with_s ? Zc + (size_t)nterms * nc * P : nullptr, to_c2<T>(o1.beta), o1.dst,                     \
                      o2 ? o2->dst : nullptr, to_c2<T>(o2 ? o2->alpha : make_double2(0, 0)),                          \
                      to_c2<T>(o2 ? o2->beta : make_double2(0, 0)), differs)
@@ -1112,17 +1118,17 @@ void fham_apply64(const FHam& f, const double* scalings, const double2* src, dou
 }
 
 void FHam::apply64(const double* scalings, const double2* src, double2* dst, const double2* addend, double2 alpha,
-                   hipStream_t s, double2 beta) const {
+                   hipStream_t s, double2 beta, const double* sc_dev) const {
   FHamOut o;
   o.dst = dst;
   o.addend = addend;
   o.alpha = alpha;
   o.beta = beta;
-  apply64_ex(scalings, src, o, nullptr, nullptr, s);
+  apply64_ex(scalings, src, o, nullptr, nullptr, s, sc_dev);
 }
 
 void FHam::apply64_ex(const double* scalings, const double2* src, const FHamOut& o1, const FHamOut* o2,
-                      const double2* reuse_ref, hipStream_t s) const {
+                      const double2* reuse_ref, hipStream_t s, const double* sc_dev) const {
   auto nz = [](double2 v) { return v.x != 0.0 || v.y != 0.0; };
   const int nrad = nterms + ((nz(o1.beta) || (o2 && nz(o2->beta))) ? 1 : 0);
   PP_REQUIRE(!reuse_ref || w64_terms >= nrad, "held radial products do not cover the requested terms");
@@ -1143,20 +1149,23 @@ void FHam::apply64_ex(const double* scalings, const double2* src, const FHamOut&
     a2.beta = o2->beta;
   }
   fham_run<double>(*this, scalings, src, a1, o2 ? &a2 : nullptr, A64, bt64, rp64, sa64, sb64, W64, X64,
-                   reinterpret_cast<const double*>(reuse_ref), reuse_ref ? reuse_flag.ptr : nullptr, s);
+                   reinterpret_cast<const double*>(reuse_ref), reuse_ref ? reuse_flag.ptr : nullptr, s, sc_dev);
   // when the products were reused they cover at least nrad terms; when recomputed, exactly nrad
   w64_terms = nrad;
 }
 
 void FHam::apply32(const double* scalings, const float2* src, float2* dst, const float2* addend, double2 alpha,
-                   hipStream_t s, double2 beta) const {
+                   hipStream_t s, double2 beta, const double* sc_dev) const {
   FOut<float> o;
   o.dst = dst;
   o.addend = addend;
   o.alpha = alpha;
   o.beta = beta;
-  fham_run<float>(*this, scalings, src, o, nullptr, A32, bt32, rp32, sa32, sb32, W32, X32, nullptr, nullptr, s);
+  fham_run<float>(*this, scalings, src, o, nullptr, A32, bt32, rp32, sa32, sb32, W32, X32, nullptr, nullptr, s,
+                  sc_dev);
 }
+
+bool FHam::sparse_angular() const { return kAngSparsePV[ang_variant64] > 0 && kAngSparsePV[ang_variant32] > 0; }
 
 }  // namespace pp
 
@@ -1299,7 +1308,7 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
     for (int k = 0; k < F.Kp; ++k) tk[k] = std::min(k / nchan, nterms - 1);  // term of column k
     // the same matrix as row lists of its non-zeros for the sparse angular form: row c' holds
     // (Z row t nc + c, A_t[c'][c], t) in increasing (t, c), terms in the reordered numbering
-    std::vector<int> sp_rowptr(nchan + 1, 0), sp_k, sp_term;
+    std::vector<int> sp_rowptr(nchan + 1, 0), sp_k, sp_term, sp_pot;
     std::vector<double> sp_coef;
     for (int r = 0; r < nchan; ++r) {
       for (int t = 0; t < nterms; ++t)
@@ -1309,6 +1318,7 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
           sp_k.push_back(t * nchan + c);
           sp_coef.push_back(v);
           sp_term.push_back(t);
+          sp_pot.push_back(F.term_pot[t]);
         }
       sp_rowptr[r + 1] = (int)sp_k.size();
     }
@@ -1316,6 +1326,7 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
       sp_k.push_back(0);
       sp_coef.push_back(0.0);
       sp_term.push_back(0);
+      sp_pot.push_back(0);
     }
     std::vector<double> bt = band_tiles(F.nband, n, bw, dband.data());
     const size_t nbt = bt.size();
@@ -1328,6 +1339,7 @@ extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double
       F.sp_rowptr.upload(sp_rowptr.data(), sp_rowptr.size(), s);
       F.sp_k.upload(sp_k.data(), sp_k.size(), s);
       F.sp_term.upload(sp_term.data(), sp_term.size(), s);
+      F.sp_pot.upload(sp_pot.data(), sp_pot.size(), s);
       F.sp_coef64.upload(sp_coef.data(), sp_coef.size(), s);
       std::vector<float> spf(sp_coef.begin(), sp_coef.end());
       F.sp_coef32.upload(spf.data(), spf.size(), s);

@@ -1500,36 +1500,51 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   const double2 ka = make_double2(0.0, a);
   // with an overlap term in the factored operator, the FP64 S + c H is one multiply
   const bool fused_s = fham && fham->has_overlap;
-  // b = S psi - i a H psi ; r = b - A psi = -2 i a H psi ; x = psi
-  if (fused_s) {
-    // Both from one multiply.  The last residual of the previous step formed the radial
-    // products (overlap term included) at the accepted solution; when psi still holds it --
-    // compared on the device -- only the angular stage with the new time scalings runs.
-    const bool reuse = z_valid && z_psi == psi && fham->gen64 == z_gen;
-    FHamOut ob, orr;
-    ob.dst = work.b64.ptr;
-    ob.alpha = make_double2(0.0, -a);
-    ob.beta = make_double2(1.0, 0.0);
-    orr.dst = work.r64.ptr;
-    orr.alpha = make_double2(0.0, -2.0 * a);
-    orr.beta = make_double2(0.0, 0.0);
-    fham->apply64_ex(scal.data(), psi, ob, &orr, reuse ? work.x64.ptr : nullptr, s);
-  } else {
-    vec_copy(n, psi, work.t64.ptr, s);
-    R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
-    if (fham) fham->apply64(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), s);
-    else H.apply(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), s);
-    vec_combine(n, make_double2(1.0, 0.0), work.t64.ptr, make_double2(0.0, -a), work.h64.ptr, work.b64.ptr, s);
-    vec_combine(n, make_double2(0.0, 0.0), work.h64.ptr, make_double2(0.0, -2.0 * a), work.h64.ptr, work.r64.ptr,
-                s);
+  // The step's time scalings also go to device memory (through pinned host memory, in stream
+  // order): the sparse angular form reads them there, so that a captured step replays with
+  // each step's values.
+  const double* scd = nullptr;
+  if (fham && fham->sparse_angular()) {
+    PP_REQUIRE(scal.size() <= (size_t)kMaxFTerms, "too many potentials");
+    if (!scal_host) PP_HIP(hipHostMalloc((void**)&scal_host, kMaxFTerms * sizeof(double), hipHostMallocDefault));
+    std::copy(scal.begin(), scal.end(), scal_host);
+    if (scal_dev.count < scal.size()) scal_dev.resize(scal.size());
+    PP_HIP(hipMemcpyAsync(scal_dev.ptr, scal_host, scal.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    scd = scal_dev.ptr;
   }
-  z_valid = false;
-  ++stats.outer_ops;
+  hipStream_t cs = s;  // the stream the step's work is issued on (the capture stream while capturing)
   // |b|^2 and |r|^2: written to the pinned block for the host and to the device for the
   // scaling of V_0; the host reads them at the first synchronisation of the first correction
   constexpr int kB = MixedWork::kSmall - 4, kR0 = MixedWork::kSmall - 3, kR = MixedWork::kSmall - 2;
   constexpr int kEst = MixedWork::kSmall - 8 - 8 * kDevLsMax;  // device-mode residual estimates
-  work.norm2_pair(work.b64.ptr, work.r64.ptr, work.small_dev + kB, work.norms.ptr, s);
+  // b = S psi - i a H psi ; r = b - A psi = -2 i a H psi ; x = psi
+  const bool reuse = fused_s && z_valid && z_psi == psi && fham->gen64 == z_gen;
+  auto issue_start = [&]() {
+    if (fused_s) {
+      // Both from one multiply.  The last residual of the previous step formed the radial
+      // products (overlap term included) at the accepted solution; when psi still holds it --
+      // compared on the device -- only the angular stage with the new time scalings runs.
+      FHamOut ob, orr;
+      ob.dst = work.b64.ptr;
+      ob.alpha = make_double2(0.0, -a);
+      ob.beta = make_double2(1.0, 0.0);
+      orr.dst = work.r64.ptr;
+      orr.alpha = make_double2(0.0, -2.0 * a);
+      orr.beta = make_double2(0.0, 0.0);
+      fham->apply64_ex(scal.data(), psi, ob, &orr, reuse ? work.x64.ptr : nullptr, cs, scd);
+    } else {
+      vec_copy(n, psi, work.t64.ptr, cs);
+      R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, cs);
+      if (fham) fham->apply64(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), cs,
+                              make_double2(0.0, 0.0), scd);
+      else H.apply(scal.data(), psi, work.h64.ptr, nullptr, make_double2(1.0, 0.0), cs);
+      vec_combine(n, make_double2(1.0, 0.0), work.t64.ptr, make_double2(0.0, -a), work.h64.ptr, work.b64.ptr, cs);
+      vec_combine(n, make_double2(0.0, 0.0), work.h64.ptr, make_double2(0.0, -2.0 * a), work.h64.ptr,
+                  work.r64.ptr, cs);
+    }
+    ++stats.outer_ops;
+    work.norm2_pair(work.b64.ptr, work.r64.ptr, work.small_dev + kB, work.norms.ptr, cs);
+  };
 
   auto apply_A32 = [&](const float2* in, float2* out) {
     // out = S in + i a H in.  S stays in its factored form U^T U here even when the factored
@@ -1537,11 +1552,12 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     // eigenvalues (relative error ~ eps32 cond(S (x) S)), and the refinement then cannot
     // reduce the error in those directions, where the residual barely sees it.
     if (fham && fham->csd == fham->bw && fham->sep_enabled) {  // S in Cholesky form inside the multiply
-      fham->apply32(scal.data(), in, out, nullptr, make_double2(0.0, a), s, make_double2(1.0, 0.0));
+      fham->apply32(scal.data(), in, out, nullptr, make_double2(0.0, a), cs, make_double2(1.0, 0.0), scd);
     } else {
-      work.overlap_multiply32(R, in, work.tmp.ptr, work.w.ptr, s);
-      if (fham) fham->apply32(scal.data(), in, out, work.tmp.ptr, make_double2(0.0, a), s);
-      else H.apply_f32(scal.data(), in, out, work.tmp.ptr, make_float2(0.f, (float)a), s);
+      work.overlap_multiply32(R, in, work.tmp.ptr, work.w.ptr, cs);
+      if (fham) fham->apply32(scal.data(), in, out, work.tmp.ptr, make_double2(0.0, a), cs, make_double2(0.0, 0.0),
+                              scd);
+      else H.apply_f32(scal.data(), in, out, work.tmp.ptr, make_float2(0.f, (float)a), cs);
     }
     ++stats.inner_ops;
   };
@@ -1555,13 +1571,13 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     const bool use = learned && (learned_mode == 2 || (j == 0 && (learned_mode == 1 || correction == 0)));
     if (use) {
       if (n2) {  // V_j is normalised here (the Kronecker path does it while loading)
-        hipLaunchKernelGGL(unit32_dev_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, n2, in);
+        hipLaunchKernelGGL(unit32_dev_kernel, dim3(grid_n(n)), dim3(256), 0, cs, n, n2, in);
         PP_HIP(hipGetLastError());
       }
-      learned_apply(*learned, K, ka, in, out, s);
+      learned_apply(*learned, K, ka, in, out, cs);
       ++stats.learned_applies;
     } else {
-      K.apply32(ka, in, out, s, 1.f, n2 != nullptr, n2);
+      K.apply32(ka, in, out, cs, 1.f, n2 != nullptr, n2);
     }
     ++stats.precond_applies;
   };
@@ -1570,12 +1586,14 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   // r = b - (S x + i a H x) into r64 (the residual pass follows)
   auto residual_multiply = [&]() {
     if (fused_s) {
-      fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, nullptr, make_double2(0.0, a), s, make_double2(1.0, 0.0));
+      fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, nullptr, make_double2(0.0, a), cs,
+                    make_double2(1.0, 0.0), scd);
     } else {
-      vec_copy(n, work.x64.ptr, work.t64.ptr, s);
-      R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, s);
-      if (fham) fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
-      else H.apply(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), s);
+      vec_copy(n, work.x64.ptr, work.t64.ptr, cs);
+      R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, cs);
+      if (fham) fham->apply64(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), cs,
+                              make_double2(0.0, 0.0), scd);
+      else H.apply(scal.data(), work.x64.ptr, work.r64.ptr, work.t64.ptr, make_double2(0.0, a), cs);
     }
     ++stats.outer_ops;
   };
@@ -1601,21 +1619,86 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
                                                                                                   : 0;
   for (int c = 0; c < dev_corr; ++c)
     if (pre_its[c] > kDevLsMax || pre_its[c] > m) dev_corr = 0;
-  if (dev_corr > 0) {
+  auto issue_device = [&]() {
     for (int c = 0; c < dev_corr; ++c) {
       ++stats.outer_iterations;
-      hipLaunchKernelGGL(f64_to_f32_unit_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, pending_dev, work.r64.ptr,
+      hipLaunchKernelGGL(f64_to_f32_unit_kernel, dim3(grid_n(n)), dim3(256), 0, cs, n, pending_dev, work.r64.ptr,
                          work.V.ptr);
       work.fgmres32_device(apply_A32, apply_P32, pre_its[c], pending_dev, x_is_psi ? psi : work.x64.ptr,
-                           work.x64.ptr, kEst + c * kDevLsMax, s);
+                           work.x64.ptr, kEst + c * kDevLsMax, cs);
       stats.inner_iterations += pre_its[c];
       x_is_psi = false;
       ++correction;
       residual_multiply();
-      work.residual(work.b64.ptr, work.r64.ptr, work.small_dev + kR, work.norms.ptr + 2, s);
+      work.residual(work.b64.ptr, work.r64.ptr, work.small_dev + kR, work.norms.ptr + 2, cs);
       pending = kR;
       pending_dev = work.norms.ptr + 2;
     }
+  };
+  // Graph mode: a device-mode step is one fixed sequence of launches for given counts, so it
+  // is captured once (on an internal stream, in relaxed mode) and replayed on s; the time
+  // scalings come from device memory.  A sequence is captured only after it has run
+  // eagerly with the same counts (every buffer it uses has then reached its size, so the
+  // capture allocates and clears nothing).
+  std::vector<long long> key;
+  if (dev_corr > 0) {
+    key = {dev_corr, reuse ? 1 : 0, (long long)(intptr_t)psi, learned ? learned_mode : -1};
+    for (int c = 0; c < dev_corr; ++c) key.push_back(pre_its[c]);
+  }
+  const bool use_graph = dev_corr > 0 && graph_mode && graph_ok && scd &&
+                         (!reuse || fham->reuse_flag.count > 0) && ((gexec && key == gkey) || key == eager_key);
+  if (use_graph) {
+    if (!gexec || key != gkey) {
+      if (!cap_stream) PP_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+      cs = cap_stream;
+      PP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+      hipGraph_t g = nullptr;
+      try {
+        issue_start();
+        issue_device();
+      } catch (...) {
+        (void)hipStreamEndCapture(cs, &g);
+        if (g) (void)hipGraphDestroy(g);
+        cs = s;
+        graph_ok = false;
+        throw;
+      }
+      const hipError_t e = hipStreamEndCapture(cs, &g);
+      cs = s;
+      if (e != hipSuccess || !g) {
+        graph_ok = false;
+        throw Error(PP_ERR_HIP, std::string("capture of the mixed step failed: ") + hipGetErrorString(e));
+      }
+      if (gexec) (void)hipGraphExecDestroy(gexec);
+      gexec = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) {
+        gexec = nullptr;
+        graph_ok = false;
+        throw Error(PP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+      }
+      gkey = key;
+      gstats = stats;  // what issuing the sequence counts, applied again on every replay
+      ++graph_captures;
+    } else {
+      stats = gstats;
+      correction = dev_corr;
+      x_is_psi = false;
+      pending = kR;
+      pending_dev = work.norms.ptr + 2;
+    }
+    PP_HIP(hipGraphLaunch(gexec, s));
+    ++graph_launches;
+  } else {
+    issue_start();
+    if (dev_corr > 0) {
+      issue_device();
+      eager_key = key;
+    }
+  }
+  z_valid = false;
+  if (dev_corr > 0) {
     PP_HIP(hipStreamSynchronize(s));
     for (int c = 0; c < dev_corr; ++c) {
       // the next step runs as many iterations as this correction needed to reach inner_rtol
@@ -1688,6 +1771,12 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     return;
   }
   if (!x_is_psi) vec_copy(n, work.x64.ptr, psi, s);
+}
+
+MixedCayley::~MixedCayley() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (cap_stream) (void)hipStreamDestroy(cap_stream);
+  if (scal_host) (void)hipHostFree(scal_host);
 }
 
 }  // namespace pp
@@ -1821,6 +1910,22 @@ extern "C" int pp_mixed_set_gram_schmidt(pp_mixed* m, int passes) {
   return guarded([&] {
     PP_REQUIRE(m && (passes == 1 || passes == 2), "Gram-Schmidt passes must be 1 or 2");
     m->c.work.gs_passes = passes;
+  });
+}
+
+extern "C" int pp_mixed_set_graph(pp_mixed* m, int graph_mode) {
+  return guarded([&] {
+    PP_REQUIRE(m, "null argument");
+    m->c.graph_mode = graph_mode != 0;
+    m->c.graph_ok = true;
+  });
+}
+
+extern "C" int pp_mixed_graph_stats(const pp_mixed* m, int* captures, int* launches) {
+  return guarded([&] {
+    PP_REQUIRE(m && captures && launches, "null argument");
+    *captures = m->c.graph_captures;
+    *launches = m->c.graph_launches;
   });
 }
 

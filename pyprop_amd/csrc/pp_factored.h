@@ -27,7 +27,7 @@ struct FHam {
   DeviceBuffer<float> A32, bt32;      // bt: band tiles in MFMA lane order [T][n][w][tiles][KS][64]
   // the angular matrix as row lists of its non-zeros (sparse angular form): row c' holds
   // entries [sp_rowptr[c'], sp_rowptr[c' + 1]) of (Z row t nc + c, A_t[c'][c], term t)
-  DeviceBuffer<int> sp_rowptr, sp_k, sp_term;
+  DeviceBuffer<int> sp_rowptr, sp_k, sp_term, sp_pot;  // sp_pot: the entry's potential (time scaling) index
   DeviceBuffer<double> sp_coef64;
   DeviceBuffer<float> sp_coef32;
   mutable DeviceBuffer<double> W64;   // radial products Z [T][nc][n][n] complex
@@ -64,16 +64,20 @@ struct FHam {
   // dst = addend + alpha * sum_t scalings[term_pot[t]] (A_t (x) R_t) src + beta (S1 (x) S2) src
   // (beta must be 0 without an overlap term; the overlap radial product is only formed when
   // a beta is non-zero)
+  // sc_dev (optional): the same scalings in device memory, [npot] doubles, read by the sparse
+  // angular form (so that a captured graph sees each replay's values); the GEMM forms take
+  // the host values as kernel arguments
   void apply64(const double* scalings, const double2* src, double2* dst, const double2* addend, double2 alpha,
-               hipStream_t s, double2 beta = make_double2(0.0, 0.0)) const;
+               hipStream_t s, double2 beta = make_double2(0.0, 0.0), const double* sc_dev = nullptr) const;
   void apply32(const double* scalings, const float2* src, float2* dst, const float2* addend, double2 alpha,
-               hipStream_t s, double2 beta = make_double2(0.0, 0.0)) const;
+               hipStream_t s, double2 beta = make_double2(0.0, 0.0), const double* sc_dev = nullptr) const;
+  bool sparse_angular() const;  // both precisions use a sparse angular form (device scalings honoured)
   // FP64 multiply with up to two outputs from the same products (o2 may be NULL and takes no
   // addend).  reuse_ref (may be NULL): the input of an earlier FP64 multiply whose radial
   // products W64 still holds (covering the terms needed now); the radial stage is skipped when
   // src equals it element for element, checked on the device.
   void apply64_ex(const double* scalings, const double2* src, const FHamOut& o1, const FHamOut* o2,
-                  const double2* reuse_ref, hipStream_t s) const;
+                  const double2* reuse_ref, hipStream_t s, const double* sc_dev = nullptr) const;
 };
 
 

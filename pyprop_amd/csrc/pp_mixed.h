@@ -132,7 +132,20 @@ struct MixedCayley {
   std::vector<int> pre_its;  // per correction: inner iterations run before the first host read
   bool device_mode = false;  // after a first host-driven step: one host synchronisation per step
   int prev_corrections = 0;  // corrections the last converged step needed
+  // Graph mode (with device mode and a sparse angular form): the device-mode step is
+  // captured once per (corrections, inner counts, reuse, psi) and replayed; the time
+  // scalings reach the kernels through scal_dev.
+  bool graph_mode = false;
+  bool graph_ok = true;          // cleared when a capture fails (the step then runs eagerly)
+  DeviceBuffer<double> scal_dev;
+  double* scal_host = nullptr;   // pinned staging of the scalings
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<long long> gkey, eager_key;  // the captured sequence's key; the last eager one's
+  MixedStats gstats;             // the counts the captured sequence adds to a step's statistics
+  int graph_captures = 0, graph_launches = 0;
   void step(double2* psi, double dt, const double* scalings, hipStream_t s);
+  ~MixedCayley();
 };
 
 }  // namespace pp

@@ -320,14 +320,20 @@ class MixedCayleyPropagator:
     def __init__(self, psi, base, kron: KroneckerPreconditioner, learned: Optional[LearnedPreconditioner] = None,
                  fallback=None, fallback_iterations: int = 40, krylov_basis_size: int = 20,
                  tolerance: float = 1e-10, inner_tolerance: float = 1e-5, max_outer: int = 8, factored=None,
-                 device_mode: bool = False, learned_mode: int = 0, gram_schmidt_passes: int = 1):
+                 device_mode: bool = False, learned_mode: int = 0, gram_schmidt_passes: int = 1,
+                 graph: bool = False):
         """device_mode: after the first converged step, solve the inner least-squares problems
         on the device and synchronise once per step (pp_mixed_set_device_mode).
         learned_mode: the FGMRES directions the learned correction acts on: 0 the first of
         the first correction, 1 the first of every correction, 2 all
         (pp_mixed_set_learned_mode).
         gram_schmidt_passes: classical Gram-Schmidt passes per inner iteration (1 saves one
-        launch per iteration; pp_mixed_set_gram_schmidt)."""
+        launch per iteration; pp_mixed_set_gram_schmidt).
+        graph: with device_mode, replay each device-mode step from a captured hipGraph
+        (pp_mixed_set_graph; needs a factored operator whose angular forms are sparse)."""
+        if graph and not device_mode:
+            raise ValueError("graph mode needs device_mode")
+        self.graph = graph
         self.learned_mode = learned_mode
         self.gram_schmidt_passes = gram_schmidt_passes
         self.psi = psi
@@ -365,6 +371,7 @@ class MixedCayleyPropagator:
         N.check(N.load().pp_mixed_set_device_mode(h, int(self.device_mode)))
         N.check(N.load().pp_mixed_set_learned_mode(h, int(self.learned_mode)))
         N.check(N.load().pp_mixed_set_gram_schmidt(h, int(self.gram_schmidt_passes)))
+        N.check(N.load().pp_mixed_set_graph(h, int(self.graph)))
 
     def SetLearned(self, learned):
         self.learned = learned
@@ -383,6 +390,12 @@ class MixedCayleyPropagator:
         N.check(N.load().pp_mixed_stats(self.h, c, r))
         return dict(outer=c[0], inner=c[1], fp64_ops=c[2], fp32_ops=c[3], precond=c[4], learned=c[5],
                     fallback=c[6], residual=r[0], rel_residual=r[1])
+
+    def GetGraphStats(self) -> dict:
+        """Captures and replays of the device-mode step (graph mode)."""
+        c, l = C.c_int(), C.c_int()
+        N.check(N.load().pp_mixed_graph_stats(self.h, C.byref(c), C.byref(l)))
+        return {"captures": c.value, "launches": l.value}
 
     def __del__(self):
         if getattr(self, "h", None) is not None and N._lib is not None:

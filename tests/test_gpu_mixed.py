@@ -541,6 +541,45 @@ def test_factored_fp32_overlap_cholesky_form(torch, rng):
     assert e < 2e-6
 
 
+def test_mixed_graph_mode_equals_eager_device_mode(torch, rng):
+    """Graph mode: the device-mode step captured into a hipGraph and replayed (with the
+    program's own synchronisation and eager copy between replays, and the time scalings of
+    each step staged in device memory) gives, step after step, bitwise the states of eager
+    device mode; the sequence is captured after it has run eagerly and replayed afterwards
+    (recaptured when the counts change)."""
+    from pyprop_amd.factored import FactoredHamiltonian
+    from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
+    say("mixed step: graph mode vs eager device mode (small helium)")
+    sysm = _small_helium()
+    repr_, psi, base = _device(sysm, torch)
+    base.SetupStep()
+    fh = FactoredHamiltonian(base)
+    sparse = fh.ANGULAR_VARIANTS.index("sparse2")
+    fh.set_angular_variant(64, sparse)
+    fh.set_angular_variant(32, sparse)
+    x0 = torch.from_numpy(crandn(rng, sysm.shape)).to("cuda")
+    kp = KroneckerPreconditioner(psi, sysm)
+    dt, t0, steps = 0.01, 120.0, 12
+    traj = {}
+    for graph in (False, True):
+        psi.data.copy_(x0)
+        prop = MixedCayleyPropagator(psi, base, kp, tolerance=1e-10, factored=fh, device_mode=True, graph=graph)
+        t = t0
+        states = []
+        for _ in range(steps):
+            prop.AdvanceStep(t, dt)
+            t += dt
+            states.append(psi.data.clone())
+        traj[graph] = (states, prop.GetGraphStats(), prop.GetStats())
+        del prop
+    (eager, ge, se), (graphed, gg, sg) = traj[False], traj[True]
+    say("  eager %s %s, graph %s %s" % (ge, se, gg, sg))
+    assert ge["launches"] == 0 and gg["captures"] >= 1 and gg["launches"] >= steps // 2
+    for k in range(steps):
+        assert torch.equal(eager[k], graphed[k]), k
+    assert se["inner"] == sg["inner"] and se["fp32_ops"] == sg["fp32_ops"]
+
+
 def test_factored_radial_variants_vs_dense(torch, rng):
     """Every radial kernel shape (terms x channel groups per wave) against the dense operator.
     The small system has 13 channels (2 groups of 8, the second partly empty; 3 groups per
