@@ -356,6 +356,14 @@ int pp_kron_solve(const pp_kron* k, pp_cplx a, pp_cplx* psi, pp_stream stream);
 /* psi <- K psi (complex128) */
 int pp_kron_multiply(const pp_kron* k, pp_cplx a, pp_cplx* psi, pp_stream stream);
 int pp_kron_solve_f32(const pp_kron* k, pp_cplx a, pp_cplxf* psi, pp_stream stream);
+/* out <- K^-1 in in FP32 (no reference counterpart; in and out distinct, so the one-launch
+ * form can run). */
+int pp_kron_solve_f32_into(const pp_kron* k, pp_cplx a, const pp_cplxf* in, pp_cplxf* out, pp_stream stream);
+/* The FP32 solve's form for n <= 64 (no reference counterpart): 0 one workgroup per channel,
+ * 1 four workgroups per channel in two launches, 2 (the default) four per channel in one launch
+ * (each forms the whole middle product; used when input, output and write-back are distinct,
+ * else form 1).  Forms 1 and 2 give the same bits. */
+int pp_kron_set_form(pp_kron* k, int form);
 /* use K (a = i dt/2) as the left preconditioner of a Cayley solver created without a
  * block preconditioner (NULL detaches) */
 int pp_cayley_set_kron(pp_cayley* c, const pp_kron* k);

@@ -88,6 +88,8 @@ SIGNATURES = [
     ("pp_kron_destroy", I, [P]),
     ("pp_kron_solve", I, [P, PPCplx, P, P]),
     ("pp_kron_solve_f32", I, [P, PPCplx, P, P]),
+    ("pp_kron_solve_f32_into", I, [P, PPCplx, P, P, P]),
+    ("pp_kron_set_form", I, [P, I]),
     ("pp_cayley_set_kron", I, [P, P]),
     ("pp_learned_create", I, [I, I, I, C.c_ulonglong, P]),
     ("pp_learned_destroy", I, [P]),

@@ -114,11 +114,13 @@ __global__ __launch_bounds__(256) void finish_partials_kernel(int blocks, const 
 //   DOT:    partial[b][j] = V_j^H w (j < m), partial[b][m] = |w|^2, after the update;
 //           else partial[b][0] = |w|^2
 // The update and the dots of a block cover the same slice with the same thread-to-element
-// map, so each thread reads back only what it wrote.
+// map, so each thread reads back only what it wrote.  wout (UPDATE without DOT, may be NULL):
+// the updated vector goes there instead of back into w.
 template <bool UPDATE, bool DOT>
 __global__ __launch_bounds__(256) void cgs32_kernel(long long n, int m, const float2* __restrict__ V, long long ld,
                                                     const double2* __restrict__ hpart, int hwidth,
-                                                    float2* __restrict__ w, double2* __restrict__ partial) {
+                                                    float2* __restrict__ w, double2* __restrict__ partial,
+                                                    float2* __restrict__ wout = nullptr) {
   __shared__ double2 h64s[128];
   __shared__ float2 hs[128];
   __shared__ double2 sh[4];
@@ -144,7 +146,8 @@ __global__ __launch_bounds__(256) void cgs32_kernel(long long n, int m, const fl
       float2 wv = w[i];
       wv.x -= t.x;
       wv.y -= t.y;
-      w[i] = wv;
+      if (!DOT && wout) wout[i] = wv;
+      else w[i] = wv;
       nrm += (double)wv.x * wv.x + (double)wv.y * wv.y;
     }
   }
@@ -245,12 +248,13 @@ static int fused_blocks(long long n) {
   return (int)b;
 }
 
-// v <- v / sqrt(n2) with the squared norm on the device (a zero vector stays zero)
-__global__ void unit32_dev_kernel(long long n, const double2* __restrict__ n2, float2* __restrict__ v) {
+// y = x / sqrt(n2) with the squared norm on the device (a zero vector stays zero)
+__global__ void unit32_to_kernel(long long n, const double2* __restrict__ n2, const float2* __restrict__ x,
+                                 float2* __restrict__ y) {
   const double q = n2->x;
   const float sc = q > 0 ? (float)(1.0 / sqrt(q)) : 0.f;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    v[i] = make_float2(sc * v[i].x, sc * v[i].y);
+    y[i] = make_float2(sc * x[i].x, sc * x[i].y);
 }
 
 __global__ void scale32_kernel(long long n, float s, const float2* __restrict__ x, float2* __restrict__ y) {
@@ -952,12 +956,13 @@ __global__ __launch_bounds__(1024) void kron_mfma_kernel(const float* __restrict
 // Four waves per workgroup, wave w owns the 16-column tile w of each strip product.
 // Strip product of 16 x 64 (LDS, row stride KLD) by 64 x 64 (LDS): out(i, j) = sum_k
 // L(i, k) R(k, j) with L(i, k) = LT ? lm[k][i0 + i] : lm[i0 + i][k] and R(k, j) = RT ? rm[j][k] : rm[k][j]
+// (j0: the first output column of the wave's 16 x 16 tile)
 template <bool LT, bool RT>
 __device__ inline void kron_strip(const float* lm, int i0, const float* r_re, const float* r_im, bool r_complex,
-                                  const float* l_im, kf32x4& cre, kf32x4& cim) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                                  const float* l_im, int j0, kf32x4& cre, kf32x4& cim) {
+  const int lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
-  const int j = wave * 16 + lr;
+  const int j = j0 + lr;
 #pragma unroll 4
   for (int k0 = 0; k0 < 64; k0 += 4) {
     const int k = k0 + lk;
@@ -1034,7 +1039,7 @@ __global__ __launch_bounds__(1024) void kron_split1_kernel(const float* __restri
   const int lane = tid & 63, wave = tid >> 6;
   kf32x4 cre = {0.f, 0.f, 0.f, 0.f}, cim = {0.f, 0.f, 0.f, 0.f};
   if (wave < 4) {
-    kron_strip<true, false>(e1, i0, Bre, Bim, true, nullptr, cre, cim);  // U = E1[:, strip]^T B
+    kron_strip<true, false>(e1, i0, Bre, Bim, true, nullptr, wave * 16, cre, cim);  // U = E1[:, strip]^T B
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = (lane >> 4) * 4 + r, j = wave * 16 + (lane & 15);
@@ -1046,7 +1051,7 @@ __global__ __launch_bounds__(1024) void kron_split1_kernel(const float* __restri
   if (wave >= 4) return;
   cre = kf32x4{0.f, 0.f, 0.f, 0.f};
   cim = kf32x4{0.f, 0.f, 0.f, 0.f};
-  kron_strip<false, false>(Ure, 0, e2, nullptr, false, Uim, cre, cim);  // G strip = U E2
+  kron_strip<false, false>(Ure, 0, e2, nullptr, false, Uim, wave * 16, cre, cim);  // G strip = U E2
   const float* lam1 = lam + (long long)l1[c] * n;
   const float* lam2 = lam + (long long)l2[c] * n;
   float2* gc = g + (long long)c * 64 * 64;
@@ -1117,7 +1122,7 @@ __global__ __launch_bounds__(1024) void kron_split2_kernel(const float* __restri
   const int lane = tid & 63, wave = tid >> 6;
   kf32x4 cre = {0.f, 0.f, 0.f, 0.f}, cim = {0.f, 0.f, 0.f, 0.f};
   if (wave < 4) {
-    kron_strip<false, false>(e1, i0, Gre, Gim, true, nullptr, cre, cim);  // V = E1[strip, :] G
+    kron_strip<false, false>(e1, i0, Gre, Gim, true, nullptr, wave * 16, cre, cim);  // V = E1[strip, :] G
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = (lane >> 4) * 4 + r, j = wave * 16 + (lane & 15);
@@ -1129,7 +1134,7 @@ __global__ __launch_bounds__(1024) void kron_split2_kernel(const float* __restri
   if (wave >= 4) return;
   cre = kf32x4{0.f, 0.f, 0.f, 0.f};
   cim = kf32x4{0.f, 0.f, 0.f, 0.f};
-  kron_strip<false, true>(Vre, 0, e2, nullptr, false, Vim, cre, cim);  // X strip = V E2^T
+  kron_strip<false, true>(Vre, 0, e2, nullptr, false, Vim, wave * 16, cre, cim);  // X strip = V E2^T
   float2* x = out + (long long)c * n2;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -1137,6 +1142,126 @@ __global__ __launch_bounds__(1024) void kron_split2_kernel(const float* __restri
     if (i < n && j < n) x[i * n + j] = make_float2(cre[r], cim[r]);
   }
 }
+
+// The same FP32 solve in one launch, workgroup (c, t) again owning rows 16t .. 16t + 15 of
+// X: each workgroup forms the whole G = (E1^T B E2) / d itself (all 16 waves, one 16 x 16
+// tile each: the four workgroups of a channel repeat that half of the work, which costs
+// less than a second launch and the exchange of G through memory), then its strip
+// V = E1[strip, :] G and X = V E2^T (waves 0-3).  Every tile is the split kernels' tile with
+// the same operands and order, so the result is theirs bit for bit.
+__global__ __launch_bounds__(1024) void kron_fused_kernel(const float* __restrict__ E, const float* __restrict__ lam,
+                                                          const int* __restrict__ l1, const int* __restrict__ l2,
+                                                          int n, float2 a, const float2* in, float in_scale,
+                                                          const float2* __restrict__ add,
+                                                          const double* __restrict__ add_norm, float add_sigma,
+                                                          const double2* __restrict__ in_n2, float2* out,
+                                                          float2* in_wb) {
+  if (in_n2) {
+    const double q = in_n2->x;
+    in_scale = q > 0 ? (float)(1.0 / sqrt(q)) : 0.f;
+  }
+  extern __shared__ float kl[];
+  float* Bre = kl;                // B, then G (64 x KLD each plane)
+  float* Bim = Bre + 64 * KLD;
+  float* e1 = Bim + 64 * KLD;
+  float* e2 = e1 + 64 * KLD;
+  float* Ure = e2 + 64 * KLD;     // U = E1^T B
+  float* Uim = Ure + 64 * KLD;
+  float* Vre = Uim + 64 * KLD;    // V strip (16 x KLD)
+  float* Vim = Vre + 16 * KLD;
+  const int c = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
+  const int i0 = t * 16;
+  const int n2 = n * n;
+  const float* E1 = E + (long long)l1[c] * n2;
+  const float* E2 = E + (long long)l2[c] * n2;
+  const float2* xin = in + (long long)c * n2;
+  // every load of a thread is issued before the LDS stores: 4 each of B, E1, E2 (and the correction)
+  float2 bv[4], yv[4];
+  float ev1[4], ev2[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int idx = tid + q * 1024, i = idx >> 6, j = idx & 63;
+    const bool ok = i < n && j < n;
+    bv[q] = ok ? xin[i * n + j] : make_float2(0.f, 0.f);
+    ev1[q] = ok ? E1[i * n + j] : 0.f;
+    ev2[q] = ok ? E2[i * n + j] : 0.f;
+    yv[q] = (add && ok) ? add[(long long)c * n2 + i * n + j] : make_float2(0.f, 0.f);
+  }
+  const float as = add ? add_sigma * (float)*add_norm : 0.f;
+  float2* xb = in_wb ? in_wb + (long long)c * n2 : nullptr;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int idx = tid + q * 1024, i = idx >> 6, j = idx & 63;
+    const float2 sv = make_float2(in_scale * bv[q].x, in_scale * bv[q].y);
+    float2 v = sv;
+    if (add) {  // in + sigma |v| add (the learned correction)
+      v.x = fmaf(as, yv[q].x, v.x);
+      v.y = fmaf(as, yv[q].y, v.y);
+    }
+    Bre[i * KLD + j] = v.x;
+    Bim[i * KLD + j] = v.y;
+    e1[i * KLD + j] = ev1[q];
+    e2[i * KLD + j] = ev2[q];
+    // the scaled input in place of in, rows of this strip (in does not alias out here)
+    if (xb && i >= i0 && i < i0 + 16 && i < n && j < n) xb[i * n + j] = sv;
+  }
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6;
+  const int si = wave >> 2, ct = wave & 3;  // this wave's tile of a 64 x 64 product
+  kf32x4 cre = {0.f, 0.f, 0.f, 0.f}, cim = {0.f, 0.f, 0.f, 0.f};
+  kron_strip<true, false>(e1, si * 16, Bre, Bim, true, nullptr, ct * 16, cre, cim);  // U = E1^T B
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = si * 16 + (lane >> 4) * 4 + r, j = ct * 16 + (lane & 15);
+    Ure[i * KLD + j] = cre[r];
+    Uim[i * KLD + j] = cim[r];
+  }
+  __syncthreads();
+  cre = kf32x4{0.f, 0.f, 0.f, 0.f};
+  cim = kf32x4{0.f, 0.f, 0.f, 0.f};
+  kron_strip<false, false>(Ure, si * 16, e2, nullptr, false, Uim, ct * 16, cre, cim);  // U E2
+  const float* lam1 = lam + (long long)l1[c] * n;
+  const float* lam2 = lam + (long long)l2[c] * n;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {  // G = U E2 / d, into B's planes (B is no longer read)
+    const int i = si * 16 + (lane >> 4) * 4 + r, j = ct * 16 + (lane & 15);
+    float vr = 0.f, vi = 0.f;
+    if (i < n && j < n) {
+      const float sl = lam1[i] + lam2[j];
+      const float dr = 1.f + a.x * sl, di = a.y * sl;
+      const float inv = 1.f / (dr * dr + di * di);
+      vr = (cre[r] * dr + cim[r] * di) * inv;
+      vi = (cim[r] * dr - cre[r] * di) * inv;
+    }
+    Bre[i * KLD + j] = vr;
+    Bim[i * KLD + j] = vi;
+  }
+  __syncthreads();
+  if (wave < 4) {
+    cre = kf32x4{0.f, 0.f, 0.f, 0.f};
+    cim = kf32x4{0.f, 0.f, 0.f, 0.f};
+    kron_strip<false, false>(e1, i0, Bre, Bim, true, nullptr, wave * 16, cre, cim);  // V = E1[strip, :] G
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = (lane >> 4) * 4 + r, j = wave * 16 + (lane & 15);
+      Vre[i * KLD + j] = cre[r];
+      Vim[i * KLD + j] = cim[r];
+    }
+  }
+  __syncthreads();
+  if (wave >= 4) return;
+  cre = kf32x4{0.f, 0.f, 0.f, 0.f};
+  cim = kf32x4{0.f, 0.f, 0.f, 0.f};
+  kron_strip<false, true>(Vre, 0, e2, nullptr, false, Vim, wave * 16, cre, cim);  // X strip = V E2^T
+  float2* x = out + (long long)c * n2;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + (lane >> 4) * 4 + r, j = wave * 16 + (lane & 15);
+    if (i < n && j < n) x[i * n + j] = make_float2(cre[r], cim[r]);
+  }
+}
+
+static size_t kron_lds_fused() { return (6 * 64 + 2 * 16) * (size_t)KLD * sizeof(float); }
 
 static size_t kron_lds_split() { return (4 * 64 + 2 * 16) * (size_t)KLD * sizeof(float); }
 
@@ -1155,6 +1280,8 @@ void Kron::setup_attributes() {
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kron_lds32(n)));
   PP_HIP(hipFuncSetAttribute((const void*)kron_mfma_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)kron_lds_mfma()));
+  PP_HIP(hipFuncSetAttribute((const void*)kron_fused_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)kron_lds_fused()));
   PP_HIP(hipFuncSetAttribute((const void*)kron_split1_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)kron_lds_split()));
   PP_HIP(hipFuncSetAttribute((const void*)kron_split2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1177,11 +1304,11 @@ void Kron::multiply64(double2 a, double2* psi, hipStream_t s) const {
   PP_HIP(hipGetLastError());
 }
 
-void Kron::apply32(double2 a, float2* psi, hipStream_t s) const { apply32(a, psi, psi, s); }
+void Kron::apply32(double2 a, float2* psi, hipStream_t s) const { apply32(a, psi, psi, s, 1.f, nullptr, nullptr); }
 
-void Kron::apply32(double2 a, float2* in, float2* out, hipStream_t s, float in_scale, bool write_back,
+void Kron::apply32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
                    const double2* in_n2) const {
-  launch32(a, in, out, s, in_scale, write_back ? in : nullptr, nullptr, nullptr, 0.f, in_n2);
+  launch32(a, in, out, s, in_scale, wb, nullptr, nullptr, 0.f, in_n2);
 }
 
 void Kron::apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
@@ -1191,6 +1318,15 @@ void Kron::apply32_add(double2 a, const float2* in, const float2* add, const dou
 
 void Kron::launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
                     const float2* add, const double* add_norm, float add_sigma, const double2* in_n2) const {
+  // one launch only without aliasing: the workgroups of a channel all read the whole input
+  // block while each writes its strip of out (and of the write-back)
+  if (n <= 64 && split32 && fused32 && in != out && (!wb || (wb != out && wb != in && !add))) {
+    const dim3 grid(nchan, (n + 15) / 16);
+    hipLaunchKernelGGL(kron_fused_kernel, grid, dim3(1024), kron_lds_fused(), s, E32.ptr, lam32.ptr, l1.ptr, l2.ptr,
+                       n, make_float2((float)a.x, (float)a.y), in, in_scale, add, add_norm, add_sigma, in_n2, out, wb);
+    PP_HIP(hipGetLastError());
+    return;
+  }
   if (n <= 64 && split32 && (!wb || (wb != out && !add))) {
     if (g32.count < (size_t)nchan * 64 * 64) g32.resize((size_t)nchan * 64 * 64);
     const dim3 grid(nchan, (n + 15) / 16);
@@ -1262,6 +1398,7 @@ void MixedWork::ensure(long long n_, int m_) {
   Z.resize((size_t)m * n);
   w.resize(n);
   tmp.resize(n);
+  wnext.resize(n);
   if (!small) {
     // results the host reads: kernels write them into mapped, coherent pinned memory, so a
     // read is one stream synchronisation and no copy
@@ -1314,7 +1451,7 @@ void MixedWork::cgs2_32(int k, float2* w_, double2* out, double2* out_dev, hipSt
   const dim3 g(blocks), b(256);
   hipLaunchKernelGGL((cgs32_kernel<false, true>), g, b, 0, s, n, k, V.ptr, n, nullptr, 0, w_, p1);
   hipLaunchKernelGGL((cgs32_kernel<true, true>), g, b, 0, s, n, k, V.ptr, n, p1, width, w_, p2);
-  hipLaunchKernelGGL((cgs32_kernel<true, false>), g, b, 0, s, n, k, V.ptr, n, p2, width, w_, p3);
+  hipLaunchKernelGGL((cgs32_kernel<true, false>), g, b, 0, s, n, k, V.ptr, n, p2, width, w_, p3, wnext.ptr);
   hipLaunchKernelGGL(finish_partials_kernel, dim3(1), b, 0, s, blocks, p1, width, width, p2, width, width, p3, 1, 1,
                      out, out_dev);
   PP_HIP(hipGetLastError());
@@ -1332,7 +1469,7 @@ void MixedWork::cgs1_32(int k, float2* w_, double2* out, double2* out_dev, hipSt
   double2* p3 = p1 + (size_t)blocks * width;
   const dim3 g(blocks), b(256);
   hipLaunchKernelGGL((cgs32_kernel<false, true>), g, b, 0, s, n, k, V.ptr, n, nullptr, 0, w_, p1);
-  hipLaunchKernelGGL((cgs32_kernel<true, false>), g, b, 0, s, n, k, V.ptr, n, p1, width, w_, p3);
+  hipLaunchKernelGGL((cgs32_kernel<true, false>), g, b, 0, s, n, k, V.ptr, n, p1, width, w_, p3, wnext.ptr);
   hipLaunchKernelGGL(finish_cgs1_kernel, dim3(1), b, 0, s, blocks, p1, k, p3, out, out_dev);
   PP_HIP(hipGetLastError());
 }
@@ -1363,7 +1500,7 @@ void MixedWork::residual(const double2* b64_, double2* r, double2* out, double2*
 }
 
 int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
-                        const std::function<void(int, float2*, const double2*, float2*)>& apply_P, const float2* b,
+                        const std::function<void(int, const float2*, const double2*, float2*, float2*)>& apply_P, const float2* b,
                         double rtol, int max_it, double* rel_out, hipStream_t s, double beta_in,
                         const double2* base,
                         double2* xout, const std::function<bool(double&)>& first_read, int pre_iters,
@@ -1404,9 +1541,12 @@ int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply
     float2* zj = Z.ptr + (size_t)j * n;
     float2* vn = V.ptr + (size_t)(j + 1) * n;
     const double2* n2 = j == 0 ? nullptr : hdev.ptr + (size_t)(j - 1) * stride + 2 * (j + 1);
-    apply_P(j, vj, n2, zj);
+    // V_0 is already normalised; later directions come as the updated vector in wnext and
+    // are normalised into V_j by the preconditioner's load
+    apply_P(j, j == 0 ? vj : wnext.ptr, n2, zj, j == 0 ? nullptr : vj);
     apply_A(zj, vn);
-    // classical Gram-Schmidt: [first dots, |w|^2 | second dots, |w|^2 | final |w|^2]
+    // classical Gram-Schmidt: [first dots, |w|^2 | second dots, |w|^2 | final |w|^2]; the
+    // updated vector goes to wnext
     gram_schmidt(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s);
     if (j + 1 < pre) continue;
     PP_HIP(hipStreamSynchronize(s));  // the finishing kernels wrote straight into small
@@ -1462,7 +1602,7 @@ int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply
 // right-hand side, whose squared norm is beta2_dev->x.  The residual estimate after each
 // iteration goes to small[est .. est + k), readable after the caller's next synchronisation.
 void MixedWork::fgmres32_device(const std::function<void(const float2*, float2*)>& apply_A,
-                                const std::function<void(int, float2*, const double2*, float2*)>& apply_P, int k,
+                                const std::function<void(int, const float2*, const double2*, float2*, float2*)>& apply_P, int k,
                                 const double2* beta2_dev, const double2* base, double2* xout, int est,
                                 hipStream_t s) {
   PP_REQUIRE(k >= 1 && k <= kDevLsMax && k <= m, "device inner solve size out of range");
@@ -1473,7 +1613,7 @@ void MixedWork::fgmres32_device(const std::function<void(const float2*, float2*)
     float2* zj = Z.ptr + (size_t)j * n;
     float2* vn = V.ptr + (size_t)(j + 1) * n;
     const double2* n2 = j == 0 ? nullptr : hdev.ptr + (size_t)(j - 1) * stride + 2 * (j + 1);
-    apply_P(j, vj, n2, zj);
+    apply_P(j, j == 0 ? vj : wnext.ptr, n2, zj, j == 0 ? nullptr : vj);
     apply_A(zj, vn);
     gram_schmidt(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s);
   }
@@ -1562,22 +1702,24 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     ++stats.inner_ops;
   };
   int correction = 0;
-  // z = P (in / |in|) with |in|^2 on the device (n2 = NULL: in is already normalised); in
-  // is replaced by in / |in| (the normalised basis vector)
-  auto apply_P32 = [&](int j, float2* in, const double2* n2, float2* out) {
+  // z = P (in / sqrt(n2)) (n2 = NULL: in is already normalised); wb (with n2): receives
+  // in / sqrt(n2), the normalised basis vector
+  auto apply_P32 = [&](int j, const float2* in, const double2* n2, float2* out, float2* wb) {
     // learned_mode 0: the first direction of the first correction, whose right-hand side
     // r = -2 i a H psi is the kind of vector the network was trained on; 1: the first
     // direction of every correction; 2: every direction
     const bool use = learned && (learned_mode == 2 || (j == 0 && (learned_mode == 1 || correction == 0)));
     if (use) {
+      const float2* v = in;
       if (n2) {  // V_j is normalised here (the Kronecker path does it while loading)
-        hipLaunchKernelGGL(unit32_dev_kernel, dim3(grid_n(n)), dim3(256), 0, cs, n, n2, in);
+        hipLaunchKernelGGL(unit32_to_kernel, dim3(grid_n(n)), dim3(256), 0, cs, n, n2, in, wb);
         PP_HIP(hipGetLastError());
+        v = wb;
       }
-      learned_apply(*learned, K, ka, in, out, cs);
+      learned_apply(*learned, K, ka, v, out, cs);
       ++stats.learned_applies;
     } else {
-      K.apply32(ka, in, out, cs, 1.f, n2 != nullptr, n2);
+      K.apply32(ka, in, out, cs, 1.f, n2 ? wb : nullptr, n2);
     }
     ++stats.precond_applies;
   };
@@ -1848,6 +1990,23 @@ extern "C" int pp_kron_solve_f32(const pp_kron* k, pp_cplx a, pp_cplxf* psi, pp_
   return guarded([&] {
     PP_REQUIRE(k && psi, "null argument");
     k->k.apply32(make_double2(a.re, a.im), reinterpret_cast<float2*>(psi), (hipStream_t)stream);
+  });
+}
+
+extern "C" int pp_kron_solve_f32_into(const pp_kron* k, pp_cplx a, const pp_cplxf* in, pp_cplxf* out,
+                                      pp_stream stream) {
+  return guarded([&] {
+    PP_REQUIRE(k && in && out, "null argument");
+    k->k.apply32(make_double2(a.re, a.im), reinterpret_cast<const float2*>(in), reinterpret_cast<float2*>(out),
+                 (hipStream_t)stream, 1.f, nullptr, nullptr);
+  });
+}
+
+extern "C" int pp_kron_set_form(pp_kron* k, int form) {
+  return guarded([&] {
+    PP_REQUIRE(k && form >= 0 && form <= 2, "Kronecker form must be 0, 1 or 2");
+    k->k.split32 = form >= 1;
+    k->k.fused32 = form == 2;
   });
 }
 

@@ -23,15 +23,16 @@ struct Kron {
   DeviceBuffer<float> E32, lam32;
   DeviceBuffer<int> l1, l2;          // per channel
   mutable DeviceBuffer<float2> g32;  // [c][64][64] middle product of the split FP32 solve
-  bool split32 = true;               // FP32 solve (n <= 64) as two launches, four workgroups per channel
+  bool split32 = true;               // FP32 solve (n <= 64) with four workgroups per channel
+  bool fused32 = true;               // ... in one launch (each workgroup forms the whole middle product)
   void setup_attributes();
   // psi <- K^-1 psi, a = complex factor of the one-electron part (i dt / 2)
   void apply64(double2 a, double2* psi, hipStream_t s) const;
   void apply32(double2 a, float2* psi, hipStream_t s) const;
-  // out <- K^-1 (in_scale in) (may alias in); write_back: in <- in_scale in as well
-  // in_n2 (device, may be NULL): the scale is 1 / sqrt(in_n2->x) instead of in_scale
-  void apply32(double2 a, float2* in, float2* out, hipStream_t s, float in_scale = 1.f, bool write_back = false,
-               const double2* in_n2 = nullptr) const;
+  // out <- K^-1 (in_scale in) (may alias in); wb (may be NULL, not out): wb <- in_scale in
+  // as well; in_n2 (device, may be NULL): in_scale = 1 / sqrt(in_n2->x)
+  void apply32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
+               const double2* in_n2) const;
   // out <- K^-1 (in + add_sigma * (*add_norm) * add), *add_norm read on the device
   void apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
                    float2* out, hipStream_t s) const;
@@ -61,6 +62,7 @@ struct MixedWork {
   int m = 0;
   DeviceBuffer<float2> V, Z, w, tmp;
   DeviceBuffer<double2> b64, r64, x64, t64, h64;
+  DeviceBuffer<float2> wnext;  // the Gram-Schmidt-updated direction of the last iteration
   double2* small = nullptr;      // kSmall results for the host (pinned, mapped, coherent)
   double2* small_dev = nullptr;  // its device address
   Reducer red;
@@ -94,16 +96,19 @@ struct MixedWork {
   // xout = base + d (FP64; base may alias xout).  rhs = NULL: V_0 already holds rhs / |rhs|
   // and beta = |rhs|, or, with first_read, beta is set by first_read at the first host
   // synchronisation, which may also return false: the solve is then abandoned without
-  // touching xout and -1 is returned.  apply_P(j, v, n2, z): v <- v / sqrt(n2->x) when n2 (a
-  // device squared norm) is given, normalising the basis vector, then z = P_j v.  The first
+  // touching xout and -1 is returned.  apply_P(j, v, n2, z, wb): with n2 (a device squared
+  // norm), wb <- v / sqrt(n2->x), the normalised basis vector V_j, and z = P_j wb; without,
+  // z = P_j v (v = V_0, already normalised).  For j >= 1, v is wnext, where the Gram-Schmidt
+  // update of the previous iteration left the new direction (out of place, so that the
+  // preconditioner never reads what it writes).  The first
   // pre_iters iterations run before the host reads anything; *needed (may be NULL) receives
   // the iteration count at which the tolerance was first met.
   // k iterations with no host read; least squares and combination on the device
   void fgmres32_device(const std::function<void(const float2*, float2*)>& apply_A,
-                       const std::function<void(int, float2*, const double2*, float2*)>& apply_P, int k,
+                       const std::function<void(int, const float2*, const double2*, float2*, float2*)>& apply_P, int k,
                        const double2* beta2_dev, const double2* base, double2* xout, int est, hipStream_t s);
   int fgmres32(const std::function<void(const float2*, float2*)>& apply_A,
-               const std::function<void(int, float2*, const double2*, float2*)>& apply_P, const float2* rhs,
+               const std::function<void(int, const float2*, const double2*, float2*, float2*)>& apply_P, const float2* rhs,
                double rtol, int max_it, double* rel_out, hipStream_t s, double beta, const double2* base,
                double2* xout, const std::function<bool(double&)>& first_read = nullptr, int pre_iters = 1,
                int* needed = nullptr);

@@ -107,6 +107,20 @@ class KroneckerPreconditioner:
         else:
             N.check(N.load().pp_kron_solve(self.h, N.cplx(self.HamiltonianScaling), D.ptr(t), D.stream_handle()))
 
+    def SolveInto(self, src, dst):
+        """dst <- K^-1 src (complex64, distinct tensors)."""
+        self.Setup()
+        N.check(N.load().pp_kron_solve_f32_into(self.h, N.cplx(self.HamiltonianScaling), D.ptr(src), D.ptr(dst),
+                                                D.stream_handle()))
+
+    FORMS = ("one-workgroup", "split", "fused")
+
+    def SetForm(self, form: int):
+        """The FP32 solve's kernel form (pp_kron_set_form): 0 one workgroup per channel,
+        1 four per channel in two launches, 2 four per channel in one launch (default)."""
+        self.Setup()
+        N.check(N.load().pp_kron_set_form(self.h, int(form)))
+
     def __del__(self):
         if getattr(self, "h", None) is not None and N._lib is not None:
             N._lib.pp_kron_destroy(self.h)
