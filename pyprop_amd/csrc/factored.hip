@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "pp_common.h"
@@ -411,14 +412,212 @@ __global__ __launch_bounds__(256) void fham_radial_pos_kernel(int nc, int ncp, i
   }
 }
 
+// LDS layout of fham_radial_win_kernel: values per window point (FP32: an odd multiple of 16,
+// so that the two points a half-wave reads lie in opposite halves of the 32 banks) and bytes.
+template <class T, int MG>
+__host__ __device__ constexpr int win_point_stride() {
+  return sizeof(T) == 4 && MG % 2 == 0 ? MG * 16 + 16 : MG * 16;
+}
+// the largest divisor of rows that is at most cap (rows of the window copied per batch)
+__host__ __device__ constexpr int win_rows_per_batch(int rows, int cap) {
+  return cap <= 1 ? 1 : rows % cap == 0 ? cap : win_rows_per_batch(rows, cap - 1);
+}
+// the window, then 16 bytes per thread that the copy's slots outside the window write
+template <class T, int BW, int MG, int TW>
+__host__ __device__ constexpr size_t win_lds_bytes() {
+  return (size_t)(2 * BW + 2) * (TW + 2 * BW) * win_point_stride<T, MG>() * sizeof(T) + 256 * 16;
+}
+
+// Stage 1, position form with the window in LDS.  A workgroup takes one r1 point i1, TW
+// consecutive r2 points i0 .. i0 + TW - 1 and channel groups gc MG .. gc MG + MG - 1.  It
+// copies the X values of the union of their windows from Xt into LDS once; its four waves
+// then form the products of TW / 4 positions each, the GEMM of fham_radial_pos_kernel per
+// position, with the X operands read from LDS and the next position's R operands loaded
+// while the current position's MFMAs run.  LDS point (jr, jc), jr <= w, jc < TW + 2 BW,
+// holds the MG x 16 operand values of grid point (i1 - BW + jr, i0 - BW + jc): zero outside
+// the grid, and in the extra row jr = w, which the last K step's lanes past w^2 read (their
+// R entries are zero).  K step ks, lane (lr, lk): k = 4 ks + lk = w jj + d reads point
+// (jj, ip + d), ip the position's offset in the tile.  With (jj0, d0) = divmod(4 ks, w) that
+// point is jj0 WC + d0 + lk, plus WC - w where d0 + lk >= w (the row wraps): a lane base
+// chosen by ks at compile time plus an immediate, so the reads take no address arithmetic.
+template <class T, int BW, int MG, int TW>
+__global__ __launch_bounds__(256) void fham_radial_win_kernel(int nc, int n, int nterms, int ngroups, int ntiles,
+                                                              long long nblocks, int ncp, const T* __restrict__ Xt,
+                                                              const T* __restrict__ RP, T* __restrict__ Z,
+                                                              const int* __restrict__ differs) {
+  if (differs && *differs == 0) return;  // input equals the one whose products Z holds
+  typedef typename MfmaT<T>::V4 V4;
+  typedef typename MfmaT<T>::C2 C2;
+  constexpr int w = 2 * BW + 1, KK = w * w, KP = (KK + 3) / 4;
+  constexpr int WC = TW + 2 * BW;  // window columns
+  constexpr int PS = win_point_stride<T, MG>();
+  constexpr int PPW = TW / 4;  // positions per wave
+  constexpr int KB = 4;        // K steps per batch of LDS reads
+  static_assert(TW % 4 == 0, "four waves share the tile's positions");
+  extern __shared__ __align__(16) char win_raw[];
+  T* win = reinterpret_cast<T*>(win_raw);
+  const int per = (int)((nblocks + 7) / 8);  // nblocks < 2^31 (n <= 64)
+  const int hw = blockIdx.x;
+  const int L = (hw % 8) * per + hw / 8;  // XCD-aware: a tile's chunks share an L2
+  if (L >= nblocks) return;               // the whole workgroup, before its barrier
+  const int nchunks = (ngroups + MG - 1) / MG;
+  const int gc = L % nchunks;
+  int rest = L / nchunks;
+  const int tile = rest % ntiles;
+  rest /= ntiles;
+  const int i1 = rest % n;
+  const int tn = rest / n;
+  const int i0 = tile * TW;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long P = (long long)n * n;
+  const T* Rt = RP + (long long)tn * P * KP * 64 + lane;
+  const int p0 = i0 + wave * PPW;  // this wave's first position
+  // R operands: position s of the wave in rb[s % 2] (past n: the last position's, unused)
+  T rb[2][KP];
+  {  // issued first: their latency overlaps the window copy
+    const T* r = Rt + ((long long)i1 * n + min(p0, n - 1)) * KP * 64;
+#pragma unroll
+    for (int ks = 0; ks < KP; ++ks) rb[0][ks] = r[ks * 64];
+  }
+  // window copy in 16-byte pieces: thread slot q of column jcb + CPI cs, row by row, RB rows
+  // per batch whose loads are all issued before their LDS stores.  Buffer loads: a piece
+  // outside the grid (or past the channel groups) gets an offset beyond the buffer's range,
+  // for which the load returns zeros, so no load sits behind a branch.
+  {
+    typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type Piece;
+    constexpr int VPP = 16 / sizeof(T);  // values per piece
+    constexpr int PPP = MG * 16 / VPP;   // pieces per point
+    constexpr int SLOTS = PPP <= 4 ? 4 : PPP <= 8 ? 8 : PPP <= 16 ? 16 : 32;
+    constexpr int CPI = 256 / SLOTS;     // columns per pass
+    constexpr int NCS = (WC + CPI - 1) / CPI;
+    constexpr int RB = win_rows_per_batch(2 * BW + 2, NCS >= 28 ? 1 : 28 / NCS);
+    static_assert(PPP <= 32, "at most 32 pieces per point");
+    constexpr int kOut = 0x7ffffff0;     // beyond any buffer here (Xt < 2 GiB)
+    const int xt_bytes = n * n * ncp * 2 * (int)sizeof(T);
+    const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void*)Xt, (short)0, xt_bytes, 0x00020000);
+    const int q = (int)threadIdx.x % SLOTS, jcb = (int)threadIdx.x / SLOTS;
+    const int spare = (2 * BW + 2) * WC * PS + (int)threadIdx.x * VPP;  // this thread's 16 spare bytes
+    const bool qok = q < PPP && gc * MG + q * VPP / 16 < ngroups;
+    const int qoff = (gc * MG * 16 + q * VPP) * (int)sizeof(T);
+    int coff[NCS];
+#pragma unroll
+    for (int cs = 0; cs < NCS; ++cs) {
+      const int jc = jcb + cs * CPI, j2 = i0 - BW + jc;
+      coff[cs] = qok && jc < WC && j2 >= 0 && j2 < n ? j2 * ncp * 2 * (int)sizeof(T) + qoff : -1;
+    }
+#pragma unroll
+    for (int jr0 = 0; jr0 <= w; jr0 += RB) {
+      Piece v[RB][NCS];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int jr = jr0 + r, j1 = i1 - BW + jr;
+        const bool rok = jr < w && j1 >= 0 && j1 < n;
+        const int roff = j1 * n * ncp * 2 * (int)sizeof(T);
+#pragma unroll
+        for (int cs = 0; cs < NCS; ++cs) {
+          const int off = rok && coff[cs] >= 0 ? roff + coff[cs] : kOut;
+          const auto x = __builtin_amdgcn_raw_buffer_load_b128(xr, off, 0, 0);
+          v[r][cs] = *reinterpret_cast<const Piece*>(&x);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int jr = jr0 + r;
+#pragma unroll
+        for (int cs = 0; cs < NCS; ++cs) {  // unconditional stores: no load is sunk into a branch
+          const int jc = jcb + cs * CPI;
+          const int dst = jc < WC && q < PPP ? (jr * WC + jc) * PS + q * VPP : spare;
+          *reinterpret_cast<Piece*>(win + dst) = v[r][cs];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int cl = MfmaT<T>::kRowMajorOut ? (lr >> 1) : (lr & 7);
+  const int part = MfmaT<T>::kRowMajorOut ? (lr & 1) : (lr >> 3);
+  const int b0 = lk * PS + cl * 2 + part;  // point lk of row 0, this lane's operand row
+  const int bwr = b0 + (WC - w) * PS;      // the same point after a row wrap
+  const int b1 = lk >= 1 ? bwr : b0, b2 = lk >= 2 ? bwr : b0, b3 = lk >= 3 ? bwr : b0;
+#pragma unroll
+  for (int s = 0; s < PPW; ++s) {
+    // the next position's R operands, KB per batch of this position's steps (so the loads
+    // in flight stay few and in the order of use)
+    T (&bc)[KP] = rb[s % 2];
+    T (&bn)[KP] = rb[(s + 1) % 2];
+    const T* rn = Rt + ((long long)i1 * n + min(p0 + s + 1, n - 1)) * KP * 64;
+    const int i2 = p0 + s;
+    if (i2 < n) {  // wave-uniform; the positions after an invalid one are invalid
+      const T* wp = win + (wave * PPW + s) * PS;
+      auto read_x = [&](int ks, T (&a)[MG]) {
+        const int d0 = (4 * ks) % w, th = w - d0;  // lanes lk >= th wrap
+        const int base = th >= 4 ? b0 : th == 3 ? b3 : th == 2 ? b2 : b1;
+        const int c0 = ((4 * ks) / w * WC + d0) * PS;
+#pragma unroll
+        for (int g = 0; g < MG; ++g) a[g] = wp[base + c0 + g * 16];
+      };
+      V4 acc[MG];
+#pragma unroll
+      for (int g = 0; g < MG; ++g) acc[g] = V4{0, 0, 0, 0};
+      T a0[KB][MG];
+#pragma unroll
+      for (int u = 0; u < KB; ++u) read_x(u, a0[u]);
+#pragma unroll
+      for (int kb = 0; kb < KP; kb += KB) {
+        T a1[KB][MG];
+#pragma unroll
+        for (int u = 0; u < KB; ++u)
+          if (kb + KB + u < KP) read_x(kb + KB + u, a1[u]);
+        if (s + 1 < PPW) {
+#pragma unroll
+          for (int u = 0; u < KB; ++u)
+            if (kb + u < KP) bn[kb + u] = rn[(kb + u) * 64];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < KB; ++u)
+          if (kb + u < KP) {
+#pragma unroll
+            for (int g = 0; g < MG; ++g) acc[g] = MfmaT<T>::mma(a0[u][g], bc[kb + u], acc[g]);
+          }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < KB; ++u)
+#pragma unroll
+          for (int g = 0; g < MG; ++g) a0[u][g] = a1[u][g];
+      }
+      const int t = tn * 16 + lr;  // result column: the term
+      if (t < nterms) {
+#pragma unroll
+        for (int g = 0; g < MG; ++g) {
+          const int grp = gc * MG + g;
+          if (grp >= ngroups) continue;
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const int c = grp * 8 + (MfmaT<T>::kRowMajorOut ? lk * 2 + q : lk + 4 * q);
+            if (c >= nc) continue;
+            C2 v;
+            v.x = MfmaT<T>::kRowMajorOut ? acc[g][2 * q] : acc[g][q];
+            v.y = MfmaT<T>::kRowMajorOut ? acc[g][2 * q + 1] : acc[g][q + 2];
+            reinterpret_cast<C2*>(Z)[(((long long)t * nc + c) * n + i1) * n + i2] = v;
+          }
+        }
+      }
+    }
+  }
+}
+
 // (terms, channel groups of 8, prefetch) per wave of each band-tile radial variant; variant
 // 0 is the one-term, one-group kernel; variants with kRadPosMG > 0 are the position form
 // with that many channel groups per wave (only with separable terms split off, which fixes
 // the set of terms its tiles hold)
-constexpr int kRadTG[kFhamRadialVariants] = {1, 2, 3, 3, 2, 3, 3, 1, 1, 1};
-constexpr int kRadGG[kFhamRadialVariants] = {1, 2, 2, 3, 2, 2, 3, 1, 1, 1};
-constexpr bool kRadPipe[kFhamRadialVariants] = {false, false, false, false, true, true, true, false, false, false};
-constexpr int kRadPosMG[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 6, 3, 2};
+constexpr int kRadTG[kFhamRadialVariants] = {1, 2, 3, 3, 2, 3, 3, 1, 1, 1, 1, 1, 1, 1};
+constexpr int kRadGG[kFhamRadialVariants] = {1, 2, 2, 3, 2, 2, 3, 1, 1, 1, 1, 1, 1, 1};
+constexpr bool kRadPipe[kFhamRadialVariants] = {false, false, false, false, true, true, true,
+                                                false, false, false, false, false, false, false};
+constexpr int kRadPosMG[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 6, 3, 2, 3, 2, 2, 1};
+// position variants with the window in LDS (fham_radial_win_kernel): positions per workgroup
+constexpr int kRadWinTW[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 16, 16, 8, 16};
 // angular variants: waves splitting K, and whether one workgroup covers all row blocks of
 // its position tile (reading each Z element once) or one row block
 constexpr int kAngWaveChoices[kFhamAngularVariants] = {16, 8, 4, 8, 16, 4, 4};
@@ -430,6 +629,27 @@ constexpr int kAngMaxRowBlocks = 4;  // all-row variants: at most 64 channels
 template <class T, int B>
 static void launch_radial(int variant, int nc, int ncp, int n, int nterms, int ngroups, int tiles, const T* Xt,
                           const T* BT, const T* RP, T* Z, const int* differs, hipStream_t s) {
+  if (kRadWinTW[variant] > 0) {
+    if (!RP) throw Error(PP_ERR_INVALID, "position-form radial variant without position tiles");
+    const int MG = kRadPosMG[variant], TW = kRadWinTW[variant];
+    const int ntiles = (n + TW - 1) / TW;
+    const long long nblocks = (long long)((nterms + 15) / 16) * n * ntiles * ((ngroups + MG - 1) / MG);
+    const dim3 grid((unsigned)((nblocks + 7) / 8 * 8));
+#define PP_RADW(G, W)                                                                                          \
+  if (MG == G && TW == W) {                                                                                    \
+    constexpr size_t lds = win_lds_bytes<T, B, G, W>();                                                        \
+    if (lds > 160 * 1024) throw Error(PP_ERR_INVALID, "radial window does not fit in LDS");                    \
+    PP_HIP(hipFuncSetAttribute((const void*)fham_radial_win_kernel<T, B, G, W>,                                \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                         \
+    hipLaunchKernelGGL((fham_radial_win_kernel<T, B, G, W>), grid, dim3(256), lds, s, nc, n, nterms, ngroups,  \
+                       ntiles, nblocks, ncp, Xt, RP, Z, differs);                                              \
+    PP_HIP(hipGetLastError());                                                                                 \
+    return;                                                                                                    \
+  }
+    PP_RADW(3, 16) PP_RADW(2, 16) PP_RADW(2, 8) PP_RADW(1, 16)
+#undef PP_RADW
+    throw Error(PP_ERR_INVALID, "radial window shape not instantiated");
+  }
   if (kRadPosMG[variant] > 0) {
     const int MG = kRadPosMG[variant];
     if (!RP) throw Error(PP_ERR_INVALID, "position-form radial variant without position tiles");
@@ -1396,12 +1616,23 @@ extern "C" int pp_fham_multiply_f32(const pp_fham* f, const double* scalings, co
   });
 }
 
+// Whether radial variant v can run for this operator in the given precision: the position
+// forms need the separable split; the LDS-window forms a window that fits in 160 KiB.
+static bool radial_window_fits(const FHam& F, int v, int precision) {
+  if (kRadWinTW[v] == 0) return true;
+  const int MG = kRadPosMG[v], esz = precision == 64 ? 8 : 4;
+  const int ps = esz == 4 && MG % 2 == 0 ? MG * 16 + 16 : MG * 16;
+  const size_t lds = (size_t)(2 * F.bw + 2) * (kRadWinTW[v] + 2 * F.bw) * ps * esz + 256 * 16;
+  return lds <= 160 * 1024;
+}
+
 extern "C" int pp_fham_set_radial_variant(pp_fham* f, int precision, int variant) {
   return guarded([&] {
     PP_REQUIRE(f && (precision == 64 || precision == 32), "precision must be 64 or 32");
     PP_REQUIRE(variant >= 0 && variant < kFhamRadialVariants, "radial variant out of range");
     PP_REQUIRE(kRadPosMG[variant] == 0 || f->f.sep_enabled,
                "position-form radial variants need the separable split (n <= 64)");
+    PP_REQUIRE(radial_window_fits(f->f, variant, precision), "radial window does not fit in LDS");
     (precision == 64 ? f->f.rad_variant64 : f->f.rad_variant32) = variant;
   });
 }
@@ -1475,7 +1706,7 @@ extern "C" int pp_fham_tune(pp_fham* f, int precision, int trials, int* best, do
     double best_ms = 1e300;
     int best_v = var;
     for (int v = 0; v < kFhamRadialVariants; ++v) {
-      if (kRadPosMG[v] > 0 && !F.sep_enabled) {
+      if ((kRadPosMG[v] > 0 && !F.sep_enabled) || !radial_window_fits(F, v, precision)) {
         if (timings_ms) timings_ms[v] = -1.0;  // not available for this operator
         continue;
       }

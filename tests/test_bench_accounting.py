@@ -1,8 +1,8 @@
 """bench.py's issued-MFMA accounting of the factored multiply against the instruction
 counts the hardware reports for the same kernels at helium (SQ_INSTS_MFMA per dispatch,
 rocprofv3 --pmc on MI355X): band-tile radial kernels 1,487,808, position-form radial
-kernels 928,800, angular kernels 259,200 in every GEMM form and none in the sparse forms
-(vector FMAs).  The one-term band-tile kernel ("1x1") issues padded band batches beyond the
+kernels 928,800 (with and without the window in LDS), angular kernels 259,200 in every GEMM
+form and none in the sparse forms (vector FMAs).  The one-term band-tile kernel ("1x1") issues padded band batches beyond the
 count here and is not covered."""
 import pytest
 
@@ -31,7 +31,9 @@ class _Helium:
 
 @pytest.mark.parametrize("angular", FactoredHamiltonian.ANGULAR_VARIANTS)
 @pytest.mark.parametrize("radial, counted", [("2x2", 1487808), ("3x2", 1487808), ("3x3p", 1487808),
-                                             ("pos6", 928800), ("pos3", 928800), ("pos2", 928800)])
+                                             ("pos6", 928800), ("pos3", 928800), ("pos2", 928800),
+                                             ("pos3w16", 928800), ("pos2w16", 928800), ("pos2w8", 928800),
+                                             ("pos1w16", 928800)])
 def test_issued_mfma_matches_hardware_counts(radial, counted, angular):
     issued, useful = bench.factored_flops(_Helium(radial, angular), 42, 60, 64)
     assert issued == (counted + (0 if angular.startswith("sparse") else 259200)) * 2048
