@@ -422,10 +422,18 @@ __host__ __device__ constexpr int win_point_stride() {
 __host__ __device__ constexpr int win_rows_per_batch(int rows, int cap) {
   return cap <= 1 ? 1 : rows % cap == 0 ? cap : win_rows_per_batch(rows, cap - 1);
 }
-// the window, then 16 bytes per thread that the copy's slots outside the window write
+// LDS of fham_radial_win_kernel: the window, then 16 bytes per thread that the copy's slots
+// outside the window write; afterwards the same space holds the products, MG x 8 x 16 rows of
+// TW + 1 complex values
+__host__ __device__ constexpr size_t win_lds_bytes(int bw, int mg, int tw, int esz) {
+  const size_t win = (size_t)(2 * bw + 2) * (tw + 2 * bw) * (esz == 4 && mg % 2 == 0 ? mg * 16 + 16 : mg * 16) * esz +
+                     256 * 16;
+  const size_t out = (size_t)mg * 8 * 16 * (tw + 1) * 2 * esz;
+  return win > out ? win : out;
+}
 template <class T, int BW, int MG, int TW>
 __host__ __device__ constexpr size_t win_lds_bytes() {
-  return (size_t)(2 * BW + 2) * (TW + 2 * BW) * win_point_stride<T, MG>() * sizeof(T) + 256 * 16;
+  return win_lds_bytes(BW, MG, TW, (int)sizeof(T));
 }
 
 // Stage 1, position form with the window in LDS.  A workgroup takes one r1 point i1, TW
@@ -539,46 +547,53 @@ __global__ __launch_bounds__(256) void fham_radial_win_kernel(int nc, int n, int
   const int b0 = lk * PS + cl * 2 + part;  // point lk of row 0, this lane's operand row
   const int bwr = b0 + (WC - w) * PS;      // the same point after a row wrap
   const int b1 = lk >= 1 ? bwr : b0, b2 = lk >= 2 ? bwr : b0, b3 = lk >= 3 ? bwr : b0;
+  // The wave's positions run as one pipeline: each batch of KB K steps issues the LDS reads
+  // of the next batch (the next position's first batch after a position's last) and the next
+  // position's R operands with the same K indices, then its MFMAs.  In a wave with a valid
+  // position the positions past n are formed too (with the last position's R) and discarded;
+  // a wave with none skips the products.
+  V4 acc[PPW][MG];
+  const bool active = p0 < n;  // wave-uniform
+  if (active) {
+    const T* wv = win + wave * PPW * PS;  // the wave's first position
+    auto read_x = [&](int sp, int ks, T (&a)[MG]) {
+      const int d0 = (4 * ks) % w, th = w - d0;  // lanes lk >= th wrap
+      const int base = th >= 4 ? b0 : th == 3 ? b3 : th == 2 ? b2 : b1;
+      const int c0 = ((4 * ks) / w * WC + d0 + sp) * PS;
 #pragma unroll
-  for (int s = 0; s < PPW; ++s) {
-    // the next position's R operands, KB per batch of this position's steps (so the loads
-    // in flight stay few and in the order of use)
-    T (&bc)[KP] = rb[s % 2];
-    T (&bn)[KP] = rb[(s + 1) % 2];
-    const T* rn = Rt + ((long long)i1 * n + min(p0 + s + 1, n - 1)) * KP * 64;
-    const int i2 = p0 + s;
-    if (i2 < n) {  // wave-uniform; the positions after an invalid one are invalid
-      const T* wp = win + (wave * PPW + s) * PS;
-      auto read_x = [&](int ks, T (&a)[MG]) {
-        const int d0 = (4 * ks) % w, th = w - d0;  // lanes lk >= th wrap
-        const int base = th >= 4 ? b0 : th == 3 ? b3 : th == 2 ? b2 : b1;
-        const int c0 = ((4 * ks) / w * WC + d0) * PS;
+      for (int g = 0; g < MG; ++g) a[g] = wv[base + c0 + g * 16];
+    };
+    const T* rpos[PPW];  // R operands of each position (past n: the last position's)
 #pragma unroll
-        for (int g = 0; g < MG; ++g) a[g] = wp[base + c0 + g * 16];
-      };
-      V4 acc[MG];
+    for (int sp = 0; sp < PPW; ++sp) rpos[sp] = Rt + ((long long)i1 * n + min(p0 + sp, n - 1)) * KP * 64;
 #pragma unroll
-      for (int g = 0; g < MG; ++g) acc[g] = V4{0, 0, 0, 0};
-      T a0[KB][MG];
+    for (int sp = 0; sp < PPW; ++sp)
 #pragma unroll
-      for (int u = 0; u < KB; ++u) read_x(u, a0[u]);
+      for (int g = 0; g < MG; ++g) acc[sp][g] = V4{0, 0, 0, 0};
+    T a0[KB][MG];
+#pragma unroll
+    for (int u = 0; u < KB; ++u) read_x(0, u, a0[u]);
+#pragma unroll
+    for (int sp = 0; sp < PPW; ++sp) {
 #pragma unroll
       for (int kb = 0; kb < KP; kb += KB) {
+        const bool last = kb + KB >= KP;
+        const int nsp = last ? sp + 1 : sp, nkb = last ? 0 : kb + KB;  // the next batch
         T a1[KB][MG];
 #pragma unroll
         for (int u = 0; u < KB; ++u)
-          if (kb + KB + u < KP) read_x(kb + KB + u, a1[u]);
-        if (s + 1 < PPW) {
+          if (nsp < PPW && nkb + u < KP) read_x(nsp, nkb + u, a1[u]);
+        if (sp + 1 < PPW) {
 #pragma unroll
           for (int u = 0; u < KB; ++u)
-            if (kb + u < KP) bn[kb + u] = rn[(kb + u) * 64];
+            if (kb + u < KP) rb[(sp + 1) % 2][kb + u] = rpos[sp + 1][(kb + u) * 64];
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < KB; ++u)
           if (kb + u < KP) {
 #pragma unroll
-            for (int g = 0; g < MG; ++g) acc[g] = MfmaT<T>::mma(a0[u][g], bc[kb + u], acc[g]);
+            for (int g = 0; g < MG; ++g) acc[sp][g] = MfmaT<T>::mma(a0[u][g], rb[sp % 2][kb + u], acc[sp][g]);
           }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -586,23 +601,44 @@ __global__ __launch_bounds__(256) void fham_radial_win_kernel(int nc, int n, int
 #pragma unroll
           for (int g = 0; g < MG; ++g) a0[u][g] = a1[u][g];
       }
-      const int t = tn * 16 + lr;  // result column: the term
-      if (t < nterms) {
+    }
+  }
+  // The products leave through LDS (the window's space once every wave is done with it), as
+  // rows (channel, term) of TW consecutive r2 points, so that the global stores are whole
+  // rows.  Layout [g][c8][t][OLD]: OLD odd, so that the 16 terms a wave writes at once fall
+  // in distinct banks.
+  constexpr int OLD = TW + 1;
+  __syncthreads();
+  C2* outs = reinterpret_cast<C2*>(win_raw);
+  if (active) {
 #pragma unroll
-        for (int g = 0; g < MG; ++g) {
-          const int grp = gc * MG + g;
-          if (grp >= ngroups) continue;
+    for (int sp = 0; sp < PPW; ++sp)
 #pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const int c = grp * 8 + (MfmaT<T>::kRowMajorOut ? lk * 2 + q : lk + 4 * q);
-            if (c >= nc) continue;
-            C2 v;
-            v.x = MfmaT<T>::kRowMajorOut ? acc[g][2 * q] : acc[g][q];
-            v.y = MfmaT<T>::kRowMajorOut ? acc[g][2 * q + 1] : acc[g][q + 2];
-            reinterpret_cast<C2*>(Z)[(((long long)t * nc + c) * n + i1) * n + i2] = v;
-          }
+      for (int g = 0; g < MG; ++g)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int c8 = MfmaT<T>::kRowMajorOut ? lk * 2 + q : lk + 4 * q;
+          C2 v;
+          v.x = MfmaT<T>::kRowMajorOut ? acc[sp][g][2 * q] : acc[sp][g][q];
+          v.y = MfmaT<T>::kRowMajorOut ? acc[sp][g][2 * q + 1] : acc[sp][g][q + 2];
+          outs[((g * 8 + c8) * 16 + lr) * OLD + wave * PPW + sp] = v;
         }
-      }
+  }
+  __syncthreads();
+  {
+    // thread: r2 point ip and term t fixed, channel rows c8g = c8g0 + RS k (256 / TW rows per
+    // pass, a multiple of 16, so t stays the thread's)
+    constexpr int RS = 256 / TW / 16;
+    static_assert(256 % (16 * TW) == 0, "the copy's passes keep each thread's term");
+    const int ip = (int)threadIdx.x % TW, t = ((int)threadIdx.x / TW) % 16, c8g0 = (int)threadIdx.x / TW / 16;
+    const int tg = tn * 16 + t, i2 = i0 + ip;
+    const bool ok = tg < nterms && i2 < n;
+    C2* zr = reinterpret_cast<C2*>(Z) + ((long long)tg * nc * n + i1) * n + i2;  // channel 0's row
+#pragma unroll
+    for (int k = 0; k < MG * 8 / RS; ++k) {
+      const int c8g = c8g0 + RS * k;
+      const int c = gc * MG * 8 + c8g;
+      if (ok && c < nc) zr[(long long)c * P] = outs[(c8g * 16 + t) * OLD + ip];
     }
   }
 }
@@ -1620,10 +1656,7 @@ extern "C" int pp_fham_multiply_f32(const pp_fham* f, const double* scalings, co
 // forms need the separable split; the LDS-window forms a window that fits in 160 KiB.
 static bool radial_window_fits(const FHam& F, int v, int precision) {
   if (kRadWinTW[v] == 0) return true;
-  const int MG = kRadPosMG[v], esz = precision == 64 ? 8 : 4;
-  const int ps = esz == 4 && MG % 2 == 0 ? MG * 16 + 16 : MG * 16;
-  const size_t lds = (size_t)(2 * F.bw + 2) * (kRadWinTW[v] + 2 * F.bw) * ps * esz + 256 * 16;
-  return lds <= 160 * 1024;
+  return win_lds_bytes(F.bw, kRadPosMG[v], kRadWinTW[v], precision == 64 ? 8 : 4) <= 160 * 1024;
 }
 
 extern "C" int pp_fham_set_radial_variant(pp_fham* f, int precision, int variant) {
