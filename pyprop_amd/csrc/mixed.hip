@@ -55,32 +55,44 @@ __global__ void f64_to_f32_unit_kernel(long long n, const double2* __restrict__ 
 
 
 // Sum over the blocks of partial[b][j] (j < count, partial rows of `width`) in a fixed order:
-// 8 threads per column stride over the blocks, then a shuffle tree.  Every caller uses
-// 256-thread blocks, so any block that evaluates it, and finish_partials_kernel, obtains the
-// same bits.  dst may be LDS or global memory; the caller synchronises.
+// 8 threads per column stride over the blocks, 16 independent chains each (the loads of up to
+// 128 blocks in flight at once), then a shuffle tree.  Every caller uses 256-thread blocks, so
+// any block that evaluates it, and finish_partials_kernel, obtains the same bits.  dst may be
+// LDS or global memory; the caller synchronises.  Loads past the last block read the last
+// block's row and are multiplied by 0 (no load behind a branch).
 __device__ void sum_partials(const double2* __restrict__ partial, int blocks, int width, int count, double2* dst) {
+  constexpr int U = 16;
   const int q = threadIdx.x & 7;
   for (int j0 = 0; j0 < count; j0 += 32) {
     const int j = j0 + (int)(threadIdx.x >> 3);
     double2 acc = make_double2(0, 0);
     if (j < count) {
-      // four independent chains so that the loads of a pass are in flight together
-      double2 a4[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-      for (int b0 = q; b0 < blocks; b0 += 32) {
-        double2 v[4];
+      double2 a[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < U; ++u) a[u] = make_double2(0, 0);
+      for (int b0 = q; b0 < blocks; b0 += 8 * U) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
           const int b = b0 + 8 * u;
-          v[u] = b < blocks ? partial[(long long)b * width + j] : make_double2(0, 0);
+          v[u] = partial[(long long)min(b, blocks - 1) * width + j];
         }
+        __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first add
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          a4[u].x += v[u].x;
-          a4[u].y += v[u].y;
+        for (int u = 0; u < U; ++u) {
+          const double k = b0 + 8 * u < blocks ? 1.0 : 0.0;
+          a[u].x += k * v[u].x;
+          a[u].y += k * v[u].y;
         }
       }
-      acc.x = (a4[0].x + a4[1].x) + (a4[2].x + a4[3].x);
-      acc.y = (a4[0].y + a4[1].y) + (a4[2].y + a4[3].y);
+#pragma unroll
+      for (int h = U / 2; h > 0; h >>= 1)
+#pragma unroll
+        for (int u = 0; u < h; ++u) {
+          a[u].x += a[u + h].x;
+          a[u].y += a[u + h].y;
+        }
+      acc = a[0];
     }
     for (int off = 4; off > 0; off >>= 1) {
       acc.x += __shfl_down(acc.x, off, 8);
@@ -115,15 +127,19 @@ __global__ __launch_bounds__(256) void finish_partials_kernel(int blocks, const 
 //           else partial[b][0] = |w|^2
 // The update and the dots of a block cover the same slice with the same thread-to-element
 // map, so each thread reads back only what it wrote.  wout (UPDATE without DOT, may be NULL):
-// the updated vector goes there instead of back into w.
+// the updated vector goes there instead of back into w.  A thread takes its elements E at a
+// time and issues all their loads (8 basis vectors per batch) before using any: the indices
+// are clamped into the slice and the basis rows past m carry the coefficient 0, so no load
+// sits behind a branch.
 template <bool UPDATE, bool DOT>
 __global__ __launch_bounds__(256) void cgs32_kernel(long long n, int m, const float2* __restrict__ V, long long ld,
                                                     const double2* __restrict__ hpart, int hwidth,
                                                     float2* __restrict__ w, double2* __restrict__ partial,
                                                     float2* __restrict__ wout = nullptr) {
+  constexpr int E = 4;  // elements per thread per pass
   __shared__ double2 h64s[128];
-  __shared__ float2 hs[128];
-  __shared__ double2 sh[4];
+  __shared__ float2 hs[136];
+  __shared__ double2 sh[4][8];
   const int width = DOT ? m + 1 : 1;
   const long long chunk = (n + gridDim.x - 1) / gridDim.x;
   const long long beg = chunk * blockIdx.x;
@@ -132,23 +148,51 @@ __global__ __launch_bounds__(256) void cgs32_kernel(long long n, int m, const fl
   if (UPDATE) {
     sum_partials(hpart, gridDim.x, hwidth, m, h64s);
     __syncthreads();
-    for (int j = threadIdx.x; j < m; j += blockDim.x) hs[j] = make_float2((float)h64s[j].x, (float)h64s[j].y);
+    for (int j = threadIdx.x; j < 136; j += blockDim.x)
+      hs[j] = j < m ? make_float2((float)h64s[j].x, (float)h64s[j].y) : make_float2(0.f, 0.f);
     __syncthreads();
-    for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
-      float2 t = make_float2(0.f, 0.f);
-      for (int j = 0; j < m; ++j) {
-        const float2 v = V[j * ld + i];
-        t.x = fmaf(v.x, hs[j].x, t.x);
-        t.x = fmaf(-v.y, hs[j].y, t.x);
-        t.y = fmaf(v.x, hs[j].y, t.y);
-        t.y = fmaf(v.y, hs[j].x, t.y);
+    for (long long i0 = beg + threadIdx.x; i0 < end; i0 += E * 256) {
+      long long ie[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) ie[e] = min(i0 + e * 256, end - 1);
+      float2 t[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) t[e] = make_float2(0.f, 0.f);
+      for (int j0 = 0; j0 < m; j0 += 8) {
+        float2 v[8][E];
+#pragma unroll
+        for (int g = 0; g < 8; ++g)
+#pragma unroll
+          for (int e = 0; e < E; ++e) v[g][e] = V[(long long)min(j0 + g, m - 1) * ld + ie[e]];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int g = 0; g < 8; ++g) {
+          const float2 h = hs[j0 + g];  // 0 past m
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            t[e].x = fmaf(v[g][e].x, h.x, t[e].x);
+            t[e].x = fmaf(-v[g][e].y, h.y, t[e].x);
+            t[e].y = fmaf(v[g][e].x, h.y, t[e].y);
+            t[e].y = fmaf(v[g][e].y, h.x, t[e].y);
+          }
+        }
       }
-      float2 wv = w[i];
-      wv.x -= t.x;
-      wv.y -= t.y;
-      if (!DOT && wout) wout[i] = wv;
-      else w[i] = wv;
-      nrm += (double)wv.x * wv.x + (double)wv.y * wv.y;
+      float2 wv[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) wv[e] = w[ie[e]];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const bool ok = i0 + e * 256 < end;
+        wv[e].x -= t[e].x;
+        wv[e].y -= t[e].y;
+        nrm += ok ? (double)wv[e].x * wv[e].x + (double)wv[e].y * wv[e].y : 0.0;
+      }
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        if (i0 + e * 256 >= end) continue;
+        if (!DOT && wout) wout[ie[e]] = wv[e];
+        else w[ie[e]] = wv[e];
+      }
     }
   }
   if (DOT) {
@@ -156,40 +200,54 @@ __global__ __launch_bounds__(256) void cgs32_kernel(long long n, int m, const fl
       double2 acc[8];
 #pragma unroll
       for (int g = 0; g < 8; ++g) acc[g] = make_double2(0, 0);
-      for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
-        const float2 wv = w[i];
+      for (long long i0 = beg + threadIdx.x; i0 < end; i0 += E * 256) {
+        long long ie[E];
 #pragma unroll
-        for (int g = 0; g < 8; ++g) {
-          const int j = j0 + g;
-          if (j < m) {
-            const float2 v = V[j * ld + i];
-            acc[g].x += (double)v.x * wv.x + (double)v.y * wv.y;
-            acc[g].y += (double)v.x * wv.y - (double)v.y * wv.x;
-          } else if (j == m) {
-            acc[g].x += (double)wv.x * wv.x + (double)wv.y * wv.y;
+        for (int e = 0; e < E; ++e) ie[e] = min(i0 + e * 256, end - 1);
+        float2 wv[E], v[8][E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) wv[e] = w[ie[e]];
+#pragma unroll
+        for (int g = 0; g < 8; ++g)
+#pragma unroll
+          for (int e = 0; e < E; ++e) v[g][e] = V[(long long)min(j0 + g, max(m - 1, 0)) * ld + ie[e]];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {  // selects, not branches (a load used in a branch is sunk into it)
+          const bool ok = i0 + e * 256 < end;
+          const double nn = (double)wv[e].x * wv[e].x + (double)wv[e].y * wv[e].y;
+#pragma unroll
+          for (int g = 0; g < 8; ++g) {
+            const int j = j0 + g;
+            const double dx = (double)v[g][e].x * wv[e].x + (double)v[g][e].y * wv[e].y;
+            const double dy = (double)v[g][e].x * wv[e].y - (double)v[g][e].y * wv[e].x;
+            acc[g].x += ok ? (j < m ? dx : j == m ? nn : 0.0) : 0.0;
+            acc[g].y += ok && j < m ? dy : 0.0;
           }
         }
       }
+      // block sums of the batch: shuffle trees per wave, one barrier, then the four waves'
+      // sums in wave order
 #pragma unroll
-      for (int g = 0; g < 8; ++g) {
-        const int j = j0 + g;
-        if (j > m) break;  // uniform over the block
-        double2 v = acc[g];
+      for (int g = 0; g < 8; ++g)
         for (int off = 32; off > 0; off >>= 1) {
-          v.x += __shfl_down(v.x, off, 64);
-          v.y += __shfl_down(v.y, off, 64);
+          acc[g].x += __shfl_down(acc[g].x, off, 64);
+          acc[g].y += __shfl_down(acc[g].y, off, 64);
         }
-        __syncthreads();
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-          double2 t = sh[0];
-          for (int q = 1; q < 4; ++q) {
-            t.x += sh[q].x;
-            t.y += sh[q].y;
-          }
-          partial[(long long)blockIdx.x * width + j] = t;
+      __syncthreads();  // the previous batch's sh is read
+      if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int g = 0; g < 8; ++g) sh[threadIdx.x >> 6][g] = acc[g];
+      }
+      __syncthreads();
+      if (threadIdx.x < 8 && j0 + (int)threadIdx.x <= m) {
+        const int g = threadIdx.x;
+        double2 t = sh[0][g];
+        for (int qq = 1; qq < 4; ++qq) {
+          t.x += sh[qq][g].x;
+          t.y += sh[qq][g].y;
         }
+        partial[(long long)blockIdx.x * width + j0 + g] = t;
       }
     }
   } else {
@@ -199,9 +257,9 @@ __global__ __launch_bounds__(256) void cgs32_kernel(long long n, int m, const fl
         nrm += (double)wv.x * wv.x + (double)wv.y * wv.y;
       }
     for (int off = 32; off > 0; off >>= 1) nrm += __shfl_down(nrm, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = make_double2(nrm, 0.0);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][0] = make_double2(nrm, 0.0);
     __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sh[0].x + sh[1].x + sh[2].x + sh[3].x, 0.0);
+    if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sh[0][0].x + sh[1][0].x + sh[2][0].x + sh[3][0].x, 0.0);
   }
 }
 
