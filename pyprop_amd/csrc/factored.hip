@@ -17,6 +17,7 @@
 // Both stages use v_mfma_{f32,f64}_16x16x4.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <type_traits>
 #include <vector>
@@ -1093,10 +1094,11 @@ static typename MfmaT<T>::C2 to_c2(double2 v) {
 //    channel-minor copy Xt[j1][j2][c][part] that the radial kernels read (the transpose
 //    launch is then skipped).
 // A pass out[o] = sum_t coef(o, t) in[o + off + t] runs along one radial axis: wave k
-// computes outputs 4k .. 4k + 3 (so the coefficients are wave-uniform: scalar loads), lane p
-// the index across the axis, holding the 3 + taps inputs it needs in registers (4 + taps - 1
-// LDS reads for 4 outputs).  LDS rows have stride kSepLD, so that lanes reading down a column
-// hit distinct banks.
+// computes outputs 4k .. 4k + 3 (so the coefficients are wave-uniform: broadcast LDS reads),
+// lane p the index across the axis, holding the 3 + taps inputs it needs in registers (4 +
+// taps - 1 LDS reads for 4 outputs).  LDS rows have stride kSepLD, so that lanes reading down
+// a column hit distinct banks.  The job's two coefficient arrays are copied into LDS with the
+// channel's block, so no pass waits on a global load.
 constexpr int kSepLD = 65;
 constexpr int kSepR = 4;  // outputs per lane along a pass (16 waves x 4 >= 64)
 template <class T, int TAPS, bool ALONG_R1, class CoefFn>
@@ -1144,24 +1146,36 @@ __global__ __launch_bounds__(1024) void fham_separable_kernel(int nc, int n, int
                                                               int cslot, T* __restrict__ Xt, int ncp) {
   typedef typename MfmaT<T>::C2 C2;
   constexpr int W = 2 * BW + 1;
-  if (differs && *differs == 0) return;
+  if (differs && *differs == 0) return;  // input equals the one whose products Z holds
   extern __shared__ char sep_raw[];
   const int m = n * n;
   C2* Xs = reinterpret_cast<C2*>(sep_raw);  // n rows of kSepLD
   C2* Ts = Xs + n * kSepLD;
+  T* Cs = reinterpret_cast<T*>(Ts + n * kSepLD);  // the job's coefficients: two arrays of cn values
   const int c = blockIdx.x;
   // with cu1, y = 0 is the Cholesky-form overlap job (dispatched first: its four dependent
   // passes are the longest chain) and y - 1 the separable term; else y is the term
   const bool chol = cu1 && blockIdx.y == 0;
   const int j = cu1 ? (int)blockIdx.y - 1 : (int)blockIdx.y;
   const C2* x = X + (size_t)c * m;
+  const T* ca = chol ? cu1 : SA + (size_t)max(j, 0) * n * W;  // U1 or a
+  const T* cb = chol ? cu2 : SB + (size_t)max(j, 0) * n * W;  // U2 or b
+  const int cn = chol ? n * (BW + 1) : n * W;
   // staging: every load of a thread is issued before the first LDS store
-  constexpr int kStageX = 4;  // m <= 4096 (separable_fits)
+  constexpr int kStageX = 4;                            // m <= 4096 (separable_fits)
+  constexpr int kStageC = (2 * 64 * W + 1023) / 1024;   // 2 cn <= 2 n W, n <= 64
   C2 xv[kStageX];
+  T cv[kStageC];
 #pragma unroll
   for (int q = 0; q < kStageX; ++q) {  // past m: element 0 again (not stored)
     const int e = threadIdx.x + q * 1024;
     xv[q] = x[e < m ? e : 0];
+  }
+#pragma unroll
+  for (int q = 0; q < kStageC; ++q) {  // past 2 cn: the first value again (not stored)
+    const int e = threadIdx.x + q * 1024;
+    const int ec = e < 2 * cn ? e : 0;
+    cv[q] = ec < cn ? ca[ec] : cb[ec - cn];
   }
 #pragma unroll
   for (int q = 0; q < kStageX; ++q) {
@@ -1170,6 +1184,11 @@ __global__ __launch_bounds__(1024) void fham_separable_kernel(int nc, int n, int
       const int i = e / n;
       Xs[i * kSepLD + (e - i * n)] = xv[q];
     }
+  }
+#pragma unroll
+  for (int q = 0; q < kStageC; ++q) {
+    const int e = threadIdx.x + q * 1024;
+    if (e < 2 * cn) Cs[e] = cv[q];
   }
   __syncthreads();
   if (Xt) {  // the channel's workgroups share the channel-minor copy, a slice each
@@ -1191,18 +1210,20 @@ __global__ __launch_bounds__(1024) void fham_separable_kernel(int nc, int n, int
       return [u, nn](int o, int t) { return u[min(o + t, nn - 1) * (BW + 1) + BW - t]; };
     };
     auto UT = [&](const T* u) { return [u](int o, int t) { return u[o * (BW + 1) + t]; }; };
-    sep_pass<T, BW + 1, false>(Xs, Ts, kSepLD, n, 0, o0, p, Uc(cu2));    // U2 along r2
+    const T* u1 = Cs;
+    const T* u2 = Cs + cn;
+    sep_pass<T, BW + 1, false>(Xs, Ts, kSepLD, n, 0, o0, p, Uc(u2));    // U2 along r2
     __syncthreads();
-    sep_pass<T, BW + 1, false>(Ts, Xs, kSepLD, n, -BW, o0, p, UT(cu2));  // U2^T along r2
+    sep_pass<T, BW + 1, false>(Ts, Xs, kSepLD, n, -BW, o0, p, UT(u2));  // U2^T along r2
     __syncthreads();
-    sep_pass<T, BW + 1, true>(Xs, Ts, kSepLD, n, 0, o0, p, Uc(cu1));     // U1 along r1
+    sep_pass<T, BW + 1, true>(Xs, Ts, kSepLD, n, 0, o0, p, Uc(u1));     // U1 along r1
     __syncthreads();
     C2* z = Z + ((size_t)cslot * nc + c) * m;
-    sep_pass<T, BW + 1, true>(Ts, z, n, n, -BW, o0, p, UT(cu1));         // U1^T along r1, to the slot
+    sep_pass<T, BW + 1, true>(Ts, z, n, n, -BW, o0, p, UT(u1));         // U1^T along r1, to the slot
     return;
   }
-  const T* fa = SA + (size_t)j * n * W;
-  const T* fb = SB + (size_t)j * n * W;
+  const T* fa = Cs;
+  const T* fb = Cs + cn;
   // T = X b^T along r2: T[i1][o] = sum_d b[o][d] X[i1][o - BW + d]
   sep_pass<T, W, false>(Xs, Ts, kSepLD, n, -BW, o0, p, [fb](int o, int t) { return fb[o * W + t]; });
   __syncthreads();
@@ -1255,7 +1276,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   auto launch_separable = [&](T* xt) {
     if (sep_jobs <= 0) return;
     typedef typename MfmaT<T>::C2 C2s;
-    const size_t lds = 2 * (size_t)n * kSepLD * sizeof(C2s);
+    const size_t lds = 2 * (size_t)n * kSepLD * sizeof(C2s) + 2 * (size_t)n * (2 * F.bw + 1) * sizeof(T);
     const T* cu1 = chol ? reinterpret_cast<const T*>(F.cu1_32.ptr) : nullptr;
     const T* cu2 = chol ? reinterpret_cast<const T*>(F.cu2_32.ptr) : nullptr;
     switch (F.bw) {
@@ -1503,8 +1524,8 @@ static bool separable_factors(const double* D, int n, int bw, std::vector<double
 
 // the separable kernel holds X_c, one pass and both factors in LDS, and maps one thread to
 // each of at most 64 columns
-static bool separable_fits(int n) {
-  return n <= 64 && 2 * (size_t)n * kSepLD * sizeof(double2) <= 160 * 1024;
+static bool separable_fits(int n) {  // FP64 block and pass in LDS, with two coefficient arrays (band <= 8)
+  return n <= 64 && 2 * (size_t)n * kSepLD * sizeof(double2) + 2 * (size_t)n * 17 * sizeof(double) <= 160 * 1024;
 }
 
 extern "C" int pp_fham_create(int nchan, int n, int nterms, int bw, const double* A, const int* term_potential,
@@ -1711,7 +1732,8 @@ extern "C" int pp_fham_get_angular_waves(const pp_fham* f, int precision, int* w
 }
 
 // Times every radial variant on a zero input (one warm-up and `trials` timed multiplies each,
-// HIP events on `stream`) and keeps the fastest for that precision.
+// HIP events on `stream`, in three interleaved rounds keeping each variant's fastest) and keeps
+// the fastest for that precision; then the angular forms with it, the same way.
 extern "C" int pp_fham_tune(pp_fham* f, int precision, int trials, int* best, double* timings_ms, pp_stream stream) {
   return guarded([&] {
     PP_REQUIRE(f && (precision == 64 || precision == 32) && trials >= 1, "invalid argument");
@@ -1736,35 +1758,8 @@ extern "C" int pp_fham_tune(pp_fham* f, int precision, int trials, int* best, do
     hipEvent_t e0, e1;
     PP_HIP(hipEventCreate(&e0));
     PP_HIP(hipEventCreate(&e1));
-    double best_ms = 1e300;
-    int best_v = var;
-    for (int v = 0; v < kFhamRadialVariants; ++v) {
-      if ((kRadPosMG[v] > 0 && !F.sep_enabled) || !radial_window_fits(F, v, precision)) {
-        if (timings_ms) timings_ms[v] = -1.0;  // not available for this operator
-        continue;
-      }
-      var = v;
-      run();  // warm-up
-      PP_HIP(hipEventRecord(e0, s));
-      for (int k = 0; k < trials; ++k) run();
-      PP_HIP(hipEventRecord(e1, s));
-      PP_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      PP_HIP(hipEventElapsedTime(&ms, e0, e1));
-      const double per = ms / trials;
-      if (timings_ms) timings_ms[v] = per;
-      if (per < best_ms) {
-        best_ms = per;
-        best_v = v;
-      }
-    }
-    var = best_v;
-    // then the angular stage's wave count, with the chosen radial shape
-    int& avar = precision == 64 ? F.ang_variant64 : F.ang_variant32;
-    double best_a = 1e300;
-    int best_av = avar;
-    for (int v = 0; v < kFhamAngularVariants; ++v) {
-      avar = v;
+    // ms per multiply of the current choice: one warm-up, then `trials` timed multiplies
+    auto time_one = [&]() {
       run();
       PP_HIP(hipEventRecord(e0, s));
       for (int k = 0; k < trials; ++k) run();
@@ -1772,12 +1767,39 @@ extern "C" int pp_fham_tune(pp_fham* f, int precision, int trials, int* best, do
       PP_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
       PP_HIP(hipEventElapsedTime(&ms, e0, e1));
-      const double per = ms / trials;
-      if (timings_ms) timings_ms[kFhamRadialVariants + v] = per;
-      if (per < best_a) {
-        best_a = per;
-        best_av = v;
+      return (double)ms / trials;
+    };
+    // every candidate is timed in kRounds interleaved rounds and keeps its fastest, so that a
+    // slow stretch of the device (clock ramp, another process) does not decide the choice
+    constexpr int kRounds = 3;
+    const int var0 = var;
+    std::vector<double> rad(kFhamRadialVariants, 1e300);
+    for (int round = 0; round < kRounds; ++round)
+      for (int v = 0; v < kFhamRadialVariants; ++v) {
+        if ((kRadPosMG[v] > 0 && !F.sep_enabled) || !radial_window_fits(F, v, precision)) continue;
+        var = v;
+        rad[v] = std::min(rad[v], time_one());
       }
+    int best_v = -1;
+    for (int v = 0; v < kFhamRadialVariants; ++v) {
+      const bool avail = rad[v] < 1e300;
+      if (timings_ms) timings_ms[v] = avail ? rad[v] : -1.0;  // -1: not available for this operator
+      if (avail && (best_v < 0 || rad[v] < rad[best_v])) best_v = v;
+    }
+    if (best_v < 0) best_v = var0;
+    var = best_v;
+    // then the angular stage's form, with the chosen radial shape
+    int& avar = precision == 64 ? F.ang_variant64 : F.ang_variant32;
+    std::vector<double> ang(kFhamAngularVariants, 1e300);
+    for (int round = 0; round < kRounds; ++round)
+      for (int v = 0; v < kFhamAngularVariants; ++v) {
+        avar = v;
+        ang[v] = std::min(ang[v], time_one());
+      }
+    int best_av = 0;
+    for (int v = 0; v < kFhamAngularVariants; ++v) {
+      if (timings_ms) timings_ms[kFhamRadialVariants + v] = ang[v];
+      if (ang[v] < ang[best_av]) best_av = v;
     }
     avar = best_av;
     PP_HIP(hipEventDestroy(e0));
