@@ -423,24 +423,24 @@ __host__ __device__ constexpr int win_point_stride() {
 __host__ __device__ constexpr int win_rows_per_batch(int rows, int cap) {
   return cap <= 1 ? 1 : rows % cap == 0 ? cap : win_rows_per_batch(rows, cap - 1);
 }
-// LDS of fham_radial_win_kernel: the window, then 16 bytes per thread that the copy's slots
-// outside the window write; afterwards the same space holds the products, MG x 8 x 16 rows of
-// TW + 1 complex values
-__host__ __device__ constexpr size_t win_lds_bytes(int bw, int mg, int tw, int esz) {
+// LDS of fham_radial_win_kernel with nw waves: the window, then 16 bytes per thread that the
+// copy's slots outside the window write; afterwards the same space holds the products,
+// MG x 8 x 16 rows of TW + 1 complex values
+__host__ __device__ constexpr size_t win_lds_bytes(int bw, int mg, int tw, int nw, int esz) {
   const size_t win = (size_t)(2 * bw + 2) * (tw + 2 * bw) * (esz == 4 && mg % 2 == 0 ? mg * 16 + 16 : mg * 16) * esz +
-                     256 * 16;
+                     (size_t)64 * nw * 16;
   const size_t out = (size_t)mg * 8 * 16 * (tw + 1) * 2 * esz;
   return win > out ? win : out;
 }
-template <class T, int BW, int MG, int TW>
+template <class T, int BW, int MG, int TW, int NW>
 __host__ __device__ constexpr size_t win_lds_bytes() {
-  return win_lds_bytes(BW, MG, TW, (int)sizeof(T));
+  return win_lds_bytes(BW, MG, TW, NW, (int)sizeof(T));
 }
 
 // Stage 1, position form with the window in LDS.  A workgroup takes one r1 point i1, TW
 // consecutive r2 points i0 .. i0 + TW - 1 and channel groups gc MG .. gc MG + MG - 1.  It
-// copies the X values of the union of their windows from Xt into LDS once; its four waves
-// then form the products of TW / 4 positions each, the GEMM of fham_radial_pos_kernel per
+// copies the X values of the union of their windows from Xt into LDS once; its NW waves
+// then form the products of TW / NW positions each, the GEMM of fham_radial_pos_kernel per
 // position, with the X operands read from LDS and the next position's R operands loaded
 // while the current position's MFMAs run.  LDS point (jr, jc), jr <= w, jc < TW + 2 BW,
 // holds the MG x 16 operand values of grid point (i1 - BW + jr, i0 - BW + jc): zero outside
@@ -449,8 +449,8 @@ __host__ __device__ constexpr size_t win_lds_bytes() {
 // (jj, ip + d), ip the position's offset in the tile.  With (jj0, d0) = divmod(4 ks, w) that
 // point is jj0 WC + d0 + lk, plus WC - w where d0 + lk >= w (the row wraps): a lane base
 // chosen by ks at compile time plus an immediate, so the reads take no address arithmetic.
-template <class T, int BW, int MG, int TW>
-__global__ __launch_bounds__(256) void fham_radial_win_kernel(int nc, int n, int nterms, int ngroups, int ntiles,
+template <class T, int BW, int MG, int TW, int NW>
+__global__ __launch_bounds__(64 * NW) void fham_radial_win_kernel(int nc, int n, int nterms, int ngroups, int ntiles,
                                                               long long nblocks, int ncp, const T* __restrict__ Xt,
                                                               const T* __restrict__ RP, T* __restrict__ Z,
                                                               const int* __restrict__ differs) {
@@ -460,9 +460,10 @@ __global__ __launch_bounds__(256) void fham_radial_win_kernel(int nc, int n, int
   constexpr int w = 2 * BW + 1, KK = w * w, KP = (KK + 3) / 4;
   constexpr int WC = TW + 2 * BW;  // window columns
   constexpr int PS = win_point_stride<T, MG>();
-  constexpr int PPW = TW / 4;  // positions per wave
-  constexpr int KB = 4;        // K steps per batch of LDS reads
-  static_assert(TW % 4 == 0, "four waves share the tile's positions");
+  constexpr int NT = 64 * NW;   // threads
+  constexpr int PPW = TW / NW;  // positions per wave
+  constexpr int KB = 4;         // K steps per batch of LDS reads
+  static_assert(TW % NW == 0, "the waves share the tile's positions evenly");
   extern __shared__ __align__(16) char win_raw[];
   T* win = reinterpret_cast<T*>(win_raw);
   const int per = (int)((nblocks + 7) / 8);  // nblocks < 2^31 (n <= 64)
@@ -497,7 +498,7 @@ __global__ __launch_bounds__(256) void fham_radial_win_kernel(int nc, int n, int
     constexpr int VPP = 16 / sizeof(T);  // values per piece
     constexpr int PPP = MG * 16 / VPP;   // pieces per point
     constexpr int SLOTS = PPP <= 4 ? 4 : PPP <= 8 ? 8 : PPP <= 16 ? 16 : 32;
-    constexpr int CPI = 256 / SLOTS;     // columns per pass
+    constexpr int CPI = NT / SLOTS;      // columns per pass
     constexpr int NCS = (WC + CPI - 1) / CPI;
     constexpr int RB = win_rows_per_batch(2 * BW + 2, NCS >= 28 ? 1 : 28 / NCS);
     static_assert(PPP <= 32, "at most 32 pieces per point");
@@ -627,10 +628,10 @@ __global__ __launch_bounds__(256) void fham_radial_win_kernel(int nc, int n, int
   }
   __syncthreads();
   {
-    // thread: r2 point ip and term t fixed, channel rows c8g = c8g0 + RS k (256 / TW rows per
+    // thread: r2 point ip and term t fixed, channel rows c8g = c8g0 + RS k (NT / TW rows per
     // pass, a multiple of 16, so t stays the thread's)
-    constexpr int RS = 256 / TW / 16;
-    static_assert(256 % (16 * TW) == 0, "the copy's passes keep each thread's term");
+    constexpr int RS = NT / TW / 16;
+    static_assert(NT % (16 * TW) == 0, "the copy's passes keep each thread's term");
     const int ip = (int)threadIdx.x % TW, t = ((int)threadIdx.x / TW) % 16, c8g0 = (int)threadIdx.x / TW / 16;
     const int tg = tn * 16 + t, i2 = i0 + ip;
     const bool ok = tg < nterms && i2 < n;
@@ -654,7 +655,10 @@ constexpr bool kRadPipe[kFhamRadialVariants] = {false, false, false, false, true
                                                 false, false, false, false, false, false, false};
 constexpr int kRadPosMG[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 6, 3, 2, 3, 2, 2, 1};
 // position variants with the window in LDS (fham_radial_win_kernel): positions per workgroup
+// and waves per workgroup (eight waves of two positions each measured no faster at helium:
+// more waves per SIMD do not shorten it)
 constexpr int kRadWinTW[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 16, 16, 8, 16};
+constexpr int kRadWinNW[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 4, 4, 4, 4};
 // angular variants: waves splitting K, and whether one workgroup covers all row blocks of
 // its position tile (reading each Z element once) or one row block
 constexpr int kAngWaveChoices[kFhamAngularVariants] = {16, 8, 4, 8, 16, 4, 4};
@@ -668,22 +672,22 @@ static void launch_radial(int variant, int nc, int ncp, int n, int nterms, int n
                           const T* BT, const T* RP, T* Z, const int* differs, hipStream_t s) {
   if (kRadWinTW[variant] > 0) {
     if (!RP) throw Error(PP_ERR_INVALID, "position-form radial variant without position tiles");
-    const int MG = kRadPosMG[variant], TW = kRadWinTW[variant];
+    const int MG = kRadPosMG[variant], TW = kRadWinTW[variant], NW = kRadWinNW[variant];
     const int ntiles = (n + TW - 1) / TW;
     const long long nblocks = (long long)((nterms + 15) / 16) * n * ntiles * ((ngroups + MG - 1) / MG);
     const dim3 grid((unsigned)((nblocks + 7) / 8 * 8));
-#define PP_RADW(G, W)                                                                                          \
-  if (MG == G && TW == W) {                                                                                    \
-    constexpr size_t lds = win_lds_bytes<T, B, G, W>();                                                        \
-    if (lds > 160 * 1024) throw Error(PP_ERR_INVALID, "radial window does not fit in LDS");                    \
-    PP_HIP(hipFuncSetAttribute((const void*)fham_radial_win_kernel<T, B, G, W>,                                \
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                         \
-    hipLaunchKernelGGL((fham_radial_win_kernel<T, B, G, W>), grid, dim3(256), lds, s, nc, n, nterms, ngroups,  \
-                       ntiles, nblocks, ncp, Xt, RP, Z, differs);                                              \
-    PP_HIP(hipGetLastError());                                                                                 \
-    return;                                                                                                    \
+#define PP_RADW(G, W, V)                                                                                          \
+  if (MG == G && TW == W && NW == V) {                                                                            \
+    constexpr size_t lds = win_lds_bytes<T, B, G, W, V>();                                                        \
+    if (lds > 160 * 1024) throw Error(PP_ERR_INVALID, "radial window does not fit in LDS");                       \
+    PP_HIP(hipFuncSetAttribute((const void*)fham_radial_win_kernel<T, B, G, W, V>,                                \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                            \
+    hipLaunchKernelGGL((fham_radial_win_kernel<T, B, G, W, V>), grid, dim3(64 * V), lds, s, nc, n, nterms,        \
+                       ngroups, ntiles, nblocks, ncp, Xt, RP, Z, differs);                                        \
+    PP_HIP(hipGetLastError());                                                                                    \
+    return;                                                                                                       \
   }
-    PP_RADW(3, 16) PP_RADW(2, 16) PP_RADW(2, 8) PP_RADW(1, 16)
+    PP_RADW(3, 16, 4) PP_RADW(2, 16, 4) PP_RADW(2, 8, 4) PP_RADW(1, 16, 4)
 #undef PP_RADW
     throw Error(PP_ERR_INVALID, "radial window shape not instantiated");
   }
@@ -1677,7 +1681,7 @@ extern "C" int pp_fham_multiply_f32(const pp_fham* f, const double* scalings, co
 // forms need the separable split; the LDS-window forms a window that fits in 160 KiB.
 static bool radial_window_fits(const FHam& F, int v, int precision) {
   if (kRadWinTW[v] == 0) return true;
-  return win_lds_bytes(F.bw, kRadPosMG[v], kRadWinTW[v], precision == 64 ? 8 : 4) <= 160 * 1024;
+  return win_lds_bytes(F.bw, kRadPosMG[v], kRadWinTW[v], kRadWinNW[v], precision == 64 ? 8 : 4) <= 160 * 1024;
 }
 
 extern "C" int pp_fham_set_radial_variant(pp_fham* f, int precision, int variant) {
