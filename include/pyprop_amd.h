@@ -568,6 +568,11 @@ int pp_mixed_set_device_mode(pp_mixed* m, int device_mode);
  * only to ~eps32 / h_{j+1,j}, which the FP64 refinement absorbs (at helium: the same 3 inner
  * iterations per correction and 1e-11 error after 20 steps).  2: two passes (4 launches). */
 int pp_mixed_set_gram_schmidt(pp_mixed* m, int passes);
+/* Where a one-pass Gram-Schmidt is finished (no reference counterpart): 1 (the default) by the
+ * next Kronecker solve when the host reads nothing before it (the solve sums the block
+ * partials itself; one launch fewer per such iteration, bitwise the same results); 0 by a
+ * finishing launch of its own after every iteration. */
+int pp_mixed_set_deferred_finish(pp_mixed* m, int on);
 /* Graph mode (no reference counterpart; with device mode 1): a device-mode step is one fixed
  * sequence of launches for given correction and inner-iteration counts; it is captured into
  * a hipGraph once, after the same sequence has run eagerly, and replayed on the caller's

@@ -114,6 +114,7 @@ SIGNATURES = [
     ("pp_mixed_set_factored", I, [P, P]),
     ("pp_mixed_set_device_mode", I, [P, I]),
     ("pp_mixed_set_gram_schmidt", I, [P, I]),
+    ("pp_mixed_set_deferred_finish", I, [P, I]),
     ("pp_mixed_set_graph", I, [P, I]),
     ("pp_mixed_graph_stats", I, [P, P, P]),
     ("pp_cayley_set_factored", I, [P, P]),

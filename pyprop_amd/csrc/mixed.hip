@@ -55,21 +55,20 @@ __global__ void f64_to_f32_unit_kernel(long long n, const double2* __restrict__ 
 
 
 // Sum over the blocks of partial[b][j] (j < count, partial rows of `width`) in a fixed order:
-// 8 threads per column stride over the blocks, 16 independent chains each (the loads of up to
-// 128 blocks in flight at once), then a shuffle tree.  Every caller uses 256-thread blocks, so
-// any block that evaluates it, and finish_partials_kernel, obtains the same bits.  dst may be
-// LDS or global memory; the caller synchronises.  Loads past the last block read the last
-// block's row and are multiplied by 0 (no load behind a branch).
-__device__ void sum_partials(const double2* __restrict__ partial, int blocks, int width, int count, double2* dst) {
-  constexpr int U = 16;
+// 8 threads per column stride over the blocks, each issuing the loads of U blocks at once and
+// adding them in block order into one accumulator, then a shuffle tree.  Every caller uses
+// this function with 256 or more threads per block, so any block that evaluates it, and
+// finish_partials_kernel, obtains the same bits.  dst may be LDS or global memory; the caller
+// synchronises.  Loads past the last block read the last block's row and are multiplied by 0
+// (no load behind a branch).
+__device__ __forceinline__ void sum_partials(const double2* __restrict__ partial, int blocks, int width, int count,
+                                             double2* dst) {
+  constexpr int U = 12;
   const int q = threadIdx.x & 7;
   for (int j0 = 0; j0 < count; j0 += 32) {
     const int j = j0 + (int)(threadIdx.x >> 3);
     double2 acc = make_double2(0, 0);
     if (j < count) {
-      double2 a[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) a[u] = make_double2(0, 0);
       for (int b0 = q; b0 < blocks; b0 += 8 * U) {
         double2 v[U];
 #pragma unroll
@@ -81,18 +80,10 @@ __device__ void sum_partials(const double2* __restrict__ partial, int blocks, in
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const double k = b0 + 8 * u < blocks ? 1.0 : 0.0;
-          a[u].x += k * v[u].x;
-          a[u].y += k * v[u].y;
+          acc.x += k * v[u].x;
+          acc.y += k * v[u].y;
         }
       }
-#pragma unroll
-      for (int h = U / 2; h > 0; h >>= 1)
-#pragma unroll
-        for (int u = 0; u < h; ++u) {
-          a[u].x += a[u + h].x;
-          a[u].y += a[u + h].y;
-        }
-      acc = a[0];
     }
     for (int off = 4; off > 0; off >>= 1) {
       acc.x += __shfl_down(acc.x, off, 8);
@@ -1130,8 +1121,9 @@ __global__ __launch_bounds__(1024) void kron_split1_kernel(const float* __restri
 
 __global__ __launch_bounds__(1024) void kron_split2_kernel(const float* __restrict__ E, const int* __restrict__ l1,
                                                            const int* __restrict__ l2, int n,
-                                                           const float2* __restrict__ g, float2* out, float2* in_wb,
-                                                           float in_scale, const double2* __restrict__ in_n2) {
+                                                           const float2* __restrict__ g, const float2* in, float2* out,
+                                                           float2* in_wb, float in_scale,
+                                                           const double2* __restrict__ in_n2) {
   extern __shared__ float kl[];
   float* Gre = kl;                // 64 x KLD
   float* Gim = Gre + 64 * KLD;
@@ -1160,7 +1152,7 @@ __global__ __launch_bounds__(1024) void kron_split2_kernel(const float* __restri
   float2 wv = make_float2(0.f, 0.f);
   const int wi = i0 + tid / 64, wj = tid % 64;  // this thread's element of the write-back strip
   const bool wok = xb && wi < n && wj < n && wi < i0 + 16;
-  if (wok) wv = xb[wi * n + wj];
+  if (wok) wv = in[(long long)c * n2 + wi * n + wj];  // the input (the write-back may be elsewhere)
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int idx = tid + q * 1024, i = idx >> 6, j = idx & 63;
@@ -1206,15 +1198,20 @@ __global__ __launch_bounds__(1024) void kron_split2_kernel(const float* __restri
 // tile each: the four workgroups of a channel repeat that half of the work, which costs
 // less than a second launch and the exchange of G through memory), then its strip
 // V = E1[strip, :] G and X = V E2^T (waves 0-3).  Every tile is the split kernels' tile with
-// the same operands and order, so the result is theirs bit for bit.
+// the same operands and order, so the result is theirs bit for bit.  With a pending
+// Gram-Schmidt finish (fin.p3), every workgroup sums the |w|^2 block partials itself (the
+// sum finish_cgs1_kernel forms, the same bits) for the scale, and workgroup (0, 0) also sums
+// the dots and writes the finished results in that kernel's layout: the finish takes no
+// launch of its own.
 __global__ __launch_bounds__(1024) void kron_fused_kernel(const float* __restrict__ E, const float* __restrict__ lam,
                                                           const int* __restrict__ l1, const int* __restrict__ l2,
                                                           int n, float2 a, const float2* in, float in_scale,
                                                           const float2* __restrict__ add,
                                                           const double* __restrict__ add_norm, float add_sigma,
                                                           const double2* __restrict__ in_n2, float2* out,
-                                                          float2* in_wb) {
-  if (in_n2) {
+                                                          float2* in_wb, GsFinish fin) {
+  __shared__ double2 fres[130];
+  if (!fin.p3 && in_n2) {
     const double q = in_n2->x;
     in_scale = q > 0 ? (float)(1.0 / sqrt(q)) : 0.f;
   }
@@ -1244,6 +1241,22 @@ __global__ __launch_bounds__(1024) void kron_fused_kernel(const float* __restric
     ev1[q] = ok ? E1[i * n + j] : 0.f;
     ev2[q] = ok ? E2[i * n + j] : 0.f;
     yv[q] = (add && ok) ? add[(long long)c * n2 + i * n + j] : make_float2(0.f, 0.f);
+  }
+  if (fin.p3) {  // after the loads above are issued, so that their latencies overlap
+    const bool first = blockIdx.x == 0 && blockIdx.y == 0;
+    if (first) sum_partials(fin.p1, fin.blocks, fin.m + 1, fin.m + 1, fres);
+    sum_partials(fin.p3, fin.blocks, 1, 1, fres + fin.m + 1);
+    __syncthreads();
+    const double q = fres[fin.m + 1].x;
+    in_scale = q > 0 ? (float)(1.0 / sqrt(q)) : 0.f;
+    if (first) {
+      const int width = fin.m + 1;
+      for (int j = tid; j < 2 * width + 1; j += blockDim.x) {
+        const double2 v = j < width ? fres[j] : (j == 2 * width ? fres[width] : make_double2(0.0, 0.0));
+        fin.out[j] = v;
+        if (fin.out2) fin.out2[j] = v;
+      }
+    }
   }
   const float as = add ? add_sigma * (float)*add_norm : 0.f;
   float2* xb = in_wb ? in_wb + (long long)c * n2 : nullptr;
@@ -1365,8 +1378,8 @@ void Kron::multiply64(double2 a, double2* psi, hipStream_t s) const {
 void Kron::apply32(double2 a, float2* psi, hipStream_t s) const { apply32(a, psi, psi, s, 1.f, nullptr, nullptr); }
 
 void Kron::apply32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
-                   const double2* in_n2) const {
-  launch32(a, in, out, s, in_scale, wb, nullptr, nullptr, 0.f, in_n2);
+                   const double2* in_n2, const GsFinish* fin) const {
+  launch32(a, in, out, s, in_scale, wb, nullptr, nullptr, 0.f, in_n2, fin);
 }
 
 void Kron::apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
@@ -1375,15 +1388,23 @@ void Kron::apply32_add(double2 a, const float2* in, const float2* add, const dou
 }
 
 void Kron::launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
-                    const float2* add, const double* add_norm, float add_sigma, const double2* in_n2) const {
+                    const float2* add, const double* add_norm, float add_sigma, const double2* in_n2,
+                    const GsFinish* fin) const {
+  PP_REQUIRE(!fin || (fin->p3 && in_n2), "a pending finish needs the device slot of its |w|^2");
   // one launch only without aliasing: the workgroups of a channel all read the whole input
   // block while each writes its strip of out (and of the write-back)
   if (n <= 64 && split32 && fused32 && in != out && (!wb || (wb != out && wb != in && !add))) {
     const dim3 grid(nchan, (n + 15) / 16);
     hipLaunchKernelGGL(kron_fused_kernel, grid, dim3(1024), kron_lds_fused(), s, E32.ptr, lam32.ptr, l1.ptr, l2.ptr,
-                       n, make_float2((float)a.x, (float)a.y), in, in_scale, add, add_norm, add_sigma, in_n2, out, wb);
+                       n, make_float2((float)a.x, (float)a.y), in, in_scale, add, add_norm, add_sigma, in_n2, out, wb,
+                       fin ? *fin : GsFinish{});
     PP_HIP(hipGetLastError());
     return;
+  }
+  if (fin) {  // the other forms read the finished |w|^2 from in_n2
+    hipLaunchKernelGGL(finish_cgs1_kernel, dim3(1), dim3(256), 0, s, fin->blocks, fin->p1, fin->m, fin->p3, fin->out,
+                       fin->out2);
+    PP_HIP(hipGetLastError());
   }
   if (n <= 64 && split32 && (!wb || (wb != out && !add))) {
     if (g32.count < (size_t)nchan * 64 * 64) g32.resize((size_t)nchan * 64 * 64);
@@ -1392,7 +1413,7 @@ void Kron::launch32(double2 a, const float2* in, float2* out, hipStream_t s, flo
     hipLaunchKernelGGL(kron_split1_kernel, grid, dim3(1024), kron_lds_split(), s, E32.ptr, lam32.ptr, l1.ptr,
                        l2.ptr, n, af, in, in_scale, add, add_norm, add_sigma, in_n2, g32.ptr);
     hipLaunchKernelGGL(kron_split2_kernel, grid, dim3(1024), kron_lds_split(), s, E32.ptr, l1.ptr, l2.ptr, n,
-                       g32.ptr, out, wb, in_scale, in_n2);
+                       g32.ptr, in, out, wb, in_scale, in_n2);
     PP_HIP(hipGetLastError());
     return;
   }
@@ -1518,7 +1539,8 @@ void MixedWork::cgs2_32(int k, float2* w_, double2* out, double2* out_dev, hipSt
 // One-pass classical Gram-Schmidt against V_0..V_{k-1} in three launches: the dots (+|w|^2),
 // the update with the norm of the updated vector, and a one-block finish (results in the
 // two-pass layout, see finish_cgs1_kernel)
-void MixedWork::cgs1_32(int k, float2* w_, double2* out, double2* out_dev, hipStream_t s) {
+void MixedWork::cgs1_32(int k, float2* w_, double2* out, double2* out_dev, hipStream_t s, bool defer) {
+  PP_REQUIRE(!pend.p3, "a Gram-Schmidt finish is still pending");
   const int blocks = fused_blocks(n);
   const int width = k + 1;
   PP_REQUIRE(2 * width + 1 <= 128, "Gram-Schmidt basis too large");
@@ -1528,12 +1550,29 @@ void MixedWork::cgs1_32(int k, float2* w_, double2* out, double2* out_dev, hipSt
   const dim3 g(blocks), b(256);
   hipLaunchKernelGGL((cgs32_kernel<false, true>), g, b, 0, s, n, k, V.ptr, n, nullptr, 0, w_, p1);
   hipLaunchKernelGGL((cgs32_kernel<true, false>), g, b, 0, s, n, k, V.ptr, n, p1, width, w_, p3, wnext.ptr);
-  hipLaunchKernelGGL(finish_cgs1_kernel, dim3(1), b, 0, s, blocks, p1, k, p3, out, out_dev);
+  if (defer) {
+    pend.p1 = p1;
+    pend.p3 = p3;
+    pend.blocks = blocks;
+    pend.m = k;
+    pend.out = out;
+    pend.out2 = out_dev;
+  } else {
+    hipLaunchKernelGGL(finish_cgs1_kernel, dim3(1), b, 0, s, blocks, p1, k, p3, out, out_dev);
+  }
   PP_HIP(hipGetLastError());
 }
 
-void MixedWork::gram_schmidt(int k, float2* w_, double2* out, double2* out_dev, hipStream_t s) {
-  if (gs_passes == 1) cgs1_32(k, w_, out, out_dev, s);
+void MixedWork::flush_finish(hipStream_t s) {
+  if (!pend.p3) return;
+  hipLaunchKernelGGL(finish_cgs1_kernel, dim3(1), dim3(256), 0, s, pend.blocks, pend.p1, pend.m, pend.p3, pend.out,
+                     pend.out2);
+  PP_HIP(hipGetLastError());
+  pend = GsFinish{};
+}
+
+void MixedWork::gram_schmidt(int k, float2* w_, double2* out, double2* out_dev, hipStream_t s, bool defer) {
+  if (gs_passes == 1) cgs1_32(k, w_, out, out_dev, s, defer && defer_finish);
   else cgs2_32(k, w_, out, out_dev, s);
 }
 
@@ -1604,8 +1643,9 @@ int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply
     apply_P(j, j == 0 ? vj : wnext.ptr, n2, zj, j == 0 ? nullptr : vj);
     apply_A(zj, vn);
     // classical Gram-Schmidt: [first dots, |w|^2 | second dots, |w|^2 | final |w|^2]; the
-    // updated vector goes to wnext
-    gram_schmidt(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s);
+    // updated vector goes to wnext.  Before the host reads, the next preconditioner launch
+    // finishes it
+    gram_schmidt(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s, j + 1 < pre);
     if (j + 1 < pre) continue;
     PP_HIP(hipStreamSynchronize(s));  // the finishing kernels wrote straight into small
     for (int jj = processed; jj <= j && !done; ++jj) {
@@ -1645,6 +1685,7 @@ int MixedWork::fgmres32(const std::function<void(const float2*, float2*)>& apply
     if (first_conv) done = true;
     pre = 0;
   }
+  flush_finish(s);
   if (needed) *needed = first_conv ? first_conv : k;
   LincombCoef yc;
   for (int i = 0; i < k; ++i) yc.c[i] = make_float2((float)y[i].real(), (float)y[i].imag());
@@ -1673,8 +1714,10 @@ void MixedWork::fgmres32_device(const std::function<void(const float2*, float2*)
     const double2* n2 = j == 0 ? nullptr : hdev.ptr + (size_t)(j - 1) * stride + 2 * (j + 1);
     apply_P(j, j == 0 ? vj : wnext.ptr, n2, zj, j == 0 ? nullptr : vj);
     apply_A(zj, vn);
-    gram_schmidt(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s);
+    // all but the last finished by the next preconditioner launch
+    gram_schmidt(j + 1, vn, small_dev + (size_t)j * stride, hdev.ptr + (size_t)j * stride, s, j + 1 < k);
   }
+  flush_finish(s);
   hipLaunchKernelGGL(hessenberg_lsq_kernel, dim3(1), dim3(64), 0, s, k, stride, hdev.ptr, beta2_dev, ycoef.ptr,
                      small_dev + est);
   hipLaunchKernelGGL(lincomb_add32_dev_kernel, dim3(grid_n(n)), dim3(256), 0, s, n, k, Z.ptr, n, ycoef.ptr, base,
@@ -1698,11 +1741,12 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   const double2 ka = make_double2(0.0, a);
   // with an overlap term in the factored operator, the FP64 S + c H is one multiply
   const bool fused_s = fham && fham->has_overlap;
-  // The step's time scalings also go to device memory (through pinned host memory, in stream
-  // order): the sparse angular form reads them there, so that a captured step replays with
-  // each step's values.
+  // In graph mode the step's time scalings also go to device memory (through pinned host
+  // memory, in stream order): the sparse angular form reads them there, so that a captured
+  // step replays with each step's values.  Eager steps pass them as kernel arguments (the
+  // copy is a launch of its own).
   const double* scd = nullptr;
-  if (fham && fham->sparse_angular()) {
+  if (fham && fham->sparse_angular() && graph_mode) {
     PP_REQUIRE(scal.size() <= (size_t)kMaxFTerms, "too many potentials");
     if (!scal_host) PP_HIP(hipHostMalloc((void**)&scal_host, kMaxFTerms * sizeof(double), hipHostMallocDefault));
     std::copy(scal.begin(), scal.end(), scal_host);
@@ -1768,6 +1812,7 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     // direction of every correction; 2: every direction
     const bool use = learned && (learned_mode == 2 || (j == 0 && (learned_mode == 1 || correction == 0)));
     if (use) {
+      work.flush_finish(cs);  // the learned path reads the finished |w|^2
       const float2* v = in;
       if (n2) {  // V_j is normalised here (the Kronecker path does it while loading)
         hipLaunchKernelGGL(unit32_to_kernel, dim3(grid_n(n)), dim3(256), 0, cs, n, n2, in, wb);
@@ -1777,7 +1822,9 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
       learned_apply(*learned, K, ka, v, out, cs);
       ++stats.learned_applies;
     } else {
-      K.apply32(ka, in, out, cs, 1.f, n2 ? wb : nullptr, n2);
+      // a finish left by the last Gram-Schmidt is done by this launch
+      K.apply32(ka, in, out, cs, 1.f, n2 ? wb : nullptr, n2, work.pend.p3 ? &work.pend : nullptr);
+      work.pend = GsFinish{};
     }
     ++stats.precond_applies;
   };
@@ -2127,6 +2174,13 @@ extern "C" int pp_mixed_set_gram_schmidt(pp_mixed* m, int passes) {
   return guarded([&] {
     PP_REQUIRE(m && (passes == 1 || passes == 2), "Gram-Schmidt passes must be 1 or 2");
     m->c.work.gs_passes = passes;
+  });
+}
+
+extern "C" int pp_mixed_set_deferred_finish(pp_mixed* m, int on) {
+  return guarded([&] {
+    PP_REQUIRE(m && (on == 0 || on == 1), "invalid argument");
+    m->c.work.defer_finish = on != 0;
   });
 }
 

@@ -14,6 +14,17 @@
 
 namespace pp {
 
+// A one-pass Gram-Schmidt finish left to the next preconditioner launch: the block partials
+// of the dots (p1, rows of m + 1) and of the updated |w|^2 (p3, one per block), and where the
+// finished values go (the layout of finish_cgs1_kernel)
+struct GsFinish {
+  const double2* p1 = nullptr;
+  const double2* p3 = nullptr;
+  int blocks = 0, m = 0;
+  double2* out = nullptr;
+  double2* out2 = nullptr;
+};
+
 // Exact inverse of the separable part of every channel block of S (x) S + a H0:
 //   K_c = S x S + a (h_l1 x S + S x h_l2),   h_l E_l = S E_l diag(lam_l),  E_l^T S E_l = I.
 struct Kron {
@@ -30,14 +41,17 @@ struct Kron {
   void apply64(double2 a, double2* psi, hipStream_t s) const;
   void apply32(double2 a, float2* psi, hipStream_t s) const;
   // out <- K^-1 (in_scale in) (may alias in); wb (may be NULL, not out): wb <- in_scale in
-  // as well; in_n2 (device, may be NULL): in_scale = 1 / sqrt(in_n2->x)
+  // as well; in_n2 (device, may be NULL): in_scale = 1 / sqrt(in_n2->x).  fin (may be NULL):
+  // a pending Gram-Schmidt finish whose |w|^2 is *in_n2: the launch performs it (the one-launch
+  // form sums the partials itself; the others run the finishing kernel first)
   void apply32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
-               const double2* in_n2) const;
+               const double2* in_n2, const GsFinish* fin = nullptr) const;
   // out <- K^-1 (in + add_sigma * (*add_norm) * add), *add_norm read on the device
   void apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
                    float2* out, hipStream_t s) const;
   void launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
-                const float2* add, const double* add_norm, float add_sigma, const double2* in_n2) const;
+                const float2* add, const double* add_norm, float add_sigma, const double2* in_n2,
+                const GsFinish* fin = nullptr) const;
   // psi <- K psi (FP64)
   void multiply64(double2 a, double2* psi, hipStream_t s) const;
 };
@@ -81,10 +95,16 @@ struct MixedWork {
   // Two-pass classical Gram-Schmidt of w against V_0..V_{k-1} (FP64 sums): out[0..k] = first
   // dots and |w|^2, out[k+1..2k+1] = second dots and |w|^2, out[2k+2] = final |w|^2
   void cgs2_32(int k, float2* w, double2* out, double2* out_dev, hipStream_t s);
-  // one pass: dots, then the update with the norm of the updated vector
-  void cgs1_32(int k, float2* w, double2* out, double2* out_dev, hipStream_t s);
-  void gram_schmidt(int k, float2* w, double2* out, double2* out_dev, hipStream_t s);
+  // one pass: dots, then the update with the norm of the updated vector; with defer, the
+  // finish is left in pend for the next preconditioner launch
+  void cgs1_32(int k, float2* w, double2* out, double2* out_dev, hipStream_t s, bool defer = false);
+  // defer: the caller issues the next iteration's preconditioner before anything reads the
+  // results (one-pass form only)
+  void gram_schmidt(int k, float2* w, double2* out, double2* out_dev, hipStream_t s, bool defer = false);
   int gs_passes = 1;  // classical Gram-Schmidt passes per inner iteration (1 or 2)
+  bool defer_finish = true;  // leave one-pass finishes to the next preconditioner launch
+  GsFinish pend;             // the finish left by the last Gram-Schmidt (p3 NULL: none)
+  void flush_finish(hipStream_t s);  // run a pending finish now
   void norm2_32(const float2* w, double2* out, hipStream_t s);  // out[0] = |w|^2
   // r = b - r, out[0] = |r|^2 (FP64; pinned out and device out_dev)
   void residual(const double2* b, double2* r, double2* out, double2* out_dev, hipStream_t s);

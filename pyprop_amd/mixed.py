@@ -335,7 +335,7 @@ class MixedCayleyPropagator:
                  fallback=None, fallback_iterations: int = 40, krylov_basis_size: int = 20,
                  tolerance: float = 1e-10, inner_tolerance: float = 1e-5, max_outer: int = 8, factored=None,
                  device_mode: bool = False, learned_mode: int = 0, gram_schmidt_passes: int = 1,
-                 graph: bool = False):
+                 graph: bool = False, deferred_finish: bool = True):
         """device_mode: after the first converged step, solve the inner least-squares problems
         on the device and synchronise once per step (pp_mixed_set_device_mode).
         learned_mode: the FGMRES directions the learned correction acts on: 0 the first of
@@ -344,12 +344,16 @@ class MixedCayleyPropagator:
         gram_schmidt_passes: classical Gram-Schmidt passes per inner iteration (1 saves one
         launch per iteration; pp_mixed_set_gram_schmidt).
         graph: with device_mode, replay each device-mode step from a captured hipGraph
-        (pp_mixed_set_graph; needs a factored operator whose angular forms are sparse)."""
+        (pp_mixed_set_graph; needs a factored operator whose angular forms are sparse).
+        deferred_finish: let the next Kronecker solve finish a one-pass Gram-Schmidt that the
+        host does not read first (one launch fewer; the same bits;
+        pp_mixed_set_deferred_finish)."""
         if graph and not device_mode:
             raise ValueError("graph mode needs device_mode")
         self.graph = graph
         self.learned_mode = learned_mode
         self.gram_schmidt_passes = gram_schmidt_passes
+        self.deferred_finish = deferred_finish
         self.psi = psi
         self.device_mode = device_mode
         self.factored = factored
@@ -385,6 +389,7 @@ class MixedCayleyPropagator:
         N.check(N.load().pp_mixed_set_device_mode(h, int(self.device_mode)))
         N.check(N.load().pp_mixed_set_learned_mode(h, int(self.learned_mode)))
         N.check(N.load().pp_mixed_set_gram_schmidt(h, int(self.gram_schmidt_passes)))
+        N.check(N.load().pp_mixed_set_deferred_finish(h, int(self.deferred_finish)))
         N.check(N.load().pp_mixed_set_graph(h, int(self.graph)))
 
     def SetLearned(self, learned):

@@ -636,6 +636,48 @@ def test_mixed_graph_mode_equals_eager_device_mode(torch, rng):
     assert se["inner"] == sg["inner"] and se["fp32_ops"] == sg["fp32_ops"]
 
 
+def test_mixed_deferred_finish_bitwise(torch, rng):
+    """The Gram-Schmidt finish folded into the next Kronecker launch (the default) against a
+    finishing launch of its own after every iteration: bitwise the same states, step after
+    step, with every FP32 Kronecker form (the forms other than the one-launch form run the
+    finish before their launch), host-driven and in device mode."""
+    from pyprop_amd.factored import FactoredHamiltonian
+    from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
+    say("mixed step: deferred Gram-Schmidt finish vs a finishing launch per iteration (small helium)")
+    sysm = _small_helium()
+    repr_, psi, base = _device(sysm, torch)
+    base.SetupStep()
+    fh = FactoredHamiltonian(base)
+    sparse = fh.ANGULAR_VARIANTS.index("sparse2")
+    fh.set_angular_variant(64, sparse)
+    fh.set_angular_variant(32, sparse)
+    x0 = torch.from_numpy(crandn(rng, sysm.shape)).to("cuda")
+    kp = KroneckerPreconditioner(psi, sysm)
+    dt, t0, steps = 0.01, 120.0, 6
+    for form in ("fused", "split", "one-workgroup"):
+        kp.SetForm(KroneckerPreconditioner.FORMS.index(form))
+        for device_mode in (False, True):
+            traj = {}
+            for deferred in (True, False):
+                psi.data.copy_(x0)
+                prop = MixedCayleyPropagator(psi, base, kp, tolerance=1e-10, factored=fh, device_mode=device_mode,
+                                             deferred_finish=deferred)
+                t = t0
+                states = []
+                for _ in range(steps):
+                    prop.AdvanceStep(t, dt)
+                    t += dt
+                    states.append(psi.data.clone())
+                traj[deferred] = (states, prop.GetStats())
+                del prop
+            (a, sa), (b, sb) = traj[True], traj[False]
+            say("  %s, device mode %d: %s" % (form, device_mode, sa))
+            assert sa["inner"] == sb["inner"] and sa["outer"] == sb["outer"]
+            for k in range(steps):
+                assert torch.equal(a[k], b[k]), (form, device_mode, k)
+    kp.SetForm(KroneckerPreconditioner.FORMS.index("fused"))
+
+
 def test_factored_radial_variants_vs_dense(torch, rng):
     """Every radial kernel shape (terms x channel groups per wave) against the dense operator.
     The small system has 13 channels (2 groups of 8, the second partly empty; 3 groups per
