@@ -1271,12 +1271,13 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
                        dim3(256), 0, s, (long long)xneed, Xb.ptr);
   }
   // separable terms (and, in FP32, the Cholesky-form overlap) run first when their kernel
-  // can also write the channel-minor copy: no transpose launch (FP32 only: the FP64 path
-  // compares the input with the held one in the transpose kernel)
+  // can also write the channel-minor copy: no transpose launch.  Not with a reuse reference
+  // (FP64): the transpose kernel then compares the input with the held one, and the radial
+  // and separable stages are skipped when it is unchanged
   const int nsep_terms = F.sep_enabled ? nterms - F.nband : 0;
   const int nsep_run = nsep_terms + (with_s && !chol && F.sep_enabled ? 1 : 0);
   const int sep_jobs = nsep_run + (chol ? 1 : 0);
-  const bool fused_xt = sizeof(T) == 4 && F.sep_enabled && sep_jobs > 0 && !ref;
+  const bool fused_xt = F.sep_enabled && sep_jobs > 0 && !ref;
   auto launch_separable = [&](T* xt) {
     if (sep_jobs <= 0) return;
     typedef typename MfmaT<T>::C2 C2s;
