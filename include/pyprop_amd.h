@@ -359,6 +359,10 @@ int pp_kron_solve_f32(const pp_kron* k, pp_cplx a, pp_cplxf* psi, pp_stream stre
 /* out <- K^-1 in in FP32 (no reference counterpart; in and out distinct, so the one-launch
  * form can run). */
 int pp_kron_solve_f32_into(const pp_kron* k, pp_cplx a, const pp_cplxf* in, pp_cplxf* out, pp_stream stream);
+/* The same, with sz <- (S (x) S) out formed from the solve's middle product G as F1 G F2^T
+ * (no reference counterpart; the one-launch form only, which this requires). */
+int pp_kron_solve_f32_overlap(const pp_kron* k, pp_cplx a, const pp_cplxf* in, pp_cplxf* out, pp_cplxf* sz,
+                              pp_stream stream);
 /* The FP32 solve's form for n <= 64 (no reference counterpart): 0 one workgroup per channel,
  * 1 four workgroups per channel in two launches, 2 (the default) four per channel in one launch
  * (each forms the whole middle product; used when input, output and write-back are distinct,
@@ -573,6 +577,12 @@ int pp_mixed_set_gram_schmidt(pp_mixed* m, int passes);
  * partials itself; one launch fewer per such iteration, bitwise the same results); 0 by a
  * finishing launch of its own after every iteration. */
 int pp_mixed_set_deferred_finish(pp_mixed* m, int on);
+/* The FP32 overlap product of each preconditioner output z (no reference counterpart): 1 (the
+ * default) formed by the Kronecker solve that produced z, as F1 G F2^T (F_l = S E_l, G the
+ * solve's middle product; E^T S E = I makes it S1 z S2), so the FP32 multiply drops its
+ * Cholesky-form overlap passes; 0 by those passes, U1^T U1 z U2^T U2.  Only with the one-launch
+ * Kronecker form and the factored operator's Cholesky factors attached. */
+int pp_mixed_set_overlap_from_solve(pp_mixed* m, int on);
 /* Graph mode (no reference counterpart; with device mode 1): a device-mode step is one fixed
  * sequence of launches for given correction and inner-iteration counts; it is captured into
  * a hipGraph once, after the same sequence has run eagerly, and replayed on the caller's

@@ -89,6 +89,7 @@ SIGNATURES = [
     ("pp_kron_solve", I, [P, PPCplx, P, P]),
     ("pp_kron_solve_f32", I, [P, PPCplx, P, P]),
     ("pp_kron_solve_f32_into", I, [P, PPCplx, P, P, P]),
+    ("pp_kron_solve_f32_overlap", I, [P, PPCplx, P, P, P, P]),
     ("pp_kron_set_form", I, [P, I]),
     ("pp_cayley_set_kron", I, [P, P]),
     ("pp_learned_create", I, [I, I, I, C.c_ulonglong, P]),
@@ -115,6 +116,7 @@ SIGNATURES = [
     ("pp_mixed_set_device_mode", I, [P, I]),
     ("pp_mixed_set_gram_schmidt", I, [P, I]),
     ("pp_mixed_set_deferred_finish", I, [P, I]),
+    ("pp_mixed_set_overlap_from_solve", I, [P, I]),
     ("pp_mixed_set_graph", I, [P, I]),
     ("pp_mixed_graph_stats", I, [P, P, P]),
     ("pp_cayley_set_factored", I, [P, P]),

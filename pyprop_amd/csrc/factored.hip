@@ -1249,16 +1249,19 @@ template <class T>
 static void fham_run(const FHam& F, const double* scalings, const typename MfmaT<T>::C2* src, const FOut<T>& o1,
                      const FOut<T>* o2, const DeviceBuffer<T>& Ab, const DeviceBuffer<T>& Db,
                      const DeviceBuffer<T>& RPb, const DeviceBuffer<T>& SAb, const DeviceBuffer<T>& SBb, DeviceBuffer<T>& Zb,
-                     DeviceBuffer<T>& Xb, const T* ref, int* differs, hipStream_t s, const double* sc_dev) {
+                     DeviceBuffer<T>& Xb, const T* ref, int* differs, hipStream_t s, const double* sc_dev,
+                     const typename MfmaT<T>::C2* zs_ext = nullptr) {
   const int nc = F.nc, n = F.n, nterms = F.nterms;
   const long long P = (long long)n * n;
   auto nz = [](double2 v) { return v.x != 0.0 || v.y != 0.0; };
   const bool with_s = nz(o1.beta) || (o2 && nz(o2->beta));
+  // the overlap product is formed here unless zs_ext brings it
+  const bool own_s = with_s && !zs_ext;
   // FP32 with the Cholesky factors: the overlap product by the four triangular passes
-  const bool chol = sizeof(T) == 4 && with_s && F.csd == F.bw && F.sep_enabled;
-  PP_REQUIRE(!with_s || F.has_overlap || chol, "overlap coefficient given but the operator has no overlap term");
+  const bool chol = sizeof(T) == 4 && own_s && F.csd == F.bw && F.sep_enabled;
+  PP_REQUIRE(!own_s || F.has_overlap || chol, "overlap coefficient given but the operator has no overlap term");
   PP_REQUIRE(!(o2 && o2->addend), "the second output takes no addend");
-  const int nrad = nterms + (with_s ? 1 : 0);  // radial terms formed (the overlap term is last)
+  const int nrad = nterms + (own_s ? 1 : 0);  // radial terms formed (the overlap term is last)
   const size_t zneed = (size_t)(nterms + ((F.has_overlap || chol) ? 1 : 0)) * nc * P * 2;
   if (Zb.count < zneed) Zb.resize(zneed);
   const int ngroups = (nc + 7) / 8;
@@ -1275,7 +1278,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   // (FP64): the transpose kernel then compares the input with the held one, and the radial
   // and separable stages are skipped when it is unchanged
   const int nsep_terms = F.sep_enabled ? nterms - F.nband : 0;
-  const int nsep_run = nsep_terms + (with_s && !chol && F.sep_enabled ? 1 : 0);
+  const int nsep_run = nsep_terms + (own_s && !chol && F.sep_enabled ? 1 : 0);
   const int sep_jobs = nsep_run + (chol ? 1 : 0);
   const bool fused_xt = F.sep_enabled && sep_jobs > 0 && !ref;
   auto launch_separable = [&](T* xt) {
@@ -1329,6 +1332,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   typedef typename MfmaT<T>::C2 C2;
   const C2* Zc = reinterpret_cast<const C2*>(Zb.ptr);
   const dim3 agrid((unsigned)((ablocks + 7) / 8 * 8));
+  const C2* zsp = zs_ext ? zs_ext : with_s ? Zc + (size_t)nterms * nc * P : nullptr;  // the overlap product
   const int avar = sizeof(T) == 8 ? F.ang_variant64 : F.ang_variant32;
   const int waves = kAngWaveChoices[avar];
   if (kAngSparsePV[avar] > 0) {
@@ -1342,7 +1346,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   hipLaunchKernelGGL((fham_angular_sparse_kernel<T, V>), sgrid, dim3(64 * kAngSparseWaves), 0, s, nc, P, nchunks,   \
                      F.sp_rowptr.ptr, F.sp_k.ptr, coef, F.sp_term.ptr, Zc, sc, sc_dev, F.sp_pot.ptr, o1.addend,     \
                      to_c2<T>(o1.alpha),                                                                             \
-                     with_s ? Zc + (size_t)nterms * nc * P : nullptr, to_c2<T>(o1.beta), o1.dst,                     \
+                     zsp, to_c2<T>(o1.beta), o1.dst,                     \
                      o2 ? o2->dst : nullptr, to_c2<T>(o2 ? o2->alpha : make_double2(0, 0)),                          \
                      to_c2<T>(o2 ? o2->beta : make_double2(0, 0)), differs)
     if (PV == 1) PP_ANGS(1);
@@ -1361,7 +1365,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                           \
     hipLaunchKernelGGL((fham_angular_rows_kernel<T, W, MR>), rgrid, dim3(64 * W), lds, s, nc, F.Kp, nterms * nc, \
                        P, Ab.ptr, F.tk.ptr, Zc, sc, o1.addend, to_c2<T>(o1.alpha),                               \
-                       with_s ? Zc + (size_t)nterms * nc * P : nullptr, to_c2<T>(o1.beta), o1.dst,              \
+                       zsp, to_c2<T>(o1.beta), o1.dst,              \
                        o2 ? o2->dst : nullptr, to_c2<T>(o2 ? o2->alpha : make_double2(0, 0)),                    \
                        to_c2<T>(o2 ? o2->beta : make_double2(0, 0)), differs);                                   \
   }
@@ -1373,7 +1377,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   }
 #define PP_ANG(W)                                                                                                    \
   hipLaunchKernelGGL((fham_angular_kernel<T, W>), agrid, dim3(64 * W), 0, s, nc, nrb, F.Kp, nterms * nc, P, Ab.ptr, \
-                     F.tk.ptr, Zc, sc, o1.addend, to_c2<T>(o1.alpha), with_s ? Zc + (size_t)nterms * nc * P : nullptr,       \
+                     F.tk.ptr, Zc, sc, o1.addend, to_c2<T>(o1.alpha), zsp,       \
                      to_c2<T>(o1.beta), o1.dst, o2 ? o2->dst : nullptr,                                              \
                      to_c2<T>(o2 ? o2->alpha : make_double2(0, 0)), to_c2<T>(o2 ? o2->beta : make_double2(0, 0)),   \
                      differs)
@@ -1437,14 +1441,14 @@ void FHam::apply64_ex(const double* scalings, const double2* src, const FHamOut&
 }
 
 void FHam::apply32(const double* scalings, const float2* src, float2* dst, const float2* addend, double2 alpha,
-                   hipStream_t s, double2 beta, const double* sc_dev) const {
+                   hipStream_t s, double2 beta, const double* sc_dev, const float2* zs_ext) const {
   FOut<float> o;
   o.dst = dst;
   o.addend = addend;
   o.alpha = alpha;
   o.beta = beta;
   fham_run<float>(*this, scalings, src, o, nullptr, A32, bt32, rp32, sa32, sb32, W32, X32, nullptr, nullptr, s,
-                  sc_dev);
+                  sc_dev, zs_ext);
 }
 
 bool FHam::sparse_angular() const { return kAngSparsePV[ang_variant64] > 0 && kAngSparsePV[ang_variant32] > 0; }

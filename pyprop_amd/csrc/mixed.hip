@@ -1202,14 +1202,17 @@ __global__ __launch_bounds__(1024) void kron_split2_kernel(const float* __restri
 // Gram-Schmidt finish (fin.p3), every workgroup sums the |w|^2 block partials itself (the
 // sum finish_cgs1_kernel forms, the same bits) for the scale, and workgroup (0, 0) also sums
 // the dots and writes the finished results in that kernel's layout: the finish takes no
-// launch of its own.
+// launch of its own.  With sz (and F, F_l = S E_l), waves 4-7 form the strip of the overlap
+// product S1 X S2 = F1 G F2^T beside X (E^T S E = I), from the same G: V' = F1[strip, :] G,
+// then V' F2^T.
 __global__ __launch_bounds__(1024) void kron_fused_kernel(const float* __restrict__ E, const float* __restrict__ lam,
                                                           const int* __restrict__ l1, const int* __restrict__ l2,
                                                           int n, float2 a, const float2* in, float in_scale,
                                                           const float2* __restrict__ add,
                                                           const double* __restrict__ add_norm, float add_sigma,
                                                           const double2* __restrict__ in_n2, float2* out,
-                                                          float2* in_wb, GsFinish fin) {
+                                                          float2* in_wb, GsFinish fin, const float* __restrict__ F,
+                                                          float2* sz) {
   __shared__ double2 fres[130];
   if (!fin.p3 && in_n2) {
     const double q = in_n2->x;
@@ -1224,15 +1227,22 @@ __global__ __launch_bounds__(1024) void kron_fused_kernel(const float* __restric
   float* Uim = Ure + 64 * KLD;
   float* Vre = Uim + 64 * KLD;    // V strip (16 x KLD)
   float* Vim = Vre + 16 * KLD;
+  float* f1 = Vim + 16 * KLD;     // with sz: rows i0 .. i0+15 of F1 (16 x KLD), F2 (64 x KLD), V' strip
+  float* f2 = f1 + 16 * KLD;
+  float* Pre = f2 + 64 * KLD;
+  float* Pim = Pre + 16 * KLD;
   const int c = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
   const int i0 = t * 16;
   const int n2 = n * n;
   const float* E1 = E + (long long)l1[c] * n2;
   const float* E2 = E + (long long)l2[c] * n2;
   const float2* xin = in + (long long)c * n2;
-  // every load of a thread is issued before the LDS stores: 4 each of B, E1, E2 (and the correction)
+  // every load of a thread is issued before the LDS stores: 4 each of B, E1, E2 (and the
+  // correction; with sz 4 of F2 and 1 of the F1 strip)
   float2 bv[4], yv[4];
-  float ev1[4], ev2[4];
+  float ev1[4], ev2[4], fv2[4], fv1 = 0.f;
+  const float* F1 = sz ? F + (long long)l1[c] * n2 : nullptr;
+  const float* F2 = sz ? F + (long long)l2[c] * n2 : nullptr;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int idx = tid + q * 1024, i = idx >> 6, j = idx & 63;
@@ -1241,7 +1251,10 @@ __global__ __launch_bounds__(1024) void kron_fused_kernel(const float* __restric
     ev1[q] = ok ? E1[i * n + j] : 0.f;
     ev2[q] = ok ? E2[i * n + j] : 0.f;
     yv[q] = (add && ok) ? add[(long long)c * n2 + i * n + j] : make_float2(0.f, 0.f);
+    fv2[q] = (sz && ok) ? F2[i * n + j] : 0.f;
   }
+  const int fi = i0 + (tid >> 6), fk = tid & 63;  // this thread's element of the F1 strip
+  if (sz && fi < n && fk < n) fv1 = F1[fi * n + fk];
   if (fin.p3) {  // after the loads above are issued, so that their latencies overlap
     const bool first = blockIdx.x == 0 && blockIdx.y == 0;
     if (first) sum_partials(fin.p1, fin.blocks, fin.m + 1, fin.m + 1, fres);
@@ -1273,9 +1286,11 @@ __global__ __launch_bounds__(1024) void kron_fused_kernel(const float* __restric
     Bim[i * KLD + j] = v.y;
     e1[i * KLD + j] = ev1[q];
     e2[i * KLD + j] = ev2[q];
+    if (sz) f2[i * KLD + j] = fv2[q];
     // the scaled input in place of in, rows of this strip (in does not alias out here)
     if (xb && i >= i0 && i < i0 + 16 && i < n && j < n) xb[i * n + j] = sv;
   }
+  if (sz) f1[(tid >> 6) * KLD + fk] = fv1;
   __syncthreads();
   const int lane = tid & 63, wave = tid >> 6;
   const int si = wave >> 2, ct = wave & 3;  // this wave's tile of a 64 x 64 product
@@ -1318,21 +1333,45 @@ __global__ __launch_bounds__(1024) void kron_fused_kernel(const float* __restric
       Vre[i * KLD + j] = cre[r];
       Vim[i * KLD + j] = cim[r];
     }
+  } else if (sz && wave < 8) {
+    cre = kf32x4{0.f, 0.f, 0.f, 0.f};
+    cim = kf32x4{0.f, 0.f, 0.f, 0.f};
+    kron_strip<false, false>(f1, 0, Bre, Bim, true, nullptr, (wave - 4) * 16, cre, cim);  // V' = F1[strip, :] G
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = (lane >> 4) * 4 + r, j = (wave - 4) * 16 + (lane & 15);
+      Pre[i * KLD + j] = cre[r];
+      Pim[i * KLD + j] = cim[r];
+    }
   }
   __syncthreads();
-  if (wave >= 4) return;
-  cre = kf32x4{0.f, 0.f, 0.f, 0.f};
-  cim = kf32x4{0.f, 0.f, 0.f, 0.f};
-  kron_strip<false, true>(Vre, 0, e2, nullptr, false, Vim, wave * 16, cre, cim);  // X strip = V E2^T
-  float2* x = out + (long long)c * n2;
+  if (wave < 4) {
+    cre = kf32x4{0.f, 0.f, 0.f, 0.f};
+    cim = kf32x4{0.f, 0.f, 0.f, 0.f};
+    kron_strip<false, true>(Vre, 0, e2, nullptr, false, Vim, wave * 16, cre, cim);  // X strip = V E2^T
+    float2* x = out + (long long)c * n2;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int i = i0 + (lane >> 4) * 4 + r, j = wave * 16 + (lane & 15);
-    if (i < n && j < n) x[i * n + j] = make_float2(cre[r], cim[r]);
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + (lane >> 4) * 4 + r, j = wave * 16 + (lane & 15);
+      if (i < n && j < n) x[i * n + j] = make_float2(cre[r], cim[r]);
+    }
+  } else if (sz && wave < 8) {
+    cre = kf32x4{0.f, 0.f, 0.f, 0.f};
+    cim = kf32x4{0.f, 0.f, 0.f, 0.f};
+    kron_strip<false, true>(Pre, 0, f2, nullptr, false, Pim, (wave - 4) * 16, cre, cim);  // strip of V' F2^T
+    float2* y = sz + (long long)c * n2;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + (lane >> 4) * 4 + r, j = (wave - 4) * 16 + (lane & 15);
+      if (i < n && j < n) y[i * n + j] = make_float2(cre[r], cim[r]);
+    }
   }
 }
 
-static size_t kron_lds_fused() { return (6 * 64 + 2 * 16) * (size_t)KLD * sizeof(float); }
+// the one-launch form's LDS; with the overlap product, the F1 strip, F2 and V' as well
+static size_t kron_lds_fused(bool with_sz = true) {
+  return ((6 * 64 + 2 * 16) + (with_sz ? 16 + 64 + 2 * 16 : 0)) * (size_t)KLD * sizeof(float);
+}
 
 static size_t kron_lds_split() { return (4 * 64 + 2 * 16) * (size_t)KLD * sizeof(float); }
 
@@ -1378,8 +1417,12 @@ void Kron::multiply64(double2 a, double2* psi, hipStream_t s) const {
 void Kron::apply32(double2 a, float2* psi, hipStream_t s) const { apply32(a, psi, psi, s, 1.f, nullptr, nullptr); }
 
 void Kron::apply32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
-                   const double2* in_n2, const GsFinish* fin) const {
-  launch32(a, in, out, s, in_scale, wb, nullptr, nullptr, 0.f, in_n2, fin);
+                   const double2* in_n2, const GsFinish* fin, float2* sz) const {
+  launch32(a, in, out, s, in_scale, wb, nullptr, nullptr, 0.f, in_n2, fin, sz);
+}
+
+bool Kron::overlap_product32(const float2* in, const float2* out, const float2* wb) const {
+  return n <= 64 && split32 && fused32 && F32.count > 0 && in != out && (!wb || (wb != out && wb != in));
 }
 
 void Kron::apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
@@ -1389,15 +1432,16 @@ void Kron::apply32_add(double2 a, const float2* in, const float2* add, const dou
 
 void Kron::launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
                     const float2* add, const double* add_norm, float add_sigma, const double2* in_n2,
-                    const GsFinish* fin) const {
+                    const GsFinish* fin, float2* sz) const {
   PP_REQUIRE(!fin || (fin->p3 && in_n2), "a pending finish needs the device slot of its |w|^2");
+  PP_REQUIRE(!sz || (!add && overlap_product32(in, out, wb)), "the overlap product needs the one-launch form");
   // one launch only without aliasing: the workgroups of a channel all read the whole input
   // block while each writes its strip of out (and of the write-back)
   if (n <= 64 && split32 && fused32 && in != out && (!wb || (wb != out && wb != in && !add))) {
     const dim3 grid(nchan, (n + 15) / 16);
-    hipLaunchKernelGGL(kron_fused_kernel, grid, dim3(1024), kron_lds_fused(), s, E32.ptr, lam32.ptr, l1.ptr, l2.ptr,
-                       n, make_float2((float)a.x, (float)a.y), in, in_scale, add, add_norm, add_sigma, in_n2, out, wb,
-                       fin ? *fin : GsFinish{});
+    hipLaunchKernelGGL(kron_fused_kernel, grid, dim3(1024), kron_lds_fused(sz != nullptr), s, E32.ptr, lam32.ptr,
+                       l1.ptr, l2.ptr, n, make_float2((float)a.x, (float)a.y), in, in_scale, add, add_norm, add_sigma,
+                       in_n2, out, wb, fin ? *fin : GsFinish{}, F32.ptr, sz);
     PP_HIP(hipGetLastError());
     return;
   }
@@ -1478,6 +1522,8 @@ void MixedWork::ensure(long long n_, int m_) {
   w.resize(n);
   tmp.resize(n);
   wnext.resize(n);
+  sz.resize(n);
+  sz_for = nullptr;
   if (!small) {
     // results the host reads: kernels write them into mapped, coherent pinned memory, so a
     // read is one stream synchronisation and no copy
@@ -1793,8 +1839,12 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     // operator holds S (x) S: in FP32, S (x) S applied entry by entry loses its small
     // eigenvalues (relative error ~ eps32 cond(S (x) S)), and the refinement then cannot
     // reduce the error in those directions, where the residual barely sees it.
-    if (fham && fham->csd == fham->bw && fham->sep_enabled) {  // S in Cholesky form inside the multiply
-      fham->apply32(scal.data(), in, out, nullptr, make_double2(0.0, a), cs, make_double2(1.0, 0.0), scd);
+    if (fham && fham->csd == fham->bw && fham->sep_enabled) {
+      // S in Cholesky form inside the multiply, or (S (x) S) in = F1 G F2^T from the
+      // Kronecker solve that produced in (accurate in the same small-eigenvalue directions)
+      const float2* zs = work.sz_for == in ? work.sz.ptr : nullptr;
+      work.sz_for = nullptr;
+      fham->apply32(scal.data(), in, out, nullptr, make_double2(0.0, a), cs, make_double2(1.0, 0.0), scd, zs);
     } else {
       work.overlap_multiply32(R, in, work.tmp.ptr, work.w.ptr, cs);
       if (fham) fham->apply32(scal.data(), in, out, work.tmp.ptr, make_double2(0.0, a), cs, make_double2(0.0, 0.0),
@@ -1822,9 +1872,14 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
       learned_apply(*learned, K, ka, v, out, cs);
       ++stats.learned_applies;
     } else {
-      // a finish left by the last Gram-Schmidt is done by this launch
-      K.apply32(ka, in, out, cs, 1.f, n2 ? wb : nullptr, n2, work.pend.p3 ? &work.pend : nullptr);
+      // a finish left by the last Gram-Schmidt is done by this launch; with the Cholesky-form
+      // FP32 overlap, the solve also forms (S (x) S) out for the multiply that follows
+      float2* wb2 = n2 ? wb : nullptr;
+      const bool with_sz = overlap_from_solve && fham && fham->csd == fham->bw && fham->sep_enabled &&
+                           K.overlap_product32(in, out, wb2);
+      K.apply32(ka, in, out, cs, 1.f, wb2, n2, work.pend.p3 ? &work.pend : nullptr, with_sz ? work.sz.ptr : nullptr);
       work.pend = GsFinish{};
+      work.sz_for = with_sz ? out : nullptr;
     }
     ++stats.precond_applies;
   };
@@ -2059,6 +2114,8 @@ extern "C" int pp_kron_create(int nchan, int n, int nl, const double* S, const d
             for (int j = 0; j < n; ++j) F[((size_t)l * n + i) * n + j] += sv * E[((size_t)l * n + q) * n + j];
           }
       k.F64.upload(F.data(), ne, s);
+      std::vector<float> ff(F.begin(), F.end());
+      k.F32.upload(ff.data(), ne, s);
       std::vector<float> ef(E, E + ne), lf(lam, lam + nlam);
       k.E32.upload(ef.data(), ne, s);
       k.lam32.upload(lf.data(), nlam, s);
@@ -2104,6 +2161,18 @@ extern "C" int pp_kron_solve_f32_into(const pp_kron* k, pp_cplx a, const pp_cplx
     PP_REQUIRE(k && in && out, "null argument");
     k->k.apply32(make_double2(a.re, a.im), reinterpret_cast<const float2*>(in), reinterpret_cast<float2*>(out),
                  (hipStream_t)stream, 1.f, nullptr, nullptr);
+  });
+}
+
+extern "C" int pp_kron_solve_f32_overlap(const pp_kron* k, pp_cplx a, const pp_cplxf* in, pp_cplxf* out,
+                                         pp_cplxf* sz, pp_stream stream) {
+  return guarded([&] {
+    PP_REQUIRE(k && in && out && sz, "null argument");
+    const float2* i = reinterpret_cast<const float2*>(in);
+    float2* o = reinterpret_cast<float2*>(out);
+    PP_REQUIRE(k->k.overlap_product32(i, o, nullptr), "the overlap product needs the one-launch form (n <= 64)");
+    k->k.apply32(make_double2(a.re, a.im), i, o, (hipStream_t)stream, 1.f, nullptr, nullptr, nullptr,
+                 reinterpret_cast<float2*>(sz));
   });
 }
 
@@ -2174,6 +2243,13 @@ extern "C" int pp_mixed_set_gram_schmidt(pp_mixed* m, int passes) {
   return guarded([&] {
     PP_REQUIRE(m && (passes == 1 || passes == 2), "Gram-Schmidt passes must be 1 or 2");
     m->c.work.gs_passes = passes;
+  });
+}
+
+extern "C" int pp_mixed_set_overlap_from_solve(pp_mixed* m, int on) {
+  return guarded([&] {
+    PP_REQUIRE(m && (on == 0 || on == 1), "invalid argument");
+    m->c.overlap_from_solve = on != 0;
   });
 }
 

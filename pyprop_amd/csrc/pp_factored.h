@@ -69,8 +69,11 @@ struct FHam {
   // the host values as kernel arguments
   void apply64(const double* scalings, const double2* src, double2* dst, const double2* addend, double2 alpha,
                hipStream_t s, double2 beta = make_double2(0.0, 0.0), const double* sc_dev = nullptr) const;
+  // zs_ext (FP32, may be NULL): the overlap product (S (x) S) src already formed elsewhere (by
+  // the Kronecker solve that produced src); it then takes the place of the operator's own
   void apply32(const double* scalings, const float2* src, float2* dst, const float2* addend, double2 alpha,
-               hipStream_t s, double2 beta = make_double2(0.0, 0.0), const double* sc_dev = nullptr) const;
+               hipStream_t s, double2 beta = make_double2(0.0, 0.0), const double* sc_dev = nullptr,
+               const float2* zs_ext = nullptr) const;
   bool sparse_angular() const;  // both precisions use a sparse angular form (device scalings honoured)
   // FP64 multiply with up to two outputs from the same products (o2 may be NULL and takes no
   // addend).  reuse_ref (may be NULL): the input of an earlier FP64 multiply whose radial

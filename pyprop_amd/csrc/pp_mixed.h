@@ -32,6 +32,7 @@ struct Kron {
   DeviceBuffer<double> E64, lam64;   // [l][n][n] row-major, [l][n]
   DeviceBuffer<double> F64;          // F_l = S E_l (= E_l^-T), for the forward product
   DeviceBuffer<float> E32, lam32;
+  DeviceBuffer<float> F32;           // F_l in FP32: the overlap product S1 X S2 = F1 G F2^T of a solve
   DeviceBuffer<int> l1, l2;          // per channel
   mutable DeviceBuffer<float2> g32;  // [c][64][64] middle product of the split FP32 solve
   bool split32 = true;               // FP32 solve (n <= 64) with four workgroups per channel
@@ -44,14 +45,18 @@ struct Kron {
   // as well; in_n2 (device, may be NULL): in_scale = 1 / sqrt(in_n2->x).  fin (may be NULL):
   // a pending Gram-Schmidt finish whose |w|^2 is *in_n2: the launch performs it (the one-launch
   // form sums the partials itself; the others run the finishing kernel first)
+  // sz (may be NULL): the overlap product (S (x) S) out as well, from the solve's middle
+  // product (one-launch form only: see overlap_product32)
   void apply32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
-               const double2* in_n2, const GsFinish* fin = nullptr) const;
+               const double2* in_n2, const GsFinish* fin = nullptr, float2* sz = nullptr) const;
+  // whether apply32 with these buffers runs the one-launch form, which can form sz
+  bool overlap_product32(const float2* in, const float2* out, const float2* wb) const;
   // out <- K^-1 (in + add_sigma * (*add_norm) * add), *add_norm read on the device
   void apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
                    float2* out, hipStream_t s) const;
   void launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
                 const float2* add, const double* add_norm, float add_sigma, const double2* in_n2,
-                const GsFinish* fin = nullptr) const;
+                const GsFinish* fin = nullptr, float2* sz = nullptr) const;
   // psi <- K psi (FP64)
   void multiply64(double2 a, double2* psi, hipStream_t s) const;
 };
@@ -77,6 +82,8 @@ struct MixedWork {
   DeviceBuffer<float2> V, Z, w, tmp;
   DeviceBuffer<double2> b64, r64, x64, t64, h64;
   DeviceBuffer<float2> wnext;  // the Gram-Schmidt-updated direction of the last iteration
+  DeviceBuffer<float2> sz;     // (S (x) S) z of the last preconditioner output z, when its solve formed it
+  const float2* sz_for = nullptr;  // that z (NULL: sz holds nothing usable)
   double2* small = nullptr;      // kSmall results for the host (pinned, mapped, coherent)
   double2* small_dev = nullptr;  // its device address
   Reducer red;
@@ -147,6 +154,7 @@ struct MixedCayley {
   double inner_rtol = 1e-5;          // inner stop: relative to the current residual
   int max_outer = 8;
   int fallback_iterations = 40;      // inner iterations of one step beyond which the plain solver redoes it
+  bool overlap_from_solve = true;    // the FP32 S z of a preconditioner output comes from its Kronecker solve
   MixedWork work;
   MixedStats stats;
   // reuse of the last FP64 radial products across steps (factored operator only)

@@ -113,6 +113,13 @@ class KroneckerPreconditioner:
         N.check(N.load().pp_kron_solve_f32_into(self.h, N.cplx(self.HamiltonianScaling), D.ptr(src), D.ptr(dst),
                                                 D.stream_handle()))
 
+    def SolveIntoWithOverlap(self, src, dst, sz):
+        """dst <- K^-1 src and sz <- (S (x) S) dst, the latter formed in the same launch from the
+        solve's middle product (one-launch form)."""
+        self.Setup()
+        N.check(N.load().pp_kron_solve_f32_overlap(self.h, N.cplx(self.HamiltonianScaling), D.ptr(src), D.ptr(dst),
+                                                   D.ptr(sz), D.stream_handle()))
+
     FORMS = ("one-workgroup", "split", "fused")
 
     def SetForm(self, form: int):
@@ -335,7 +342,7 @@ class MixedCayleyPropagator:
                  fallback=None, fallback_iterations: int = 40, krylov_basis_size: int = 20,
                  tolerance: float = 1e-10, inner_tolerance: float = 1e-5, max_outer: int = 8, factored=None,
                  device_mode: bool = False, learned_mode: int = 0, gram_schmidt_passes: int = 1,
-                 graph: bool = False, deferred_finish: bool = True):
+                 graph: bool = False, deferred_finish: bool = True, overlap_from_solve: bool = True):
         """device_mode: after the first converged step, solve the inner least-squares problems
         on the device and synchronise once per step (pp_mixed_set_device_mode).
         learned_mode: the FGMRES directions the learned correction acts on: 0 the first of
@@ -347,13 +354,17 @@ class MixedCayleyPropagator:
         (pp_mixed_set_graph; needs a factored operator whose angular forms are sparse).
         deferred_finish: let the next Kronecker solve finish a one-pass Gram-Schmidt that the
         host does not read first (one launch fewer; the same bits;
-        pp_mixed_set_deferred_finish)."""
+        pp_mixed_set_deferred_finish).
+        overlap_from_solve: form the FP32 overlap product of each preconditioner output in its
+        Kronecker solve, as F1 G F2^T, instead of the multiply's Cholesky-form passes
+        (pp_mixed_set_overlap_from_solve)."""
         if graph and not device_mode:
             raise ValueError("graph mode needs device_mode")
         self.graph = graph
         self.learned_mode = learned_mode
         self.gram_schmidt_passes = gram_schmidt_passes
         self.deferred_finish = deferred_finish
+        self.overlap_from_solve = overlap_from_solve
         self.psi = psi
         self.device_mode = device_mode
         self.factored = factored
@@ -390,6 +401,7 @@ class MixedCayleyPropagator:
         N.check(N.load().pp_mixed_set_learned_mode(h, int(self.learned_mode)))
         N.check(N.load().pp_mixed_set_gram_schmidt(h, int(self.gram_schmidt_passes)))
         N.check(N.load().pp_mixed_set_deferred_finish(h, int(self.deferred_finish)))
+        N.check(N.load().pp_mixed_set_overlap_from_solve(h, int(self.overlap_from_solve)))
         N.check(N.load().pp_mixed_set_graph(h, int(self.graph)))
 
     def SetLearned(self, learned):

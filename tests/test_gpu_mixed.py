@@ -226,6 +226,90 @@ def test_kron_fp32_forms(torch, rng):
     assert torch.equal(inplace, out["fused"])
 
 
+def test_kron_overlap_product(torch, rng):
+    """The overlap product S1 X S2 of the one-launch Kronecker solve's output X, formed in the
+    same launch as F1 G F2^T (F = S E, G the middle product; E^T S E = I): against S1 X S2 in
+    FP64 from the returned X, for a random input and for one whose solution is the
+    smallest-eigenvalue direction of S (x) S in every channel (where FP32 products of S lose
+    the most); X itself is bitwise the plain solve's."""
+    from pyprop_amd.mixed import KroneckerPreconditioner, radial_eigenbasis
+    say("Kronecker solve with the overlap product")
+    sysm = _small_helium()
+    repr_, psi, base = _device(sysm, torch)
+    kp = KroneckerPreconditioner(psi, sysm)
+    a = 0.005j
+    kp.SetHamiltonianScaling(a)
+    kp.Setup()
+    E, lam, l1, l2, S = radial_eigenbasis(sysm)
+    nc, n, _ = sysm.shape
+    w, U = np.linalg.eigh(S)
+    hard = np.outer(U[:, 0], U[:, 0]) * (1.0 + 0.5j)
+    cases = {"random": crandn(rng, sysm.shape)}
+    rhs = np.zeros(sysm.shape, dtype=np.complex128)
+    for c in range(nc):
+        h1 = S @ E[l1[c]] @ np.diag(lam[l1[c]]) @ E[l1[c]].T @ S
+        h2 = S @ E[l2[c]] @ np.diag(lam[l2[c]]) @ E[l2[c]].T @ S
+        rhs[c] = S @ hard @ S + a * (h1 @ hard @ S + S @ hard @ h2)  # K applied to the hard direction
+    cases["smallest S (x) S direction"] = rhs
+    for name, x in cases.items():
+        src = torch.from_numpy(x.astype(np.complex64)).to("cuda")
+        plain = torch.zeros_like(src)
+        kp.SolveInto(src, plain)
+        dst = torch.zeros_like(src)
+        sz = torch.zeros_like(src)
+        kp.SolveIntoWithOverlap(src, dst, sz)
+        assert torch.equal(dst, plain)
+        z = dst.cpu().numpy().astype(np.complex128)
+        ref = np.stack([S @ z[c] @ S for c in range(nc)])
+        err = rel(sz.cpu().numpy(), ref)
+        say("  %s: |sz - S z S| / |S z S| = %.2e" % (name, err))
+        assert err < (2e-5 if name == "random" else 1e-4), (name, err)  # measured 5.3e-6, 1.4e-5
+        if name != "random":  # the exact product for the intended solution (measured 4.0e-7)
+            exact = np.stack([S @ hard @ S for _ in range(nc)])
+            e2 = rel(sz.cpu().numpy(), exact)
+            say("    against S z_exact S: %.2e" % e2)
+            assert e2 < 4e-6
+
+
+def test_mixed_overlap_from_solve_vs_cholesky(torch, rng):
+    """Mixed steps with the FP32 overlap product from the Kronecker solve (the default) and
+    with the multiply's Cholesky-form passes: the same outer and inner counts and both
+    within 1e-8 of the dense FP64 solution, host-driven and in device mode."""
+    from pyprop_amd.factored import FactoredHamiltonian
+    from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
+    say("mixed step: overlap product from the solve vs Cholesky passes (small helium)")
+    sysm = _small_helium()
+    repr_, psi, base = _device(sysm, torch)
+    x0 = crandn(rng, sysm.shape)
+    fh = FactoredHamiltonian(base, overlap=True)
+    kp = KroneckerPreconditioner(psi, sysm)
+    for device_mode in (False, True):
+        res = {}
+        for from_solve in (True, False):
+            psi.data.copy_(torch.from_numpy(x0).to("cuda"))
+            prop = MixedCayleyPropagator(psi, base, kp, tolerance=1e-11, factored=fh, device_mode=device_mode,
+                                         overlap_from_solve=from_solve)
+            dt, t = 0.01, 40.0
+            cur = x0
+            stats = []
+            for k in range(4):
+                sc = base.scalings(t + dt / 2)
+                sol, A, b = _dense_cayley(sysm, cur, dt, sc, True)
+                prop.AdvanceStep(t, dt)
+                st = prop.GetStats()
+                stats.append((st["outer"], st["inner"]))
+                assert st["rel_residual"] <= 1e-11 and st["fallback"] == 0
+                cur = psi.numpy()
+                _check_solution(cur, sol, A, b, 2e-11, 1e-8)
+                t += dt
+            res[from_solve] = (cur, stats)
+            del prop
+        d = np.linalg.norm(res[True][0] - res[False][0]) / np.linalg.norm(res[False][0])
+        say("  device mode %d: counts %s vs %s, states differ by %.2e" % (device_mode, res[True][1], res[False][1], d))
+        assert res[True][1] == res[False][1]
+        assert d < 1e-9
+
+
 def _dense_cayley(sysm, x, dt, sc, with_system=False):
     from tests.test_oracle import dense_operator
     H, S = dense_operator(sysm, sc)
