@@ -18,6 +18,7 @@
 // sum |y - t / sigma|^2 / rows.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -427,73 +428,121 @@ union BfPack {  // eight BF16 values as one MFMA operand
   uint16_t u[8];
 };
 
-// part[slice][m][n] = sum_{k in slice} bf16(v[m][k] / |v|) W1[n][k]   (v: rows x K floats)
+constexpr int INP = 128;  // FP64 partial sums of squares behind the inference input's norm
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// 16 bytes at byte offset off of the buffer behind r (range-checked: zeros past its end, so a
+// masked-out lane passes kOff16 and its load needs no branch)
+constexpr int kOff16 = 0x7ffffff0;  // beyond every buffer read this way (all under 2 GiB)
+__device__ __forceinline__ u32x4 ld16(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, long long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// n2p[b] = sum of squares (FP64) over the b-th of INP equal contiguous ranges of the nf floats
+// of v (nf < 2^29).  Each pass of a thread issues its four 16-byte loads before summing them in
+// order; the range ends (nf, or v not 16-byte aligned) are summed value by value.
+__global__ __launch_bounds__(256) void norm_part_kernel(long long nf, const float* __restrict__ v,
+                                                        double* __restrict__ n2p) {
+  __shared__ double sh[4];
+  const int t = threadIdx.x;
+  const int chunk = (int)(((nf + INP - 1) / INP + 3) / 4 * 4);
+  const int lo = (int)blockIdx.x * chunk, hi = (int)min(nf, (long long)lo + chunk);
+  double acc = 0;
+  if ((reinterpret_cast<uintptr_t>(v) & 15) == 0) {
+    const __amdgpu_buffer_rsrc_t vr = rsrc(v, nf * 4);
+    const int hv = lo + max(0, hi - lo) / 4 * 4;  // end of the whole 16-byte groups
+    for (int base = lo; base < hv; base += 4096) {
+      u32x4 q[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = base + (t + u * 256) * 4;
+        q[u] = ld16(vr, i < hv ? i * 4 : kOff16);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float4 x = __builtin_bit_cast(float4, q[u]);
+        acc += (double)x.x * x.x + (double)x.y * x.y + (double)x.z * x.z + (double)x.w * x.w;
+      }
+    }
+    if (hv + t < hi) acc += (double)v[hv + t] * v[hv + t];
+  } else {
+    for (int i = lo + t; i < hi; i += 256) acc += (double)v[i] * v[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((t & 63) == 0) sh[t >> 6] = acc;
+  __syncthreads();
+  if (t == 0) n2p[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// part[slice][m][n] = sum_{k in slice} bf16(v[m][k] / |v|) W1[n][k]   (v: rows x K floats,
+// K a multiple of 8, v 16-byte aligned, v and W1 under 2 GiB).  A wave issues its W1 operands first (they wait for
+// nothing), then the norm partials and its share of v; |v| = sqrt(sum n2p) is summed by every
+// wave in the same order, and workgroup (0, 0) stores it in gout[0].
 __global__ __launch_bounds__(256) void infer_l1_kernel(int M, int N, int K, const float* __restrict__ v,
-                                                       double* __restrict__ nrm,
+                                                       const double* __restrict__ n2p, double* __restrict__ gout,
                                                        const uint16_t* __restrict__ w, float* __restrict__ part) {
   __shared__ __align__(16) uint16_t As[IMR * (IKC + 8)];
-  double g2 = 0;  // the global norm, summed in the same order as block_pack_kernel
-  for (int c = 0; c < M; ++c) g2 += nrm[c] * nrm[c];
-  const double g = std::sqrt(g2);
-  const float inv = g > 0 ? (float)(1.0 / g) : 0.f;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) nrm[M] = g;  // read by the Kronecker combination
   const int k0 = blockIdx.y * IKC;
-  const int kn = min(IKC, K - k0);
-  // stage bf16(v / |v|): each thread converts groups of 8 consecutive values (two 16-byte
-  // loads, all issued before any is used), 6 groups per thread
-  constexpr int GROUPS = IMR * IKC / 8 / 256;
-  float4 lo[GROUPS], hi[GROUPS];
-  const bool vec = (K & 3) == 0;
+  const int kn = min(IKC, K - k0);  // a multiple of 8
+  const int col = (blockIdx.x * 4 + wave) * 16 + (lane & 15);
+  constexpr int KS = IKC / 32;
+  const __amdgpu_buffer_rsrc_t wr = rsrc(w, (long long)N * K * 2), vr = rsrc(v, (long long)M * K * 4);
+  u32x4 b[KS];
 #pragma unroll
-  for (int q = 0; q < GROUPS; ++q) {
-    const int gidx = t + q * 256, m = gidx / (IKC / 8), kk = (gidx % (IKC / 8)) * 8;
-    const float* src = v + (long long)m * K + k0 + kk;
-    if (m < M && kk + 8 <= kn && vec) {
-      lo[q] = *reinterpret_cast<const float4*>(src);
-      hi[q] = *reinterpret_cast<const float4*>(src + 4);
-    } else {
-      float tmp[8];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) tmp[e] = (m < M && kk + e < kn) ? src[e] : 0.f;
-      lo[q] = make_float4(tmp[0], tmp[1], tmp[2], tmp[3]);
-      hi[q] = make_float4(tmp[4], tmp[5], tmp[6], tmp[7]);
-    }
+  for (int s = 0; s < KS; ++s) {
+    const int kq = s * 32 + (lane >> 4) * 8;
+    b[s] = ld16(wr, col < N && kq < kn ? (col * K + k0 + kq) * 2 : kOff16);
   }
+  static_assert(INP == 128, "two norm partials per lane");
+  const double p0 = n2p[lane], p1 = n2p[lane + 64];
+  // v in groups of 8 consecutive values (two 16-byte loads), 6 groups per thread
+  constexpr int GROUPS = IMR * IKC / 8 / 256;
+  u32x4 vq[GROUPS][2];
 #pragma unroll
   for (int q = 0; q < GROUPS; ++q) {
     const int gidx = t + q * 256, m = gidx / (IKC / 8), kk = (gidx % (IKC / 8)) * 8;
+    const int off = m < M && kk < kn ? (m * K + k0 + kk) * 4 : kOff16;
+    vq[q][0] = ld16(vr, off);
+    vq[q][1] = ld16(vr, off == kOff16 ? kOff16 : off + 16);
+  }
+  __builtin_amdgcn_sched_barrier(0);  // every load above in flight before the norm is summed
+  double g2 = p0 + p1;
+  for (int off = 32; off > 0; off >>= 1) g2 += __shfl_down(g2, off, 64);
+  const double g = std::sqrt(__shfl(g2, 0, 64));
+  const float inv = g > 0 ? (float)(1.0 / g) : 0.f;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) gout[0] = g;  // read by the Kronecker combination
+#pragma unroll
+  for (int q = 0; q < GROUPS; ++q) {
+    const int gidx = t + q * 256, m = gidx / (IKC / 8), kk = (gidx % (IKC / 8)) * 8;
+    const float4 lo = __builtin_bit_cast(float4, vq[q][0]), hi = __builtin_bit_cast(float4, vq[q][1]);
     BfPack pk;
-    pk.u[0] = f2bf(lo[q].x * inv);
-    pk.u[1] = f2bf(lo[q].y * inv);
-    pk.u[2] = f2bf(lo[q].z * inv);
-    pk.u[3] = f2bf(lo[q].w * inv);
-    pk.u[4] = f2bf(hi[q].x * inv);
-    pk.u[5] = f2bf(hi[q].y * inv);
-    pk.u[6] = f2bf(hi[q].z * inv);
-    pk.u[7] = f2bf(hi[q].w * inv);
+    pk.u[0] = f2bf(lo.x * inv);
+    pk.u[1] = f2bf(lo.y * inv);
+    pk.u[2] = f2bf(lo.z * inv);
+    pk.u[3] = f2bf(lo.w * inv);
+    pk.u[4] = f2bf(hi.x * inv);
+    pk.u[5] = f2bf(hi.y * inv);
+    pk.u[6] = f2bf(hi.z * inv);
+    pk.u[7] = f2bf(hi.w * inv);
     *reinterpret_cast<bf16x8*>(&As[m * (IKC + 8) + kk]) = pk.v;
   }
   __syncthreads();
-  const int n0 = (blockIdx.x * 4 + wave) * 16;
-  if (n0 >= N) return;
-  const int col = n0 + (lane & 15);
   f32x4 acc[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int ks = 0; ks < kn; ks += 32) {
-    const int kq = ks + (lane >> 4) * 8;
-    BfPack b;
-    if (col < N && kq + 8 <= kn && (((long long)col * K + k0 + kq) & 7) == 0) {
-      b.v = *reinterpret_cast<const bf16x8*>(w + (long long)col * K + k0 + kq);
-    } else {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) b.u[q] = (col < N && kq + q < kn) ? w[(long long)col * K + k0 + kq + q] : (uint16_t)0;
-    }
+  for (int s = 0; s < KS; ++s) {
+    if (s * 32 >= kn) break;
+    const int kq = s * 32 + (lane >> 4) * 8;
+    const bf16x8 bv = __builtin_bit_cast(bf16x8, b[s]);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const bf16x8 av = *reinterpret_cast<const bf16x8*>(&As[(r * 16 + (lane & 15)) * (IKC + 8) + kq]);
-      acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b.v, acc[r], 0, 0, 0);
+      acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[r], 0, 0, 0);
     }
   }
   float* out = part + (long long)blockIdx.y * M * N;
@@ -522,87 +571,126 @@ __global__ void infer_act1_kernel(int M, int N, int slices, const float* __restr
   }
 }
 
-// out[m][n] = act(sum_k A[m][k] W[n][k] + b[n]) for K <= IKMAX: A (BF16 rows x K) staged in
-// LDS, each wave one 16-column strip over the whole K.  GELU_BF16: bf16(GELU(.)) into hout,
-// else the FP32 value into fout.
-template <bool GELU_BF16>
+// out[m][n] = act(sum_k A[m][k] W[n][k] + b[n]) for rows <= 48 and K <= 128 S (a multiple of
+// 8; A, W and out under 2 GiB).  A workgroup owns NS 16-column strips; its four waves split K, each over at most S
+// 32-wide steps, and load their MFMA operands (A rows and W columns, 16 bytes per lane and
+// step) straight from memory, all before the first MFMA.  The four partial tiles are added
+// through LDS in wave order.  GELU_BF16: bf16(GELU(.)) into hout, else the FP32 value into fout.
+constexpr int ISMAX = (IKMAX / 32 + 3) / 4;  // K steps of one wave at K = IKMAX
+template <bool GELU_BF16, int NS, int S>
 __global__ __launch_bounds__(256) void infer_layer_kernel(int M, int N, int K, const uint16_t* __restrict__ a,
                                                           const uint16_t* __restrict__ w,
                                                           const float* __restrict__ bias,
                                                           uint16_t* __restrict__ hout, float* __restrict__ fout) {
-  __shared__ __align__(16) uint16_t As[IMR * (IKMAX + 8)];
+  __shared__ f32x4 red[3][3 * NS][64];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int kp = (K + 31) / 32 * 32;
-  // stage A in 16-byte groups of 8 values, all loads of a thread issued before the stores
-  constexpr int GROUPS = IMR * IKMAX / 8 / 256;  // upper bound; groups past IMR x kp are skipped
-  BfPack g[GROUPS];
-  const int gpr = kp / 8;  // groups per row
-  const bool vec = (K & 7) == 0;
+  const int s0 = wave * S, nsteps = (K + 31) / 32;
+  const int n0 = blockIdx.x * 16 * NS;
+  u32x4 av[S][3], bv[S][NS];
 #pragma unroll
-  for (int q = 0; q < GROUPS; ++q) {
-    const int gidx = t + q * 256, m = gidx / gpr, kk = (gidx % gpr) * 8;
-    if (gidx < IMR * gpr && m < M && kk + 8 <= K && vec) {
-      g[q].v = *reinterpret_cast<const bf16x8*>(a + (long long)m * K + kk);
-    } else {
-#pragma unroll
-      for (int e = 0; e < 8; ++e) g[q].u[e] = (gidx < IMR * gpr && m < M && kk + e < K) ? a[(long long)m * K + kk + e]
-                                                                                   : (uint16_t)0;
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < GROUPS; ++q) {
-    const int gidx = t + q * 256, m = gidx / gpr, kk = (gidx % gpr) * 8;
-    if (gidx < IMR * gpr) *reinterpret_cast<bf16x8*>(&As[m * (IKMAX + 8) + kk]) = g[q].v;
-  }
-  __syncthreads();
-  const int n0 = (blockIdx.x * 4 + wave) * 16;
-  if (n0 >= N) return;
-  const int col = n0 + (lane & 15);
-  f32x4 acc[3];
-#pragma unroll
-  for (int r = 0; r < 3; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int ks = 0; ks < K; ks += 32) {
-    const int kq = ks + (lane >> 4) * 8;
-    BfPack b;
-    if (col < N && kq + 8 <= K && (((long long)col * K + kq) & 7) == 0) {
-      b.v = *reinterpret_cast<const bf16x8*>(w + (long long)col * K + kq);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) b.u[q] = (col < N && kq + q < K) ? w[(long long)col * K + kq + q] : (uint16_t)0;
-    }
+  for (int s = 0; s < S; ++s) {
+    const int kq = (s0 + s) * 32 + (lane >> 4) * 8;
+    const bool kok = kq < K;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      const bf16x8 av = *reinterpret_cast<const bf16x8*>(&As[(r * 16 + (lane & 15)) * (IKMAX + 8) + kq]);
-      acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b.v, acc[r], 0, 0, 0);
+      const int row = r * 16 + (lane & 15);
+      const bool ok = kok && row < M;
+      const u32x4 x = *reinterpret_cast<const u32x4*>(ok ? a + (long long)row * K + kq : a);
+      av[s][r] = ok ? x : u32x4{0u, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int col = n0 + j * 16 + (lane & 15);
+      const bool ok = kok && col < N;
+      const u32x4 x = *reinterpret_cast<const u32x4*>(ok ? w + (long long)col * K + kq : w);
+      bv[s][j] = ok ? x : u32x4{0u, 0u, 0u, 0u};
     }
   }
-  if (col >= N) return;
-  const float bn = bias[col];
+  f32x4 acc[3][NS];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int row = r * 16 + (lane >> 4) * 4 + q;
-      if (row >= M) continue;
-      const float v = acc[r][q] + bn;
-      if (GELU_BF16) hout[(long long)row * N + col] = f2bf(gelu(v));
-      else fout[(long long)row * N + col] = v;
-    }
+    for (int j = 0; j < NS; ++j) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (s0 + s >= nsteps) break;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int j = 0; j < NS; ++j)
+        acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[s][r]),
+                                                            __builtin_bit_cast(bf16x8, bv[s][j]), acc[r][j], 0, 0, 0);
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int j = 0; j < NS; ++j) red[wave - 1][r * NS + j][lane] = acc[r][j];
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int j = 0; j < NS; ++j) acc[r][j] += red[p][r * NS + j][lane];
+  // range-checked buffer stores: a masked-out value goes to an offset past the end and is
+  // dropped, so no store waits behind a branch
+  constexpr int esz = GELU_BF16 ? 2 : 4;
+  const __amdgpu_buffer_rsrc_t orr = rsrc(GELU_BF16 ? (const void*)hout : (const void*)fout, (long long)M * N * esz);
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const int col = n0 + j * 16 + (lane & 15);
+    const float bn = bias[min(col, N - 1)];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = r * 16 + (lane >> 4) * 4 + q;
+        const int off = row < M && col < N ? (row * N + col) * esz : kOff16;
+        const float v = acc[r][j][q] + bn;
+        if (GELU_BF16) __builtin_amdgcn_raw_buffer_store_b16(f2bf(gelu(v)), orr, off, 0, 0);
+        else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), orr, off, 0, 0);
+      }
+  }
 }
 
-// y (rows = channels) from the block norms and v, with the streaming kernels above
+template <bool GELU_BF16, int NS>
+static void launch_infer_layer(int M, int N, int K, const uint16_t* a, const uint16_t* w, const float* bias,
+                               uint16_t* hout, float* fout, hipStream_t s) {
+  const int steps = (int)ceil_div((K + 31) / 32, 4);
+  const dim3 grid((unsigned)ceil_div(N, 16 * NS));
+  switch (steps) {
+#define PP_INFER_LAYER(S)                                                                                        \
+  case S:                                                                                                        \
+    hipLaunchKernelGGL((infer_layer_kernel<GELU_BF16, NS, S>), grid, dim3(256), 0, s, M, N, K, a, w, bias, hout, \
+                       fout);                                                                                    \
+    break;
+    PP_INFER_LAYER(1)
+    PP_INFER_LAYER(2)
+    PP_INFER_LAYER(3)
+    PP_INFER_LAYER(4)
+    PP_INFER_LAYER(5)
+#undef PP_INFER_LAYER
+    default: PP_REQUIRE(false, "learned inference: hidden width above the streaming limit");
+  }
+  static_assert(ISMAX == 5, "the dispatch covers 1 .. ISMAX steps per wave");
+}
+
+// y (rows = channels) from v: the norm partials, then the streaming kernels above
 static void infer_streaming(Learned& L, const float2* in, hipStream_t s) {
   const int M = L.nchan, H = L.hidden, I = L.in;
   const int slices = (int)ceil_div(I, IKC);
   PP_REQUIRE((size_t)slices * M * H <= L.part.count, "learned: partial buffer too small");
+  hipLaunchKernelGGL(norm_part_kernel, dim3(INP), dim3(256), 0, s, (long long)M * I,
+                     reinterpret_cast<const float*>(in), L.n2p.ptr);
   hipLaunchKernelGGL(infer_l1_kernel, dim3((unsigned)ceil_div(H, 64), slices), dim3(256), 0, s, M, H, I,
-                     reinterpret_cast<const float*>(in), L.nrm.ptr, L.w1h.ptr, L.part.ptr);
+                     reinterpret_cast<const float*>(in), L.n2p.ptr, L.nrm.ptr + M, L.w1h.ptr, L.part.ptr);
   hipLaunchKernelGGL(infer_act1_kernel, dim3((unsigned)ceil_div((long long)M * H, 256)), dim3(256), 0, s, M, H, slices,
                      L.part.ptr, L.params.ptr + L.o_b1, L.params.ptr + L.o_emb, L.h1h.ptr);
-  hipLaunchKernelGGL(infer_layer_kernel<true>, dim3((unsigned)ceil_div(H, 64)), dim3(256), 0, s, M, H, H, L.h1h.ptr,
-                     L.w2h.ptr, L.params.ptr + L.o_b2, L.h2h.ptr, (float*)nullptr);
-  hipLaunchKernelGGL(infer_layer_kernel<false>, dim3((unsigned)ceil_div(I, 64)), dim3(256), 0, s, M, I, H, L.h2h.ptr,
-                     L.w3h.ptr, L.params.ptr + L.o_b3, (uint16_t*)nullptr, L.y.ptr);
+  launch_infer_layer<true, 1>(M, H, H, L.h1h.ptr, L.w2h.ptr, L.params.ptr + L.o_b2, L.h2h.ptr, nullptr, s);
+  launch_infer_layer<false, 2>(M, I, H, L.h2h.ptr, L.w3h.ptr, L.params.ptr + L.o_b3, nullptr, L.y.ptr, s);
   PP_HIP(hipGetLastError());
 }
 
@@ -611,10 +699,11 @@ void learned_apply(Learned& L, const Kron& K, double2 a, const float2* in, float
   const int nb = L.n * L.n;
   L.ensure(L.nchan);
   const long long n = (long long)L.nchan * nb;
-  hipLaunchKernelGGL((block_norm_kernel<float2>), dim3(L.nchan), dim3(256), 0, s, nb, in, L.nrm.ptr);
-  if (L.nchan <= IMR && L.hidden <= IKMAX && L.hidden % 8 == 0) {
+  if (L.nchan <= IMR && L.hidden <= IKMAX && L.hidden % 8 == 0 && L.in % 8 == 0 &&
+      (reinterpret_cast<uintptr_t>(in) & 15) == 0 && (long long)L.in * std::max(L.nchan, L.hidden) * 4 < (1LL << 31)) {
     infer_streaming(L, in, s);
   } else {
+    hipLaunchKernelGGL((block_norm_kernel<float2>), dim3(L.nchan), dim3(256), 0, s, nb, in, L.nrm.ptr);
     hipLaunchKernelGGL(block_pack_kernel, dim3(grid_for_n(n)), dim3(256), 0, s, L.nchan, nb, in, L.nrm.ptr, L.xh.ptr);
     L.forward(L.nchan, L.iota.ptr, s);
   }
@@ -675,6 +764,7 @@ extern "C" int pp_learned_create(int nchan, int n, int hidden, unsigned long lon
     for (int c = 0; c < nchan; ++c) iota[c] = c;
     L.iota.upload(iota.data(), iota.size(), s);
     L.nrm.resize(nchan + 1);
+    L.n2p.resize(INP);
     L.refresh_bf16(s);
     PP_HIP(hipStreamSynchronize(s));
     *out = p;

@@ -30,6 +30,7 @@ struct Learned {
   DeviceBuffer<double2> loss_part;
   DeviceBuffer<int> iota;
   DeviceBuffer<double> nrm;          // per-block norms, [nchan] = global norm
+  DeviceBuffer<double> n2p;          // partial sums of squares of the inference input
   DeviceBuffer<double2> tmp64;
   int splits_max = 32;
   void ensure(int rows);
