@@ -694,7 +694,7 @@ static void infer_streaming(Learned& L, const float2* in, hipStream_t s) {
   PP_HIP(hipGetLastError());
 }
 
-void learned_apply(Learned& L, const Kron& K, double2 a, const float2* in, float2* out, hipStream_t s) {
+void learned_apply(Learned& L, const Kron& K, double2 a, const float2* in, float2* out, hipStream_t s, float2* sz) {
   PP_REQUIRE(K.nchan == L.nchan && K.n == L.n, "learned preconditioner and Kronecker shapes differ");
   const int nb = L.n * L.n;
   L.ensure(L.nchan);
@@ -708,7 +708,7 @@ void learned_apply(Learned& L, const Kron& K, double2 a, const float2* in, float
     L.forward(L.nchan, L.iota.ptr, s);
   }
   // out = K^-1 (in + sigma |v| y): the combination is formed as the Kronecker solve loads its input
-  K.apply32_add(a, in, reinterpret_cast<const float2*>(L.y.ptr), L.nrm.ptr + L.nchan, L.sigma, out, s);
+  K.apply32_add(a, in, reinterpret_cast<const float2*>(L.y.ptr), L.nrm.ptr + L.nchan, L.sigma, out, s, sz);
   PP_HIP(hipGetLastError());
 }
 

@@ -1426,15 +1426,15 @@ bool Kron::overlap_product32(const float2* in, const float2* out, const float2* 
 }
 
 void Kron::apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
-                       float2* out, hipStream_t s) const {
-  launch32(a, in, out, s, 1.f, nullptr, add, add_norm, add_sigma, nullptr);
+                       float2* out, hipStream_t s, float2* sz) const {
+  launch32(a, in, out, s, 1.f, nullptr, add, add_norm, add_sigma, nullptr, nullptr, sz);
 }
 
 void Kron::launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
                     const float2* add, const double* add_norm, float add_sigma, const double2* in_n2,
                     const GsFinish* fin, float2* sz) const {
   PP_REQUIRE(!fin || (fin->p3 && in_n2), "a pending finish needs the device slot of its |w|^2");
-  PP_REQUIRE(!sz || (!add && overlap_product32(in, out, wb)), "the overlap product needs the one-launch form");
+  PP_REQUIRE(!sz || overlap_product32(in, out, wb), "the overlap product needs the one-launch form");
   // one launch only without aliasing: the workgroups of a channel all read the whole input
   // block while each writes its strip of out (and of the write-back)
   if (n <= 64 && split32 && fused32 && in != out && (!wb || (wb != out && wb != in && !add))) {
@@ -1854,6 +1854,8 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     ++stats.inner_ops;
   };
   int correction = 0;
+  // the solves form the FP32 multiply's overlap product (Cholesky-form overlap only)
+  const bool sz_from_solve = overlap_from_solve && fham && fham->csd == fham->bw && fham->sep_enabled;
   // z = P (in / sqrt(n2)) (n2 = NULL: in is already normalised); wb (with n2): receives
   // in / sqrt(n2), the normalised basis vector
   auto apply_P32 = [&](int j, const float2* in, const double2* n2, float2* out, float2* wb) {
@@ -1869,14 +1871,15 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
         PP_HIP(hipGetLastError());
         v = wb;
       }
-      learned_apply(*learned, K, ka, v, out, cs);
+      const bool with_sz = sz_from_solve && K.overlap_product32(v, out, nullptr);
+      learned_apply(*learned, K, ka, v, out, cs, with_sz ? work.sz.ptr : nullptr);
+      work.sz_for = with_sz ? out : nullptr;
       ++stats.learned_applies;
     } else {
       // a finish left by the last Gram-Schmidt is done by this launch; with the Cholesky-form
       // FP32 overlap, the solve also forms (S (x) S) out for the multiply that follows
       float2* wb2 = n2 ? wb : nullptr;
-      const bool with_sz = overlap_from_solve && fham && fham->csd == fham->bw && fham->sep_enabled &&
-                           K.overlap_product32(in, out, wb2);
+      const bool with_sz = sz_from_solve && K.overlap_product32(in, out, wb2);
       K.apply32(ka, in, out, cs, 1.f, wb2, n2, work.pend.p3 ? &work.pend : nullptr, with_sz ? work.sz.ptr : nullptr);
       work.pend = GsFinish{};
       work.sz_for = with_sz ? out : nullptr;

@@ -42,8 +42,10 @@ struct Learned {
   void adam(float lr, float beta1, float beta2, float eps, hipStream_t s);
 };
 
-// out = K^-1 (in + sigma |in| net(in_c / |in|)) per channel block c (FP32 vectors)
-void learned_apply(Learned& L, const Kron& K, double2 a, const float2* in, float2* out, hipStream_t s);
+// out = K^-1 (in + sigma |in| net(in_c / |in|)) per channel block c (FP32 vectors); sz (may be
+// NULL): also (S (x) S) out, formed by the Kronecker solve (Kron::apply32_add)
+void learned_apply(Learned& L, const Kron& K, double2 a, const float2* in, float2* out, hipStream_t s,
+                   float2* sz = nullptr);
 
 // C[M x N] (+)= sum_k A(m, k) B(k, n), A(m, k) = a[m*sam + k*sak], B(k, n) = b[k*sbk + n*sbn]
 // (BF16 inputs, FP32 result).  Splits K into at most `splits` slices, slice z written to

@@ -51,9 +51,10 @@ struct Kron {
                const double2* in_n2, const GsFinish* fin = nullptr, float2* sz = nullptr) const;
   // whether apply32 with these buffers runs the one-launch form, which can form sz
   bool overlap_product32(const float2* in, const float2* out, const float2* wb) const;
-  // out <- K^-1 (in + add_sigma * (*add_norm) * add), *add_norm read on the device
+  // out <- K^-1 (in + add_sigma * (*add_norm) * add), *add_norm read on the device; sz (may be
+  // NULL, needs overlap_product32(in, out, NULL)): also (S (x) S) out, as apply32 forms it
   void apply32_add(double2 a, const float2* in, const float2* add, const double* add_norm, float add_sigma,
-                   float2* out, hipStream_t s) const;
+                   float2* out, hipStream_t s, float2* sz = nullptr) const;
   void launch32(double2 a, const float2* in, float2* out, hipStream_t s, float in_scale, float2* wb,
                 const float2* add, const double* add_norm, float add_sigma, const double2* in_n2,
                 const GsFinish* fin = nullptr, float2* sz = nullptr) const;

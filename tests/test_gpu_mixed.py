@@ -271,24 +271,37 @@ def test_kron_overlap_product(torch, rng):
             assert e2 < 4e-6
 
 
-def test_mixed_overlap_from_solve_vs_cholesky(torch, rng):
+@pytest.mark.parametrize("learned", [False, True])
+def test_mixed_overlap_from_solve_vs_cholesky(torch, rng, learned):
     """Mixed steps with the FP32 overlap product from the Kronecker solve (the default) and
     with the multiply's Cholesky-form passes: the same outer and inner counts and both
-    within 1e-8 of the dense FP64 solution, host-driven and in device mode."""
+    within 1e-8 of the dense FP64 solution, host-driven and in device mode.  learned: a
+    network with random output weights (scale 0.05) on every direction, so every solve is
+    the learned one (the combined input's solve forms the product)."""
     from pyprop_amd.factored import FactoredHamiltonian
-    from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
-    say("mixed step: overlap product from the solve vs Cholesky passes (small helium)")
+    from pyprop_amd.mixed import KroneckerPreconditioner, LearnedPreconditioner, MixedCayleyPropagator
+    say("mixed step: overlap product from the solve vs Cholesky passes (small helium%s)"
+        % (", learned on every direction" if learned else ""))
     sysm = _small_helium()
     repr_, psi, base = _device(sysm, torch)
     x0 = crandn(rng, sysm.shape)
     fh = FactoredHamiltonian(base, overlap=True)
     kp = KroneckerPreconditioner(psi, sysm)
+    lp = None
+    if learned:
+        lp = LearnedPreconditioner(kp, hidden=64, seed=5)
+        nc, n, _ = sysm.shape
+        flat = lp.get_params().astype(np.float64)
+        o, shp = mlp.layout(2 * n * n, 64, nc)[0]["W3"]
+        flat[o:o + int(np.prod(shp))] = rng.standard_normal(int(np.prod(shp))) * 0.1
+        lp.set_params(flat.astype(np.float32))
+        lp.set_scale(0.05)
     for device_mode in (False, True):
         res = {}
         for from_solve in (True, False):
             psi.data.copy_(torch.from_numpy(x0).to("cuda"))
-            prop = MixedCayleyPropagator(psi, base, kp, tolerance=1e-11, factored=fh, device_mode=device_mode,
-                                         overlap_from_solve=from_solve)
+            prop = MixedCayleyPropagator(psi, base, kp, lp, tolerance=1e-11, factored=fh, device_mode=device_mode,
+                                         overlap_from_solve=from_solve, learned_mode=2 if learned else 0)
             dt, t = 0.01, 40.0
             cur = x0
             stats = []
@@ -299,6 +312,7 @@ def test_mixed_overlap_from_solve_vs_cholesky(torch, rng):
                 st = prop.GetStats()
                 stats.append((st["outer"], st["inner"]))
                 assert st["rel_residual"] <= 1e-11 and st["fallback"] == 0
+                assert st["learned"] == (st["precond"] if learned else 0)
                 cur = psi.numpy()
                 _check_solution(cur, sol, A, b, 2e-11, 1e-8)
                 t += dt
