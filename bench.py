@@ -498,6 +498,9 @@ def main():
                          "synchronise once per step (pp_mixed_set_device_mode)")
     ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
                     help="mixed solver in device mode: 1 = replay each step from a captured hipGraph")
+    ap.add_argument("--learned-mode", type=int, default=0, choices=(0, 1, 2),
+                    help="directions that get the learned correction: 0 the first of the first correction, "
+                         "1 the first of every correction, 2 every direction (pp_mixed_set_learned_mode)")
     ap.add_argument("--gram-schmidt", type=int, default=1, choices=(1, 2),
                     help="classical Gram-Schmidt passes per inner FGMRES iteration of the mixed solver")
     ap.add_argument("--profile-mixed", type=int, default=0,
@@ -733,7 +736,7 @@ def main():
     for name, lrn in (("fp32_kron", None), ("fp32_learned", learned)):
         prop = MixedCayleyPropagator(psi, base, kron, lrn, fallback=plain, tolerance=1e-10, factored=fh,
                                      gram_schmidt_passes=a.gram_schmidt, device_mode=bool(a.device_mode or a.graph),
-                                     graph=bool(a.graph))
+                                     graph=bool(a.graph), learned_mode=a.learned_mode)
         sps, sec, t_e, end, st, chk = timed(prop, name)
         runs[name] = dict(steps_per_s=sps, ms_per_step=1e3 * sec / a.steps, error_vs_plain=err_vs(end),
                           stats=agg(st), check=chk,
@@ -794,6 +797,7 @@ def main():
                        "fp32_learned_kron_factored": lrn["steps_per_s"]},
                    "runs": runs,
                    "gram_schmidt_passes": a.gram_schmidt,
+                   "learned_mode": a.learned_mode,
                    "device_mode": int(a.device_mode or a.graph),
                    "graph": a.graph,
                    "wavefunction_error_vs_oracle": oracle_err,
