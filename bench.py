@@ -480,9 +480,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-atom batched pAMP section")
     ap.add_argument("--no-hydrogen", action="store_true", help="skip the H-atom section")
     ap.add_argument("--profile-batch", type=int, default=0, help="only run this many batched steps (for rocprof)")
-    ap.add_argument("--train-trajectories", type=int, default=64)
+    ap.add_argument("--train-trajectories", type=int, default=128)
     ap.add_argument("--train-steps", type=int, default=24)
-    ap.add_argument("--train-epochs", type=int, default=30)
+    ap.add_argument("--train-epochs", type=int, default=200)
     ap.add_argument("--profile-matvec", action="store_true", help="only run the matvec loop (for rocprof)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only copy 1 GiB device to device 20 times (HBM counter calibration for rocprof)")
@@ -498,6 +498,9 @@ def main():
                          "synchronise once per step (pp_mixed_set_device_mode)")
     ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
                     help="mixed solver in device mode: 1 = replay each step from a captured hipGraph")
+    ap.add_argument("--learned-params", default="",
+                    help="file (.npz) of the network's parameters and output scale: the full run writes it after "
+                         "training, --profile-learned reads it instead of its short training when it exists")
     ap.add_argument("--learned-mode", type=int, default=0, choices=(0, 1, 2),
                     help="directions that get the learned correction: 0 the first of the first correction, "
                          "1 the first of every correction, 2 every direction (pp_mixed_set_learned_mode)")
@@ -564,8 +567,14 @@ def main():
         psi0 = psi.data.clone()
         kron = KroneckerPreconditioner(psi, sysm)
         learned = LearnedPreconditioner(kron, hidden=512, seed=1)
-        x, tt, ch = generate_training_pairs(learned, base, plain, psi, dt, 4, 4, seed=2024)
-        learned.fit(x, ch, tt, epochs=2, batch=256, lr=1e-3, seed=1)
+        if a.learned_params and os.path.exists(a.learned_params):
+            with np.load(a.learned_params) as f:
+                learned.set_params(f["params"])
+                learned.set_scale(float(f["sigma"]))
+            log("learned parameters from %s" % a.learned_params)
+        else:
+            x, tt, ch = generate_training_pairs(learned, base, plain, psi, dt, 4, 4, seed=2024)
+            learned.fit(x, ch, tt, epochs=2, batch=256, lr=1e-3, seed=1)
         psi.data.copy_(psi0)
         prop = MixedCayleyPropagator(psi, base, kron, learned=learned, tolerance=1e-10,
                                      factored=tuned_factored(base)[0], gram_schmidt_passes=a.gram_schmidt,
@@ -690,6 +699,8 @@ def main():
          learned.loss_history[0], learned.loss_history[-1]))
     n_rows = int(x.shape[0])
     del x, tt, ch
+    if a.learned_params:
+        np.savez(a.learned_params, params=learned.get_params(), sigma=np.float64(learned.sigma))
 
     # what the network gains: residual left after one preconditioned direction on held-out
     # steps (the benchmark pulse and start, not seen in training), Kronecker alone vs with the network
