@@ -485,8 +485,8 @@ def test_learned_preconditioner_gain(torch):
     on held-out steps of a new pulse compare the residual left after one preconditioned
     direction: Kronecker alone vs Kronecker + learned correction (at least 20% lower); and
     the FGMRES iterations the mixed solver spends on 12 held-out steps with each (the
-    learned correction on the first direction of every correction): no more with it, and
-    the counts are reported (at full size bench.py measures 5.85 against 6.0 per step)."""
+    learned correction on the first direction of every correction): fewer with it (at full
+    size bench.py measures 5.0 against 6.0 per step)."""
     from pyprop_amd.mixed import (KroneckerPreconditioner, LearnedPreconditioner, PulseDistribution,
                                   generate_training_pairs, scalings_for)
     from pyprop_amd.propagator import CayleyPropagator, RadialTwoElectronPreconditioner
@@ -538,7 +538,8 @@ def test_learned_preconditioner_gain(torch):
         its[name] = total
         del prop
     say("  FGMRES iterations over 12 held-out steps: %s" % its)
-    assert its["learned"] <= its["kron"]
+    # measured 69 against 72 (the training and the steps are seeded and deterministic)
+    assert its["learned"] < its["kron"]
 
 
 def test_helium_full_size_mixed_vs_plain(torch):
