@@ -199,7 +199,7 @@ def factored_flops(fh, nc, n, precision):
     and angular variants chosen for `precision`.  Radial stage: band-tile variants with
     register reuse issue one 16x16x4 MFMA per (band-tile term, i1, jj, i2 tile, channel
     group, K step); the position variants ("pos<g>", and "pos<g>w<t>" with the window in
-    LDS) one per (16 band terms, position, channel group, K step of the w x w window),
+    LDS, "s" with the swizzled window) one per (16 band terms, position, channel group, K step of the w x w window),
     channel groups rounded up to a multiple of g.  The separable terms run as two 1-D band passes on the vector units and are not
     counted.  Angular stage, GEMM forms: two MFMAs (re, im) per (16 channels, 16 positions,
     K step of 4), each wave's share of the K steps rounded up to whole batches (12 steps, 4

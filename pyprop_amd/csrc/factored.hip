@@ -415,9 +415,12 @@ __global__ __launch_bounds__(256) void fham_radial_pos_kernel(int nc, int ncp, i
 
 // LDS layout of fham_radial_win_kernel: values per window point (FP32: an odd multiple of 16,
 // so that the two points a half-wave reads lie in opposite halves of the 32 banks) and bytes.
-template <class T, int MG>
+// SW (FP32, even MG): no padding; instead value v of point p sits at p PS + (v ^ 16 (p & 1)),
+// which puts consecutive points' 16-value halves in opposite bank halves just the same, in
+// the smaller window that lets three workgroups share a CU.
+template <class T, int MG, bool SW = false>
 __host__ __device__ constexpr int win_point_stride() {
-  return sizeof(T) == 4 && MG % 2 == 0 ? MG * 16 + 16 : MG * 16;
+  return sizeof(T) == 4 && MG % 2 == 0 && !SW ? MG * 16 + 16 : MG * 16;
 }
 // the largest divisor of rows that is at most cap (rows of the window copied per batch)
 __host__ __device__ constexpr int win_rows_per_batch(int rows, int cap) {
@@ -426,15 +429,16 @@ __host__ __device__ constexpr int win_rows_per_batch(int rows, int cap) {
 // LDS of fham_radial_win_kernel with nw waves: the window, then 16 bytes per thread that the
 // copy's slots outside the window write; afterwards the same space holds the products,
 // MG x 8 x 16 rows of TW + 1 complex values
-__host__ __device__ constexpr size_t win_lds_bytes(int bw, int mg, int tw, int nw, int esz) {
-  const size_t win = (size_t)(2 * bw + 2) * (tw + 2 * bw) * (esz == 4 && mg % 2 == 0 ? mg * 16 + 16 : mg * 16) * esz +
+__host__ __device__ constexpr size_t win_lds_bytes(int bw, int mg, int tw, int nw, int esz, bool sw = false) {
+  const size_t win = (size_t)(2 * bw + 2) * (tw + 2 * bw) * (esz == 4 && mg % 2 == 0 && !sw ? mg * 16 + 16 : mg * 16) *
+                         esz +
                      (size_t)64 * nw * 16;
   const size_t out = (size_t)mg * 8 * 16 * (tw + 1) * 2 * esz;
   return win > out ? win : out;
 }
-template <class T, int BW, int MG, int TW, int NW>
+template <class T, int BW, int MG, int TW, int NW, bool SW = false>
 __host__ __device__ constexpr size_t win_lds_bytes() {
-  return win_lds_bytes(BW, MG, TW, NW, (int)sizeof(T));
+  return win_lds_bytes(BW, MG, TW, NW, (int)sizeof(T), SW);
 }
 
 // Stage 1, position form with the window in LDS.  A workgroup takes one r1 point i1, TW
@@ -449,7 +453,7 @@ __host__ __device__ constexpr size_t win_lds_bytes() {
 // (jj, ip + d), ip the position's offset in the tile.  With (jj0, d0) = divmod(4 ks, w) that
 // point is jj0 WC + d0 + lk, plus WC - w where d0 + lk >= w (the row wraps): a lane base
 // chosen by ks at compile time plus an immediate, so the reads take no address arithmetic.
-template <class T, int BW, int MG, int TW, int NW>
+template <class T, int BW, int MG, int TW, int NW, bool SW = false>
 __global__ __launch_bounds__(64 * NW) void fham_radial_win_kernel(int nc, int n, int nterms, int ngroups, int ntiles,
                                                               long long nblocks, int ncp, const T* __restrict__ Xt,
                                                               const T* __restrict__ RP, T* __restrict__ Z,
@@ -459,11 +463,13 @@ __global__ __launch_bounds__(64 * NW) void fham_radial_win_kernel(int nc, int n,
   typedef typename MfmaT<T>::C2 C2;
   constexpr int w = 2 * BW + 1, KK = w * w, KP = (KK + 3) / 4;
   constexpr int WC = TW + 2 * BW;  // window columns
-  constexpr int PS = win_point_stride<T, MG>();
+  constexpr int PS = win_point_stride<T, MG, SW>();
   constexpr int NT = 64 * NW;   // threads
   constexpr int PPW = TW / NW;  // positions per wave
   constexpr int KB = 4;         // K steps per batch of LDS reads
   static_assert(TW % NW == 0, "the waves share the tile's positions evenly");
+  static_assert(!SW || (sizeof(T) == 4 && MG % 2 == 0 && PPW % 2 == 0 && (TW - 1) % 2 == 1),
+                "the swizzled window: FP32, pairs of 16-value halves, an even first point per wave");
   extern __shared__ __align__(16) char win_raw[];
   T* win = reinterpret_cast<T*>(win_raw);
   const int per = (int)((nblocks + 7) / 8);  // nblocks < 2^31 (n <= 64)
@@ -536,7 +542,8 @@ __global__ __launch_bounds__(64 * NW) void fham_radial_win_kernel(int nc, int n,
 #pragma unroll
         for (int cs = 0; cs < NCS; ++cs) {  // unconditional stores: no load is sunk into a branch
           const int jc = jcb + cs * CPI;
-          const int dst = jc < WC && q < PPP ? (jr * WC + jc) * PS + q * VPP : spare;
+          const int pt = jr * WC + jc;
+          const int dst = jc < WC && q < PPP ? pt * PS + (SW ? (q * VPP) ^ ((pt & 1) << 4) : q * VPP) : spare;
           *reinterpret_cast<Piece*>(win + dst) = v[r][cs];
         }
       }
@@ -549,6 +556,20 @@ __global__ __launch_bounds__(64 * NW) void fham_radial_win_kernel(int nc, int n,
   const int b0 = lk * PS + cl * 2 + part;  // point lk of row 0, this lane's operand row
   const int bwr = b0 + (WC - w) * PS;      // the same point after a row wrap
   const int b1 = lk >= 1 ? bwr : b0, b2 = lk >= 2 ? bwr : b0, b3 = lk >= 3 ? bwr : b0;
+  // swizzled window: the lane's point is the read's compile-time point offset (even wave
+  // start) plus lk, or lk + WC - w (= TW - 1, odd) past a row wrap; eb[h][j] is base j with
+  // the 16-value half h ^ (parity of the lane's part of the point) selected
+  int eb[2][4];
+  if (SW) {
+    const int pa = lk & 1, pw = (lk + WC - w) & 1;
+    const int par[4] = {pa, lk >= 1 ? pw : pa, lk >= 2 ? pw : pa, lk >= 3 ? pw : pa};
+    const int bs[4] = {b0, b1, b2, b3};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      eb[0][j] = bs[j] + (par[j] << 4);
+      eb[1][j] = bs[j] + ((par[j] ^ 1) << 4);
+    }
+  }
   // The wave's positions run as one pipeline: each batch of KB K steps issues the LDS reads
   // of the next batch (the next position's first batch after a position's last) and the next
   // position's R operands with the same K indices, then its MFMAs.  In a wave with a valid
@@ -560,10 +581,17 @@ __global__ __launch_bounds__(64 * NW) void fham_radial_win_kernel(int nc, int n,
     const T* wv = win + wave * PPW * PS;  // the wave's first position
     auto read_x = [&](int sp, int ks, T (&a)[MG]) {
       const int d0 = (4 * ks) % w, th = w - d0;  // lanes lk >= th wrap
-      const int base = th >= 4 ? b0 : th == 3 ? b3 : th == 2 ? b2 : b1;
-      const int c0 = ((4 * ks) / w * WC + d0 + sp) * PS;
+      const int cpt = (4 * ks) / w * WC + d0 + sp;  // the read's point offset
+      const int c0 = cpt * PS;
+      if (SW) {
+        const int j = th >= 4 ? 0 : th == 3 ? 3 : th == 2 ? 2 : 1;
 #pragma unroll
-      for (int g = 0; g < MG; ++g) a[g] = wv[base + c0 + g * 16];
+        for (int g = 0; g < MG; ++g) a[g] = wv[eb[(g & 1) ^ (cpt & 1)][j] + c0 + (g & ~1) * 16];
+      } else {
+        const int base = th >= 4 ? b0 : th == 3 ? b3 : th == 2 ? b2 : b1;
+#pragma unroll
+        for (int g = 0; g < MG; ++g) a[g] = wv[base + c0 + g * 16];
+      }
     };
     const T* rpos[PPW];  // R operands of each position (past n: the last position's)
 #pragma unroll
@@ -649,16 +677,19 @@ __global__ __launch_bounds__(64 * NW) void fham_radial_win_kernel(int nc, int n,
 // 0 is the one-term, one-group kernel; variants with kRadPosMG > 0 are the position form
 // with that many channel groups per wave (only with separable terms split off, which fixes
 // the set of terms its tiles hold)
-constexpr int kRadTG[kFhamRadialVariants] = {1, 2, 3, 3, 2, 3, 3, 1, 1, 1, 1, 1, 1, 1};
-constexpr int kRadGG[kFhamRadialVariants] = {1, 2, 2, 3, 2, 2, 3, 1, 1, 1, 1, 1, 1, 1};
+constexpr int kRadTG[kFhamRadialVariants] = {1, 2, 3, 3, 2, 3, 3, 1, 1, 1, 1, 1, 1, 1, 1};
+constexpr int kRadGG[kFhamRadialVariants] = {1, 2, 2, 3, 2, 2, 3, 1, 1, 1, 1, 1, 1, 1, 1};
 constexpr bool kRadPipe[kFhamRadialVariants] = {false, false, false, false, true, true, true,
-                                                false, false, false, false, false, false, false};
-constexpr int kRadPosMG[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 6, 3, 2, 3, 2, 2, 1};
+                                                false, false, false, false, false, false, false, false};
+constexpr int kRadPosMG[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 6, 3, 2, 3, 2, 2, 1, 2};
 // position variants with the window in LDS (fham_radial_win_kernel): positions per workgroup
 // and waves per workgroup (eight waves of two positions each measured no faster at helium:
 // more waves per SIMD do not shorten it)
-constexpr int kRadWinTW[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 16, 16, 8, 16};
-constexpr int kRadWinNW[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 4, 4, 4, 4};
+constexpr int kRadWinTW[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 16, 16, 8, 16, 16};
+constexpr int kRadWinNW[kFhamRadialVariants] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 4, 4, 4, 4, 4};
+// the swizzled (unpadded) window, FP32 only: 54 KB at helium, three workgroups per CU
+constexpr bool kRadWinSW[kFhamRadialVariants] = {false, false, false, false, false, false, false, false,
+                                                 false, false, false, false, false, false, true};
 // angular variants: waves splitting K, and whether one workgroup covers all row blocks of
 // its position tile (reading each Z element once) or one row block
 constexpr int kAngWaveChoices[kFhamAngularVariants] = {16, 8, 4, 8, 16, 4, 4};
@@ -673,21 +704,26 @@ static void launch_radial(int variant, int nc, int ncp, int n, int nterms, int n
   if (kRadWinTW[variant] > 0) {
     if (!RP) throw Error(PP_ERR_INVALID, "position-form radial variant without position tiles");
     const int MG = kRadPosMG[variant], TW = kRadWinTW[variant], NW = kRadWinNW[variant];
+    const bool SWv = kRadWinSW[variant];
+    if (SWv && sizeof(T) != 4) throw Error(PP_ERR_INVALID, "the swizzled radial window is FP32 only");
     const int ntiles = (n + TW - 1) / TW;
     const long long nblocks = (long long)((nterms + 15) / 16) * n * ntiles * ((ngroups + MG - 1) / MG);
     const dim3 grid((unsigned)((nblocks + 7) / 8 * 8));
-#define PP_RADW(G, W, V)                                                                                          \
-  if (MG == G && TW == W && NW == V) {                                                                            \
-    constexpr size_t lds = win_lds_bytes<T, B, G, W, V>();                                                        \
-    if (lds > 160 * 1024) throw Error(PP_ERR_INVALID, "radial window does not fit in LDS");                       \
-    PP_HIP(hipFuncSetAttribute((const void*)fham_radial_win_kernel<T, B, G, W, V>,                                \
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                            \
-    hipLaunchKernelGGL((fham_radial_win_kernel<T, B, G, W, V>), grid, dim3(64 * V), lds, s, nc, n, nterms,        \
-                       ngroups, ntiles, nblocks, ncp, Xt, RP, Z, differs);                                        \
-    PP_HIP(hipGetLastError());                                                                                    \
-    return;                                                                                                       \
+#define PP_RADW(G, W, V, S)                                                                                       \
+  if constexpr (!S || sizeof(T) == 4) {                                                                           \
+    if (MG == G && TW == W && NW == V && SWv == S) {                                                              \
+      constexpr size_t lds = win_lds_bytes<T, B, G, W, V, S>();                                                   \
+      if (lds > 160 * 1024) throw Error(PP_ERR_INVALID, "radial window does not fit in LDS");                     \
+      PP_HIP(hipFuncSetAttribute((const void*)fham_radial_win_kernel<T, B, G, W, V, S>,                           \
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                          \
+      hipLaunchKernelGGL((fham_radial_win_kernel<T, B, G, W, V, S>), grid, dim3(64 * V), lds, s, nc, n, nterms,   \
+                         ngroups, ntiles, nblocks, ncp, Xt, RP, Z, differs);                                      \
+      PP_HIP(hipGetLastError());                                                                                  \
+      return;                                                                                                     \
+    }                                                                                                             \
   }
-    PP_RADW(3, 16, 4) PP_RADW(2, 16, 4) PP_RADW(2, 8, 4) PP_RADW(1, 16, 4)
+    PP_RADW(3, 16, 4, false) PP_RADW(2, 16, 4, false) PP_RADW(2, 8, 4, false) PP_RADW(1, 16, 4, false)
+    PP_RADW(2, 16, 4, true)
 #undef PP_RADW
     throw Error(PP_ERR_INVALID, "radial window shape not instantiated");
   }
@@ -1683,10 +1719,13 @@ extern "C" int pp_fham_multiply_f32(const pp_fham* f, const double* scalings, co
 }
 
 // Whether radial variant v can run for this operator in the given precision: the position
-// forms need the separable split; the LDS-window forms a window that fits in 160 KiB.
+// forms need the separable split; the LDS-window forms a window that fits in 160 KiB (the
+// swizzled window: FP32 only).
 static bool radial_window_fits(const FHam& F, int v, int precision) {
   if (kRadWinTW[v] == 0) return true;
-  return win_lds_bytes(F.bw, kRadPosMG[v], kRadWinTW[v], kRadWinNW[v], precision == 64 ? 8 : 4) <= 160 * 1024;
+  if (kRadWinSW[v] && precision == 64) return false;  // the swizzled window is FP32 only
+  return win_lds_bytes(F.bw, kRadPosMG[v], kRadWinTW[v], kRadWinNW[v], precision == 64 ? 8 : 4, kRadWinSW[v]) <=
+         160 * 1024;
 }
 
 extern "C" int pp_fham_set_radial_variant(pp_fham* f, int precision, int variant) {
@@ -1695,6 +1734,7 @@ extern "C" int pp_fham_set_radial_variant(pp_fham* f, int precision, int variant
     PP_REQUIRE(variant >= 0 && variant < kFhamRadialVariants, "radial variant out of range");
     PP_REQUIRE(kRadPosMG[variant] == 0 || f->f.sep_enabled,
                "position-form radial variants need the separable split (n <= 64)");
+    PP_REQUIRE(!(kRadWinSW[variant] && precision == 64), "the swizzled radial window is FP32 only");
     PP_REQUIRE(radial_window_fits(f->f, variant, precision), "radial window does not fit in LDS");
     (precision == 64 ? f->f.rad_variant64 : f->f.rad_variant32) = variant;
   });

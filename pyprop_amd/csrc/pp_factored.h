@@ -10,7 +10,7 @@
 namespace pp {
 
 constexpr int kMaxFTerms = 32;
-constexpr int kFhamRadialVariants = 14;  // radial-stage kernel shapes (band-tile and position forms)
+constexpr int kFhamRadialVariants = 15;  // radial-stage kernel shapes (band-tile and position forms)
 constexpr int kFhamAngularVariants = 7;  // angular-stage forms (GEMM: waves splitting K, row blocks per workgroup; sparse)
 
 struct FHamOut {

@@ -122,10 +122,12 @@ class FactoredHamiltonian:
     # radial variants: "<terms>x<channel groups>" per wave on band tiles ("p": next band step
     # prefetched), "pos<groups>": each position's products of all band terms as one GEMM over
     # the w x w window, "pos<groups>w<positions>": the same with the window of a workgroup's
-    # positions (consecutive along r2) copied into LDS once (the position forms need the
-    # separable split; tune reports -1 ms for a variant that cannot run)
+    # positions (consecutive along r2) copied into LDS once, "...s" the same with the window
+    # swizzled instead of padded (FP32 only; a smaller window, three workgroups per CU) (the
+    # position forms need the separable split; tune reports -1 ms for a variant that cannot
+    # run)
     RADIAL_SHAPES = ("1x1", "2x2", "3x2", "3x3", "2x2p", "3x2p", "3x3p", "pos6", "pos3", "pos2",
-                     "pos3w16", "pos2w16", "pos2w8", "pos1w16")
+                     "pos3w16", "pos2w16", "pos2w8", "pos1w16", "pos2w16s")
 
     def set_radial_variant(self, precision: int, variant: int):
         """Radial-stage kernel shape for the complex128 (precision 64) or complex64 (32)

@@ -33,7 +33,7 @@ class _Helium:
 @pytest.mark.parametrize("radial, counted", [("2x2", 1487808), ("3x2", 1487808), ("3x3p", 1487808),
                                              ("pos6", 928800), ("pos3", 928800), ("pos2", 928800),
                                              ("pos3w16", 928800), ("pos2w16", 928800), ("pos2w8", 928800),
-                                             ("pos1w16", 928800)])
+                                             ("pos1w16", 928800), ("pos2w16s", 928800)])
 def test_issued_mfma_matches_hardware_counts(radial, counted, angular):
     issued, useful = bench.factored_flops(_Helium(radial, angular), 42, 60, 64)
     assert issued == (counted + (0 if angular.startswith("sparse") else 259200)) * 2048
