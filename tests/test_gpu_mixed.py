@@ -413,17 +413,24 @@ def test_mixed_cayley_fallback_to_plain(torch, rng):
     _check_solution(psi.numpy(), sol, A, b, 1e-11, 1e-8)
 
 
-@pytest.mark.parametrize("size", ["small", "helium"])
+@pytest.mark.parametrize("size", ["small", "small_even", "helium"])
 def test_learned_apply_is_kron_plus_network(torch, rng, size):
-    """The preconditioner's inference (the streaming kernels: one 16-column strip per wave
-    over all channel rows) against the training forward pass (tiled GEMM) fed the same
-    BF16 inputs, then the Kronecker solve; small helium (15 channels, hidden 64) and the
-    helium workload's shape (42 channels, 60 x 60 blocks, hidden 512)."""
+    """The preconditioner's inference against the training forward pass (tiled GEMM) fed
+    the same BF16 inputs, then the Kronecker solve.  small: 15 channels of 13 x 13 (2 n^2 =
+    338 inputs, not a multiple of 8: the generic path, block norms and the training GEMMs),
+    hidden 64; small_even: 15 channels of 12 x 12 (288 inputs) and hidden 72, the streaming
+    kernels with a partial 16-column strip, a partial K slice and waves without K steps;
+    helium: the workload's shape (42 channels, 60 x 60 blocks, hidden 512), streaming."""
     from pyprop_amd.mixed import KroneckerPreconditioner, LearnedPreconditioner
     from pyprop_amd.systems import helium_system
     say("learned apply = K^-1 (v + sigma |v| net(v_c/|v|)), %s" % size)
-    sysm = _small_helium() if size == "small" else helium_system()
-    hidden = 64 if size == "small" else 512
+    if size == "small":
+        sysm, hidden = _small_helium(), 64
+    elif size == "small_even":
+        sysm = helium_system(lmax=2, L=(0, 1, 2), order=5, n_breakpoints=11, rmax=15.0, n_inner=6, gamma=2.5)
+        hidden = 72
+    else:
+        sysm, hidden = helium_system(), 512
     repr_, psi, base = _device(sysm, torch)
     kp = KroneckerPreconditioner(psi, sysm)
     lp = LearnedPreconditioner(kp, hidden=hidden, seed=3)
