@@ -414,14 +414,15 @@ void Learned::adam(float lr, float b1, float b2, float eps, hipStream_t s) {
 // ------------------------------------------------------------------------------------
 // The training GEMM tiles 64 x 64 outputs and splits K; with a few dozen rows most of its
 // tile is padding and each weight tile is loaded for one row block.  Inference instead
-// gives each wave a 16-column strip of the output and all rows (three 16-row MFMA tiles),
-// so every weight element is read from HBM once and used for all rows.  The inputs of a
-// workgroup's K range are staged once in LDS, as BF16, for its four waves.  Rounding
-// points are those of the training forward pass: x = bf16(v_c / |v|), h1 and h2 in BF16,
-// y in FP32.
+// covers all rows (three 16-row MFMA tiles) in every wave, so every weight element is read
+// from HBM once and used for all rows: the first layer gives each wave a 16-column strip of
+// a 256-wide K slice (the normalised input staged in LDS as BF16), the two dense layers a
+// quarter of K of a workgroup's strips, with the operands loaded straight into registers.
+// Rounding points are those of the training forward pass: x = bf16(v_c / |v|), h1 and h2
+// in BF16, y in FP32.
 constexpr int IMR = 48;        // rows covered (3 x 16)
 constexpr int IKC = 256;       // K range of one first-layer workgroup
-constexpr int IKMAX = 640;     // K of the hidden layers staged whole in LDS
+constexpr int IKMAX = 640;     // K of the dense layers (hidden width) the streaming path takes
 
 union BfPack {  // eight BF16 values as one MFMA operand
   bf16x8 v;
