@@ -1009,7 +1009,8 @@ __global__ __launch_bounds__(64 * kAngSparseWaves) void fham_angular_sparse_kern
     FScales<T> sc, const double* __restrict__ scd, const int* __restrict__ cpot,
     const typename MfmaT<T>::C2* addend, typename MfmaT<T>::C2 alpha,
     const typename MfmaT<T>::C2* __restrict__ zs, typename MfmaT<T>::C2 beta, typename MfmaT<T>::C2* out,
-    typename MfmaT<T>::C2* out2, typename MfmaT<T>::C2 alpha2, typename MfmaT<T>::C2 beta2, int* __restrict__ reset) {
+    typename MfmaT<T>::C2* out2, typename MfmaT<T>::C2 alpha2, typename MfmaT<T>::C2 beta2, int* __restrict__ reset,
+    double* __restrict__ npart) {
   typedef typename MfmaT<T>::C2 C2;
   __shared__ T part_re[kAngSparseWaves - 1][PV][64], part_im[kAngSparseWaves - 1][PV][64];
   const long long nblocks = (long long)nc * nchunks;
@@ -1077,6 +1078,7 @@ __global__ __launch_bounds__(64 * kAngSparseWaves) void fham_angular_sparse_kern
       re[r] += part_re[q][r][lane];
       im[r] += part_im[q][r][lane];
     }
+  double n1 = 0, n2 = 0;  // squared norms of the block's outputs (npart)
 #pragma unroll
   for (int r = 0; r < PV; ++r) {
     if (!ok[r]) continue;
@@ -1097,11 +1099,23 @@ __global__ __launch_bounds__(64 * kAngSparseWaves) void fham_angular_sparse_kern
     o.x += beta.x * v.x - beta.y * v.y;
     o.y += beta.x * v.y + beta.y * v.x;
     out[idx] = o;
+    n1 += (double)o.x * o.x + (double)o.y * o.y;
     if (out2) {
       C2 o2;
       o2.x = alpha2.x * vr - alpha2.y * vi + beta2.x * v.x - beta2.y * v.y;
       o2.y = alpha2.x * vi + alpha2.y * vr + beta2.x * v.y + beta2.y * v.x;
       out2[idx] = o2;
+      n2 += (double)o2.x * o2.x + (double)o2.y * o2.y;
+    }
+  }
+  if (npart) {  // [L][0] = |out|^2, [L][1] = |out2|^2 over the block, a shuffle tree in lane order
+    for (int off = 32; off > 0; off >>= 1) {
+      n1 += __shfl_down(n1, off, 64);
+      n2 += __shfl_down(n2, off, 64);
+    }
+    if (lane == 0) {
+      npart[2 * L] = n1;
+      npart[2 * L + 1] = n2;
     }
   }
 }
@@ -1286,7 +1300,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
                      const FOut<T>* o2, const DeviceBuffer<T>& Ab, const DeviceBuffer<T>& Db,
                      const DeviceBuffer<T>& RPb, const DeviceBuffer<T>& SAb, const DeviceBuffer<T>& SBb, DeviceBuffer<T>& Zb,
                      DeviceBuffer<T>& Xb, const T* ref, int* differs, hipStream_t s, const double* sc_dev,
-                     const typename MfmaT<T>::C2* zs_ext = nullptr) {
+                     const typename MfmaT<T>::C2* zs_ext = nullptr, double* npart = nullptr) {
   const int nc = F.nc, n = F.n, nterms = F.nterms;
   const long long P = (long long)n * n;
   auto nz = [](double2 v) { return v.x != 0.0 || v.y != 0.0; };
@@ -1371,6 +1385,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
   const C2* zsp = zs_ext ? zs_ext : with_s ? Zc + (size_t)nterms * nc * P : nullptr;  // the overlap product
   const int avar = sizeof(T) == 8 ? F.ang_variant64 : F.ang_variant32;
   const int waves = kAngWaveChoices[avar];
+  PP_REQUIRE(!npart || kAngSparsePV[avar] > 0, "output norm partials need the sparse angular form");
   if (kAngSparsePV[avar] > 0) {
     const int PV = kAngSparsePV[avar];
     const int nchunks = (int)ceil_div(P, 64LL * PV);
@@ -1384,7 +1399,7 @@ static void fham_run(const FHam& F, const double* scalings, const typename MfmaT
                      to_c2<T>(o1.alpha),                                                                             \
                      zsp, to_c2<T>(o1.beta), o1.dst,                     \
                      o2 ? o2->dst : nullptr, to_c2<T>(o2 ? o2->alpha : make_double2(0, 0)),                          \
-                     to_c2<T>(o2 ? o2->beta : make_double2(0, 0)), differs)
+                     to_c2<T>(o2 ? o2->beta : make_double2(0, 0)), differs, npart)
     if (PV == 1) PP_ANGS(1);
     else PP_ANGS(2);
 #undef PP_ANGS
@@ -1450,7 +1465,7 @@ void FHam::apply64(const double* scalings, const double2* src, double2* dst, con
 }
 
 void FHam::apply64_ex(const double* scalings, const double2* src, const FHamOut& o1, const FHamOut* o2,
-                      const double2* reuse_ref, hipStream_t s, const double* sc_dev) const {
+                      const double2* reuse_ref, hipStream_t s, const double* sc_dev, double* npart) const {
   auto nz = [](double2 v) { return v.x != 0.0 || v.y != 0.0; };
   const int nrad = nterms + ((nz(o1.beta) || (o2 && nz(o2->beta))) ? 1 : 0);
   PP_REQUIRE(!reuse_ref || w64_terms >= nrad, "held radial products do not cover the requested terms");
@@ -1471,7 +1486,8 @@ void FHam::apply64_ex(const double* scalings, const double2* src, const FHamOut&
     a2.beta = o2->beta;
   }
   fham_run<double>(*this, scalings, src, a1, o2 ? &a2 : nullptr, A64, bt64, rp64, sa64, sb64, W64, X64,
-                   reinterpret_cast<const double*>(reuse_ref), reuse_ref ? reuse_flag.ptr : nullptr, s, sc_dev);
+                   reinterpret_cast<const double*>(reuse_ref), reuse_ref ? reuse_flag.ptr : nullptr, s, sc_dev,
+                   nullptr, npart);
   // when the products were reused they cover at least nrad terms; when recomputed, exactly nrad
   w64_terms = nrad;
 }
@@ -1488,6 +1504,11 @@ void FHam::apply32(const double* scalings, const float2* src, float2* dst, const
 }
 
 bool FHam::sparse_angular() const { return kAngSparsePV[ang_variant64] > 0 && kAngSparsePV[ang_variant32] > 0; }
+
+long long FHam::norm_blocks64() const {
+  const int pv = kAngSparsePV[ang_variant64];
+  return pv > 0 ? (long long)nc * ceil_div((long long)n * n, 64LL * pv) : 0;
+}
 
 }  // namespace pp
 

@@ -289,6 +289,43 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(long long n, cons
   if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sh[0] + sh[1] + sh[2] + sh[3], 0.0);
 }
 
+// out[j] = (sum_b part[2 b + j], 0) for j < count (<= 2), one block: every thread sums its
+// blocks (all loads issued first) in order, then a shuffle tree and the four waves in order;
+// out2 (device memory, may be NULL) receives the same values
+__global__ __launch_bounds__(256) void finish_norms_kernel(int blocks, const double2* __restrict__ part, int count,
+                                                           double2* __restrict__ out, double2* __restrict__ out2) {
+  constexpr int U = 8;
+  __shared__ double sh[2][4];
+  double a0 = 0, a1 = 0;
+  for (int b0 = threadIdx.x; b0 < blocks; b0 += 256 * U) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = part[min(b0 + 256 * u, blocks - 1)];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double k = b0 + 256 * u < blocks ? 1.0 : 0.0;
+      a0 += k * v[u].x;
+      a1 += k * v[u].y;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a0 += __shfl_down(a0, off, 64);
+    a1 += __shfl_down(a1, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a0;
+    sh[1][threadIdx.x >> 6] = a1;
+  }
+  __syncthreads();
+  if (threadIdx.x < count) {
+    const int j = threadIdx.x;
+    const double2 r = make_double2(((sh[j][0] + sh[j][1]) + sh[j][2]) + sh[j][3], 0.0);
+    out[j] = r;
+    if (out2) out2[j] = r;
+  }
+}
+
 // blocks of the fused reductions: about 1024 elements each
 static int fused_blocks(long long n) {
   long long b = (n + 1023) / 1024;
@@ -1633,6 +1670,13 @@ void MixedWork::norm2_32(const float2* w_, double2* out, hipStream_t s) {
   PP_HIP(hipGetLastError());
 }
 
+void MixedWork::finish_norms(long long blocks, int count, double2* out, double2* out_dev, hipStream_t s) {
+  PP_REQUIRE(blocks >= 1 && blocks < (1LL << 31) && count >= 1 && count <= 2, "invalid norm partials");
+  hipLaunchKernelGGL(finish_norms_kernel, dim3(1), dim3(256), 0, s, (int)blocks,
+                     reinterpret_cast<const double2*>(npart.ptr), count, out, out_dev);
+  PP_HIP(hipGetLastError());
+}
+
 void MixedWork::residual(const double2* b64_, double2* r, double2* out, double2* out_dev, hipStream_t s) {
   const int blocks = fused_blocks(n);
   red.ensure(blocks, 1);
@@ -1787,6 +1831,9 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
   const double2 ka = make_double2(0.0, a);
   // with an overlap term in the factored operator, the FP64 S + c H is one multiply
   const bool fused_s = fham && fham->has_overlap;
+  // FP64 sparse angular form: its epilogue writes the block partials of the outputs' norms
+  const long long nblk64 = fused_s ? fham->norm_blocks64() : 0;
+  if (nblk64 > 0 && work.npart.count < (size_t)(2 * nblk64)) work.npart.resize((size_t)(2 * nblk64));
   // In graph mode the step's time scalings also go to device memory (through pinned host
   // memory, in stream order): the sparse angular form reads them there, so that a captured
   // step replays with each step's values.  Eager steps pass them as kernel arguments (the
@@ -1819,7 +1866,15 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
       orr.dst = work.r64.ptr;
       orr.alpha = make_double2(0.0, -2.0 * a);
       orr.beta = make_double2(0.0, 0.0);
-      fham->apply64_ex(scal.data(), psi, ob, &orr, reuse ? work.x64.ptr : nullptr, cs, scd);
+      // with the sparse angular form its epilogue also writes the block partials of |b|^2
+      // and |r|^2, so no separate norm pass reads the two vectors again
+      fham->apply64_ex(scal.data(), psi, ob, &orr, reuse ? work.x64.ptr : nullptr, cs, scd,
+                       nblk64 > 0 ? work.npart.ptr : nullptr);
+      if (nblk64 > 0) {
+        ++stats.outer_ops;
+        work.finish_norms(nblk64, 2, work.small_dev + kB, work.norms.ptr, cs);
+        return;
+      }
     } else {
       vec_copy(n, psi, work.t64.ptr, cs);
       R.overlap(PP_OVERLAP_MULTIPLY, false, work.t64.ptr, cs);
@@ -1902,6 +1957,24 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     }
     ++stats.outer_ops;
   };
+  // r = b - A x with |r|^2 finished in the same launches: the FP64 angular epilogue forms
+  // b - (S x + i a H x) and its block partials (sparse angular form), else the multiply and a
+  // residual pass
+  auto residual_pass = [&](hipStream_t st) {
+    if (fused_s && nblk64 > 0) {
+      FHamOut o;
+      o.dst = work.r64.ptr;
+      o.addend = work.b64.ptr;
+      o.alpha = make_double2(0.0, -a);
+      o.beta = make_double2(-1.0, 0.0);
+      fham->apply64_ex(scal.data(), work.x64.ptr, o, nullptr, nullptr, cs, scd, work.npart.ptr);
+      ++stats.outer_ops;
+      work.finish_norms(nblk64, 1, work.small_dev + kR, work.norms.ptr + 2, st);
+    } else {
+      residual_multiply();
+      work.residual(work.b64.ptr, work.r64.ptr, work.small_dev + kR, work.norms.ptr + 2, st);
+    }
+  };
   double bnorm = -1.0, rn = -1.0;
   // a residual norm written by a kernel and not yet read: its index in the pinned block and
   // its device copy
@@ -1934,8 +2007,7 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
       stats.inner_iterations += pre_its[c];
       x_is_psi = false;
       ++correction;
-      residual_multiply();
-      work.residual(work.b64.ptr, work.r64.ptr, work.small_dev + kR, work.norms.ptr + 2, cs);
+      residual_pass(cs);
       pending = kR;
       pending_dev = work.norms.ptr + 2;
     }
@@ -2055,8 +2127,7 @@ void MixedCayley::step(double2* psi, double dt, const double* sc, hipStream_t s)
     stats.inner_iterations += its;
     x_is_psi = false;
     ++correction;
-    residual_multiply();
-    work.residual(work.b64.ptr, work.r64.ptr, work.small_dev + kR, work.norms.ptr + 2, s);
+    residual_pass(s);
     pending = kR;
     pending_dev = work.norms.ptr + 2;
     predicted = (bnorm > 0 ? rn / bnorm : 0.0) * rel;

@@ -79,8 +79,13 @@ struct FHam {
   // addend).  reuse_ref (may be NULL): the input of an earlier FP64 multiply whose radial
   // products W64 still holds (covering the terms needed now); the radial stage is skipped when
   // src equals it element for element, checked on the device.
+  // npart (may be NULL; FP64 sparse angular form only): per angular workgroup L, npart[2 L]
+  // = |o1.dst|^2 and npart[2 L + 1] = |o2->dst|^2 over the block's elements, norm_blocks64()
+  // workgroups (the outputs' squared norms without a separate pass)
   void apply64_ex(const double* scalings, const double2* src, const FHamOut& o1, const FHamOut* o2,
-                  const double2* reuse_ref, hipStream_t s, const double* sc_dev = nullptr) const;
+                  const double2* reuse_ref, hipStream_t s, const double* sc_dev = nullptr,
+                  double* npart = nullptr) const;
+  long long norm_blocks64() const;  // 0 when the FP64 angular form is not sparse
 };
 
 

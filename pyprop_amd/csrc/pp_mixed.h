@@ -96,6 +96,7 @@ struct MixedWork {
   void ensure(long long n, int m);
   void read_small(int count, hipStream_t s);  // synchronise s, copy small[0..count) to host_small
   DeviceBuffer<double2> norms;   // device copies of step-level squared norms (|b|^2, |r0|^2, |r|^2)
+  DeviceBuffer<double> npart;    // per-workgroup squared norms from the FP64 angular epilogue
   DeviceBuffer<double2> hdev;    // device copy of the inner solve's results (the Arnoldi scaling)
   DeviceBuffer<float2> ycoef;    // inner least-squares solution of the device mode
   // out[0] = |a|^2, out[1] = |b|^2 (pinned), the same into out_dev (device)
@@ -116,6 +117,9 @@ struct MixedWork {
   void norm2_32(const float2* w, double2* out, hipStream_t s);  // out[0] = |w|^2
   // r = b - r, out[0] = |r|^2 (FP64; pinned out and device out_dev)
   void residual(const double2* b, double2* r, double2* out, double2* out_dev, hipStream_t s);
+  // out[j] (and out_dev[j]) = (sum over blocks of npart[2 b + j], 0), j < count (<= 2): the
+  // squared norms whose block partials the FP64 angular epilogue wrote
+  void finish_norms(long long blocks, int count, double2* out, double2* out_dev, hipStream_t s);
   bool ss_one_wg = false;  // S x S with one workgroup per channel (else a block of r1 rows each)
   // out = S x S in (FP32, out-of-place; scratch has n elements)
   void overlap_multiply32(const Repr& r, const float2* in, float2* out, float2* scratch, hipStream_t s);
