@@ -324,6 +324,50 @@ def test_mixed_overlap_from_solve_vs_cholesky(torch, rng, learned):
         assert d < 1e-9
 
 
+def test_mixed_residual_norms_from_multiply(torch, rng):
+    """Mixed steps whose FP64 residuals and norms come from the sparse angular epilogue
+    (r = b - A x and the block partials of |r|^2, |b|^2 in the multiply) against the same
+    steps with a GEMM angular form (the multiply, then the residual and norm passes): the
+    same outer and inner counts, both within 1e-8 of the dense FP64 solution, and the two
+    states within 1e-9 of each other, host-driven and in device mode (small helium: 15
+    channels, 13 x 13, partial position chunks)."""
+    from pyprop_amd.factored import FactoredHamiltonian
+    from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
+    say("mixed step: residual norms from the FP64 angular epilogue vs separate passes (small helium)")
+    sysm = _small_helium()
+    repr_, psi, base = _device(sysm, torch)
+    x0 = crandn(rng, sysm.shape)
+    fh = FactoredHamiltonian(base, overlap=True)
+    kp = KroneckerPreconditioner(psi, sysm)
+    sparse = fh.ANGULAR_VARIANTS.index("sparse2")
+    for device_mode in (False, True):
+        res = {}
+        for name, var in (("epilogue", sparse), ("passes", 0)):
+            fh.set_angular_variant(64, var)
+            psi.data.copy_(torch.from_numpy(x0).to("cuda"))
+            prop = MixedCayleyPropagator(psi, base, kp, tolerance=1e-11, factored=fh, device_mode=device_mode)
+            dt, t = 0.01, 40.0
+            cur = x0
+            stats = []
+            for k in range(4):
+                sc = base.scalings(t + dt / 2)
+                sol, A, b = _dense_cayley(sysm, cur, dt, sc, True)
+                prop.AdvanceStep(t, dt)
+                st = prop.GetStats()
+                stats.append((st["outer"], st["inner"]))
+                assert st["rel_residual"] <= 1e-11 and st["fallback"] == 0
+                cur = psi.numpy()
+                _check_solution(cur, sol, A, b, 2e-11, 1e-8)
+                t += dt
+            res[name] = (cur, stats)
+            del prop
+        d = np.linalg.norm(res["epilogue"][0] - res["passes"][0]) / np.linalg.norm(res["passes"][0])
+        say("  device mode %d: counts %s vs %s, states differ by %.2e" % (device_mode, res["epilogue"][1],
+                                                                          res["passes"][1], d))
+        assert res["epilogue"][1] == res["passes"][1]
+        assert d < 1e-9
+
+
 def _dense_cayley(sysm, x, dt, sc, with_system=False):
     from tests.test_oracle import dense_operator
     H, S = dense_operator(sysm, sc)
