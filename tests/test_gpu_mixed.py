@@ -747,29 +747,45 @@ def test_factored_fp32_overlap_cholesky_form(torch, rng):
     assert e < 2e-6
 
 
-def test_mixed_graph_mode_equals_eager_device_mode(torch, rng):
+@pytest.mark.parametrize("variant", ["kron", "overlap_learned"])
+def test_mixed_graph_mode_equals_eager_device_mode(torch, rng, variant):
     """Graph mode: the device-mode step captured into a hipGraph and replayed (with the
     program's own synchronisation and eager copy between replays, and the time scalings of
     each step staged in device memory) gives, step after step, bitwise the states of eager
     device mode; the sequence is captured after it has run eagerly and replayed afterwards
-    (recaptured when the counts change)."""
+    (recaptured when the counts change).  overlap_learned: the operator with its overlap
+    term (the FP64 residual and norms from the angular epilogue, the overlap product from
+    the solves) and the learned correction on the first direction (a network with random
+    output weights, on the streaming inference kernels: 15 channels of 12 x 12)."""
     from pyprop_amd.factored import FactoredHamiltonian
-    from pyprop_amd.mixed import KroneckerPreconditioner, MixedCayleyPropagator
-    say("mixed step: graph mode vs eager device mode (small helium)")
-    sysm = _small_helium()
+    from pyprop_amd.mixed import KroneckerPreconditioner, LearnedPreconditioner, MixedCayleyPropagator
+    from pyprop_amd.systems import helium_system
+    say("mixed step: graph mode vs eager device mode (small helium, %s)" % variant)
+    learned = variant == "overlap_learned"
+    sysm = (helium_system(lmax=2, L=(0, 1, 2), order=5, n_breakpoints=11, rmax=15.0, n_inner=6, gamma=2.5)
+            if learned else _small_helium())
     repr_, psi, base = _device(sysm, torch)
     base.SetupStep()
-    fh = FactoredHamiltonian(base)
+    fh = FactoredHamiltonian(base, overlap=learned)
     sparse = fh.ANGULAR_VARIANTS.index("sparse2")
     fh.set_angular_variant(64, sparse)
     fh.set_angular_variant(32, sparse)
     x0 = torch.from_numpy(crandn(rng, sysm.shape)).to("cuda")
     kp = KroneckerPreconditioner(psi, sysm)
+    lp = None
+    if learned:
+        nc, n, _ = sysm.shape
+        lp = LearnedPreconditioner(kp, hidden=72, seed=9)
+        flat = lp.get_params().astype(np.float64)
+        o, shp = mlp.layout(2 * n * n, 72, nc)[0]["W3"]
+        flat[o:o + int(np.prod(shp))] = rng.standard_normal(int(np.prod(shp))) * 0.1
+        lp.set_params(flat.astype(np.float32))
+        lp.set_scale(0.05)
     dt, t0, steps = 0.01, 120.0, 12
     traj = {}
     for graph in (False, True):
         psi.data.copy_(x0)
-        prop = MixedCayleyPropagator(psi, base, kp, tolerance=1e-10, factored=fh, device_mode=True, graph=graph)
+        prop = MixedCayleyPropagator(psi, base, kp, lp, tolerance=1e-10, factored=fh, device_mode=True, graph=graph)
         t = t0
         states = []
         for _ in range(steps):
@@ -784,6 +800,7 @@ def test_mixed_graph_mode_equals_eager_device_mode(torch, rng):
     for k in range(steps):
         assert torch.equal(eager[k], graphed[k]), k
     assert se["inner"] == sg["inner"] and se["fp32_ops"] == sg["fp32_ops"]
+    assert (se["learned"] > 0) == learned and se["learned"] == sg["learned"]
 
 
 def test_mixed_deferred_finish_bitwise(torch, rng):
